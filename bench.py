@@ -1,0 +1,288 @@
+"""Benchmark: central-slice extraction + adjoint (fwd + bwd) at 256^3 complex64.
+
+One step = extract 4096 central slices from a (256, 256, 129) complex64 rfft
+volume (the gather) and apply the adjoint to 4096 slices of N(0,1) complex64
+gradients (the volume gradient, i.e. the backward of the extraction), exactly
+as ``extract_central_slices_rfft_3d(...).backward(grad)`` runs it: zero the
+gradient volume, gather kernel, scatter kernel.  Inputs are resident in HBM
+before timing; synthetic data (seeded blob volume, uniform SO(3) rotations).
+
+Prints ONE JSON line.  Besides the contract fields it reports the roofline of
+the dominant kernel (algorithmic bytes / HIP-event duration), the fused step's
+roofline, the plain-torch restatement of the reference on the same GPU, and
+the FP64 oracle timed on the host CPU on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-torch] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 256
+N_ROT = 4096
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-torch", action="store_true", help="skip the plain-torch comparator")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--deterministic", action="store_true")
+    return ap.parse_args()
+
+
+def make_inputs(device):
+    from teamtomo_amd.synthetic import blob_volume, uniform_rotations, volume_rfft
+
+    vol = blob_volume(N, 200, seed=2, device=device)
+    V = volume_rfft(vol).contiguous()
+    R = uniform_rotations(N_ROT, seed=0, device=device)
+    g = torch.Generator(device=device).manual_seed(3)
+    gS = torch.complex(
+        torch.randn((N_ROT, N, N // 2 + 1), generator=g, device=device),
+        torch.randn((N_ROT, N, N // 2 + 1), generator=g, device=device),
+    ).contiguous()
+    return V, R, gS
+
+
+class Step:
+    """extract + adjoint through the C ABI, with HIP events around each kernel."""
+
+    def __init__(self, V, R, gS, deterministic=False):
+        from teamtomo_amd import _lib
+        from teamtomo_amd.fourier_slice import _prepare_rotations
+
+        self.lib = _lib
+        self.V, self.gS = V, gS
+        self.Rz = _prepare_rotations(R, V.device, False, 1e-8).reshape(-1, 3, 3).contiguous()
+        self.n = self.Rz.shape[0]
+        self.out = torch.empty((self.n, N, N // 2 + 1), dtype=torch.complex64, device=V.device)
+        self.gV = torch.empty_like(V)
+        self.det = int(deterministic)
+        self.stream = _lib.stream_of(V)
+        self.events = []
+
+    def __call__(self, timed=False):
+        L, V = self.lib, self.V
+        s = torch.cuda.current_stream()
+        if timed:
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record(s)
+        L.call("tt_extract_c64", L.ptr(V), N, N, N // 2 + 1, N, N, N, L.ptr(self.Rz), self.n, -1.0, 1e-8,
+               L.ptr(self.out), self.stream)
+        if timed:
+            e[1].record(s)
+        self.gV.zero_()
+        L.call("tt_extract_bwd_c64", L.ptr(self.gS), L.ptr(V), N, N, N // 2 + 1, N, N, N, L.ptr(self.Rz),
+               self.n, -1.0, 1e-8, L.ptr(self.gV), None, self.det, self.stream)
+        if timed:
+            e[2].record(s)
+            self.events.append(e)
+
+
+def algorithmic_bytes():
+    P = N * (N // 2 + 1)
+    V = N * N * (N // 2 + 1)
+    extract = N_ROT * (8 * P + 36) + 8 * V
+    adjoint = N_ROT * (8 * P + 36) + 8 * V
+    return extract, adjoint
+
+
+def _torch_coords(image_shape, R, device):
+    """Reference op sequence up to the sampling coordinates
+    (slice_extraction/_extract_central_slices_rfft_3d.py:147-210, _dft_utils.py:5-72)."""
+    d, h, w = image_shape
+    R = R.to(torch.float32).clone()
+    R[torch.abs(R) < 1e-8] = 0.0
+    R = torch.flip(R, dims=(-2, -1))
+    fy = torch.fft.fftshift(torch.fft.fftfreq(h, device=device))
+    fx = torch.fft.rfftfreq(w, device=device)
+    grid = torch.stack([torch.zeros(h, fx.shape[0], device=device), fy[:, None].expand(h, fx.shape[0]),
+                        fx[None, :].expand(h, fx.shape[0])], dim=-1).reshape(-1, 3)
+    rc = (R[:, None] @ grid[..., None])[..., 0]  # (n, b, 3)
+    conj = rc[..., 2] < -1e-8
+    near = torch.abs(rc[..., 2]) < 1e-8
+    r11, r12 = R[:, 2, 2][:, None].expand_as(near), R[:, 2, 1][:, None].expand_as(near)
+    tie = torch.where(torch.abs(r11) <= 1e-7, r12 < 0, r11 < 0)
+    conj = conj | (near & tie)
+    rc[conj] *= -1
+    shape = torch.tensor([d, h, w], device=device, dtype=torch.float32)
+    c = torch.empty_like(rc)
+    c[..., :2] = (rc[..., :2] + 0.5) / (1 - 1 / shape[:2]) * (shape[:2] - 1)
+    c[..., 2] = rc[..., 2] * 2 * (w // 2)
+    return c, conj
+
+
+def torch_extract(V, image_shape, R):
+    """grid_sample on the stride-0 expanded volume (image_interpolation_3d.py:53-99)."""
+    c, conj = _torch_coords(image_shape, R, V.device)
+    n, b, _ = c.shape
+    cf = c.reshape(-1, 3)
+    img = torch.view_as_real(V).permute(3, 0, 1, 2)[None].expand(cf.shape[0], 2, *V.shape)
+    dims = torch.tensor(V.shape, device=V.device, dtype=torch.float32)
+    grid = torch.flip(cf / (0.5 * dims - 0.5) - 1, dims=(-1,)).reshape(-1, 1, 1, 1, 3)
+    s = torch.nn.functional.grid_sample(img, grid, mode="bilinear", padding_mode="border", align_corners=True)
+    s = torch.view_as_complex(s.reshape(-1, 2).contiguous())
+    inside = torch.all((cf >= 0) & (cf <= dims - 1), dim=-1)
+    s = s * inside
+    s = s.reshape(n, b)
+    s[conj] = torch.conj(s[conj])
+    return s.reshape(n, image_shape[1], image_shape[2] // 2 + 1)
+
+
+def torch_adjoint(gS, gV, image_shape, R):
+    """Adjoint as the reference's splat op sequence: index_put_(accumulate=True)
+    of w * conj-folded gradients at the 8 corners (insertion :289-346)."""
+    c, conj = _torch_coords(image_shape, R, gV.device)
+    g = gS.reshape(c.shape[0], -1).clone()
+    g[conj] = torch.conj(g[conj])
+    cf, g = c.reshape(-1, 3), g.reshape(-1)
+    dims = torch.tensor(gV.shape, device=gV.device, dtype=torch.float32)
+    keep = torch.all((cf >= 0) & (cf <= dims - 1), dim=-1)
+    cf, g = cf[keep], g[keep]
+    lo = torch.floor(cf)
+    t = cf - lo
+    lo = lo.long()
+    for q in range(8):
+        dz, dy, dx = q >> 2, (q >> 1) & 1, q & 1
+        w = (t[:, 0] if dz else 1 - t[:, 0]) * (t[:, 1] if dy else 1 - t[:, 1]) * (t[:, 2] if dx else 1 - t[:, 2])
+        iz = torch.clamp(lo[:, 0] + dz, max=gV.shape[0] - 1)
+        iy = torch.clamp(lo[:, 1] + dy, max=gV.shape[1] - 1)
+        ix = torch.clamp(lo[:, 2] + dx, max=gV.shape[2] - 1)
+        gV.index_put_((iz, iy, ix), g * w, accumulate=True)
+
+
+def time_torch_comparator(V, R, gS, steps=1, chunk=512):
+    """Plain-torch restatement of the reference op sequence on the same GPU."""
+
+    def step():
+        out = []
+        gV = torch.zeros_like(V)
+        for r0 in range(0, R.shape[0], chunk):
+            out.append(torch_extract(V, (N, N, N), R[r0:r0 + chunk]))
+            torch_adjoint(gS[r0:r0 + chunk], gV, (N, N, N), R[r0:r0 + chunk])
+        return out, gV
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return N_ROT / dt, dt * 1e3, chunk
+
+
+def cpu_baseline(V, R, gS, n_rot=4):
+    """FP64 oracle (NumPy) on the host, extract + adjoint of n_rot slices at 256^3."""
+    from oracle import fourier_slice as O
+
+    Vn = V.cpu().numpy().astype(np.complex128)
+    Rn = O.prepare_rotations(R[:n_rot].cpu().numpy(), False, 1e-8)
+    g = gS[:n_rot].cpu().numpy()
+    t0 = time.perf_counter()
+    O.extract_central_slices(Vn, (N, N, N), Rn)
+    O.extract_adjoint(g, Vn.shape, (N, N, N), Rn)
+    dt = time.perf_counter() - t0
+    return {"value": n_rot / dt, "unit": "slices/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy oracle, extract+adjoint of {n_rot} of the {N_ROT} rotations at {N}^3, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not launched)"}))
+        sys.exit(2)
+    from teamtomo_amd import _lib
+
+    _lib.load()
+    device = torch.device("cuda:0")
+    torch.cuda.set_device(device)
+    V, R, gS = make_inputs(device)
+    step = Step(V, R, gS, args.deterministic)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        step(timed=True)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(stop)
+    ms_step = total_ms / args.steps
+    ext_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in step.events]))
+    adj_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in step.events]))
+    ext_b, adj_b = algorithmic_bytes()
+    kernels = {
+        "extract_fwd_kernel": {"ms": ext_ms, "bytes": ext_b, "GBs": ext_b / ext_ms / 1e6},
+        "adjoint (zero + scatter)": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    achieved = kernels[dom]["GBs"]
+    step_bytes = ext_b + adj_b
+    step_gbs = step_bytes / ms_step / 1e6
+    value = N_ROT / (ms_step / 1e3)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+
+    line = {
+        "metric": "slices/s fwd+adjoint at 256^3 c64 vs plain torch on same GPU; % HBM roofline",
+        "value": value,
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded 200-blob 256^3 volume, uniform SO(3) rotations, N(0,1) c64 gradients)",
+        "config": {"workload": "extract+adjoint 256^3 rfft (256,256,129) c64, 4096 rotations",
+                   "volume": [N, N, N // 2 + 1], "rotations": N_ROT, "parallelism": "single GPU",
+                   "deterministic": bool(args.deterministic)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "roofline_step": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS, "bytes": step_bytes},
+        "kernels": kernels,
+        "wall_s": wall,
+    }
+    if not args.no_torch:
+        try:
+            tv, tms, chunk = time_torch_comparator(V, R, gS)
+            line["torch_same_gpu"] = {"value": tv, "unit": "slices/s", "ms_per_step": tms,
+                                      "rotation_chunk": chunk, "speedup": value / tv}
+        except Exception as exc:  # report, do not hide the main result
+            line["torch_same_gpu"] = {"error": repr(exc)[:300]}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(V, R, gS)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

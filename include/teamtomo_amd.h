@@ -1,0 +1,144 @@
+/*
+ * teamtomo_amd.h -- C ABI of the MI355X (gfx950) Fourier-slice gather/scatter library.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * passed as `void *` (NULL = the null stream).  Launches are asynchronous and
+ * never synchronise the stream.  Outputs are written (or accumulated, where
+ * stated) into buffers the caller allocated.  The return value is TT_OK or an
+ * error code; tt_last_error() returns a message for the calling thread.
+ *
+ * Layouts (C-contiguous, zyx):
+ *   complex64 volume  (D, H, Wr)  interleaved (re, im) float pairs, DC at (D/2, H/2, 0)
+ *   complex64 slices  (n, Hi, Wri) interleaved, DC at (Hi/2, 0)
+ *   rotations         (n, 3, 3) float32, already cast, near-zero clamped and
+ *                     flipped to zyx by the caller (rows/cols ordered z, y, x)
+ *   rotation grads    (n, 3, 3) float64, accumulated (+=)
+ *
+ * Reference interfaces replaced (paths relative to
+ * packages/primitives/ of teamtomo/teamtomo):
+ *   torch-fourier-slice/src/torch_fourier_slice/slice_extraction/_extract_central_slices_rfft_3d.py:84
+ *   torch-fourier-slice/src/torch_fourier_slice/slice_insertion/_insert_central_slices_rfft_3d.py:9,289
+ *   torch-image-interpolation/src/torch_image_interpolation/image_interpolation_3d.py:11,104
+ */
+#ifndef TEAMTOMO_AMD_H
+#define TEAMTOMO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TT_OK 0
+#define TT_ERR_INVALID 1 /* bad argument (shape, size, null pointer)        */
+#define TT_ERR_HIP 2     /* a HIP runtime call or kernel launch failed       */
+
+#define TT_INTERP_NEAREST 0
+#define TT_INTERP_TRILINEAR 1
+
+/* Message for the last error on the calling thread ("" if none). */
+const char *tt_last_error(void);
+
+/* ABI version; bumped whenever a signature below changes. */
+int tt_abi_version(void);
+
+/*
+ * Central-slice extraction (the gather).
+ * Replaces extract_central_slices_rfft_3d
+ * (torch-fourier-slice/.../slice_extraction/_extract_central_slices_rfft_3d.py:84-222)
+ * and its sample_image_3d / grid_sample call (.../image_interpolation_3d.py:11-101).
+ *   volume      (vol_d, vol_h, vol_wr) complex64
+ *   img_*       image_shape (D, H, W) that drives the frequency grid
+ *   slices      (n_rot, img_h, img_w/2+1) complex64, written (every pixel)
+ *   fftfreq_max < 0 means None; x_tolerance as in the reference.
+ */
+int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr,
+                   int img_d, int img_h, int img_w,
+                   const float *rot_zyx, int64_t n_rot,
+                   double fftfreq_max, double x_tolerance,
+                   void *slices, void *stream);
+
+/*
+ * Backward of the extraction: the adjoint scatter into grad_volume
+ * (accumulated, complex64 (vol_d, vol_h, vol_wr)) and/or the analytic
+ * rotation gradient (accumulated into grad_rot, float64 (n_rot, 3, 3) zyx).
+ * Either output may be NULL.  `volume` is read only when grad_rot != NULL.
+ * Replaces torch autograd through grid_sample (the grid_sampler_3d_backward
+ * reached from .../image_interpolation_3d.py:69-75).
+ * deterministic != 0 selects the fixed-order scatter (bitwise reproducible).
+ */
+int tt_extract_bwd_c64(const void *grad_slices, const void *volume,
+                       int vol_d, int vol_h, int vol_wr,
+                       int img_d, int img_h, int img_w,
+                       const float *rot_zyx, int64_t n_rot,
+                       double fftfreq_max, double x_tolerance,
+                       void *grad_volume, double *grad_rot,
+                       int deterministic, void *stream);
+
+/*
+ * Central-slice insertion (the scatter), accumulating into a volume and a
+ * weights volume of shape (vol_d, vol_h, vol_w/2+1).
+ * Replaces insert_central_slices_rfft_3d and _insert_into_3d_dft_linear
+ * (torch-fourier-slice/.../slice_insertion/_insert_central_slices_rfft_3d.py:9-106,289-406),
+ * including the x=0-plane Friedel mirror.
+ *   slices      (n_rot, vol_h, vol_w/2+1)
+ *   _c64 : slices/volume complex64, weights float32
+ *   _c128: slices/volume complex128, weights float64
+ */
+int tt_insert_c64(const void *slices, int vol_d, int vol_h, int vol_w,
+                  const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                  void *volume, float *weights, int deterministic, void *stream);
+int tt_insert_c128(const void *slices, int vol_d, int vol_h, int vol_w,
+                   const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                   void *volume, double *weights, int deterministic, void *stream);
+
+/*
+ * Backward of the insertion: the gradient w.r.t. the slices (the adjoint
+ * gather, written into grad_slices) and/or w.r.t. the rotations (accumulated
+ * into grad_rot).  grad_weights may be NULL (treated as 0); `slices` is read
+ * only when grad_rot != NULL.
+ */
+int tt_insert_bwd_c64(const void *grad_volume, const float *grad_weights,
+                      const void *slices, int vol_d, int vol_h, int vol_w,
+                      const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                      void *grad_slices, double *grad_rot, void *stream);
+int tt_insert_bwd_c128(const void *grad_volume, const double *grad_weights,
+                       const void *slices, int vol_d, int vol_h, int vol_w,
+                       const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                       void *grad_slices, double *grad_rot, void *stream);
+
+/*
+ * Generic sampler: replaces sample_image_3d
+ * (torch-image-interpolation/.../image_interpolation_3d.py:11-101).
+ *   image   (channels, d, h, w) real, or complex interleaved if is_complex
+ *   coords  (n, 3) zyx array coordinates
+ *   out     (n, channels) values (complex interleaved if is_complex)
+ *   mode    TT_INTERP_TRILINEAR or TT_INTERP_NEAREST
+ * _f32: float32 image/coords/out; _f64: float64.
+ */
+int tt_sample3d_f32(const float *image, int channels, int d, int h, int w, int is_complex,
+                    const float *coords, int64_t n, int mode, float *out, void *stream);
+int tt_sample3d_f64(const double *image, int channels, int d, int h, int w, int is_complex,
+                    const double *coords, int64_t n, int mode, double *out, void *stream);
+
+/*
+ * Generic splat, accumulating in place: replaces insert_into_image_3d
+ * (torch-image-interpolation/.../image_interpolation_3d.py:104-275).
+ *   values  (n, channels) (complex interleaved if is_complex)
+ *   coords  (n, 3) float32 zyx
+ *   image   (channels, d, h, w) accumulated
+ *   weights (d, h, w) accumulated; float32 if weights_f64 == 0 else float64
+ * _f32: float32 values/image; _f64: float64 values/image.
+ */
+int tt_splat3d_f32(const float *values, const float *coords, int64_t n, int channels,
+                   int is_complex, int d, int h, int w, int mode,
+                   float *image, void *weights, int weights_f64, void *stream);
+int tt_splat3d_f64(const double *values, const float *coords, int64_t n, int channels,
+                   int is_complex, int d, int h, int w, int mode,
+                   double *image, void *weights, int weights_f64, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEAMTOMO_AMD_H */

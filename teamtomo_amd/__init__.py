@@ -1,0 +1,67 @@
+"""teamtomo_amd: MI355X (gfx950) Fourier-slice gather/scatter for TeamTomo.
+
+Drop-in replacements for the 3-D central-slice operators of
+``torch_fourier_slice`` and the 3-D sampling/splatting of
+``torch_image_interpolation``, running as hand-written HIP kernels behind a
+C ABI (``include/teamtomo_amd.h``).  ``install_dropin()`` registers modules
+under the reference package names so unchanged callers pick them up.
+"""
+
+from __future__ import annotations
+
+import sys
+import types
+
+from ._config import deterministic_enabled, set_deterministic
+from .fourier_slice import (
+    backproject_2d_to_3d,
+    backproject_2d_to_3d_multichannel,
+    extract_central_slices_rfft_3d,
+    extract_central_slices_rfft_3d_multichannel,
+    insert_central_slices_rfft_3d,
+    insert_central_slices_rfft_3d_multichannel,
+    project_3d_to_2d,
+    project_3d_to_2d_multichannel,
+)
+from .image_interpolation import insert_into_image_3d, sample_image_3d
+
+__all__ = [
+    "extract_central_slices_rfft_3d",
+    "extract_central_slices_rfft_3d_multichannel",
+    "insert_central_slices_rfft_3d",
+    "insert_central_slices_rfft_3d_multichannel",
+    "project_3d_to_2d",
+    "project_3d_to_2d_multichannel",
+    "backproject_2d_to_3d",
+    "backproject_2d_to_3d_multichannel",
+    "sample_image_3d",
+    "insert_into_image_3d",
+    "set_deterministic",
+    "deterministic_enabled",
+    "install_dropin",
+]
+
+
+def install_dropin() -> None:
+    """Register ``torch_fourier_slice`` and ``torch_image_interpolation`` modules
+    (and their ``slice_extraction`` / ``slice_insertion`` submodules) backed by
+    this package, so ``from torch_fourier_slice import project_3d_to_2d`` in
+    unchanged caller code resolves to the gfx950 implementation."""
+    tfs = types.ModuleType("torch_fourier_slice")
+    ext = types.ModuleType("torch_fourier_slice.slice_extraction")
+    ins = types.ModuleType("torch_fourier_slice.slice_insertion")
+    for name in ("extract_central_slices_rfft_3d", "extract_central_slices_rfft_3d_multichannel"):
+        setattr(ext, name, globals()[name])
+    for name in ("insert_central_slices_rfft_3d", "insert_central_slices_rfft_3d_multichannel"):
+        setattr(ins, name, globals()[name])
+    for name in __all__[:8]:
+        setattr(tfs, name, globals()[name])
+    tfs.slice_extraction, tfs.slice_insertion = ext, ins
+    tii = types.ModuleType("torch_image_interpolation")
+    tii.sample_image_3d, tii.insert_into_image_3d = sample_image_3d, insert_into_image_3d
+    sys.modules.update({
+        "torch_fourier_slice": tfs,
+        "torch_fourier_slice.slice_extraction": ext,
+        "torch_fourier_slice.slice_insertion": ins,
+        "torch_image_interpolation": tii,
+    })

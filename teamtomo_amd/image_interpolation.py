@@ -1,0 +1,119 @@
+"""Drop-in replacements for torch_image_interpolation's 3-D entry points.
+
+Reference: packages/primitives/torch-image-interpolation/src/
+torch_image_interpolation/image_interpolation_3d.py.  Sampling and splatting
+run as gfx950 HIP kernels through the C ABI (include/teamtomo_amd.h).
+"""
+
+from __future__ import annotations
+
+from typing import Literal
+
+import torch
+
+from . import _lib
+
+__all__ = ["sample_image_3d", "insert_into_image_3d"]
+
+_REAL_OF = {
+    torch.float32: torch.float32,
+    torch.complex64: torch.float32,
+    torch.float64: torch.float64,
+    torch.complex128: torch.float64,
+}
+
+
+def _mode(interpolation: str) -> int:
+    if interpolation == "trilinear":
+        return _lib.TT_INTERP_TRILINEAR
+    if interpolation == "nearest":
+        return _lib.TT_INTERP_NEAREST
+    # grid_sample's own error for an unknown mode
+    raise ValueError(
+        "nn.functional.grid_sample(): expected mode to be 'bilinear', 'nearest' or "
+        f"'bicubic', but got: '{interpolation}'"
+    )
+
+
+def sample_image_3d(
+    image: torch.Tensor,
+    coordinates: torch.Tensor,
+    interpolation: Literal["nearest", "trilinear"] = "trilinear",
+) -> torch.Tensor:
+    """Sample a (d, h, w) or (c, d, h, w) image at (..., 3) zyx array coordinates.
+
+    Reference: image_interpolation_3d.py:11-101.  Returns (...) or (..., c);
+    points outside [0, N-1] in any dimension give 0.
+    """
+    if image.ndim not in (3, 4):
+        raise ValueError(f"image should have shape (d, h, w) or (c, d, h, w), got {image.shape}")
+    _lib.require_gpu(image)
+    if image.dtype not in _REAL_OF:
+        raise TypeError(f"sample_image_3d: unsupported image dtype {image.dtype}")
+    real_t = _REAL_OF[image.dtype]
+    if coordinates.is_floating_point() and coordinates.dtype != real_t:
+        raise RuntimeError(
+            "grid_sampler(): expected input and grid to have same dtype, but input has "
+            f"{real_t} and grid has {coordinates.dtype}"
+        )
+    mode = _mode(interpolation)
+    multichannel = image.ndim == 4
+    img = image if multichannel else image.unsqueeze(0)
+    C, d, h, w = img.shape
+    lead = tuple(coordinates.shape[:-1])
+    crd = coordinates.to(device=image.device, dtype=real_t).reshape(-1, 3).contiguous()
+    n = crd.shape[0]
+    is_c = image.is_complex()
+    img = img.contiguous()
+    out = torch.empty((n, C), dtype=image.dtype, device=image.device)
+    fn = "tt_sample3d_f32" if real_t == torch.float32 else "tt_sample3d_f64"
+    _lib.call(fn, _lib.ptr(img), C, d, h, w, int(is_c), _lib.ptr(crd), n, mode, _lib.ptr(out),
+              _lib.stream_of(image))
+    out = out.reshape(*lead, C)
+    return out if multichannel else out[..., 0]
+
+
+def insert_into_image_3d(
+    values: torch.Tensor,
+    coordinates: torch.Tensor,
+    image: torch.Tensor,
+    weights: torch.Tensor | None = None,
+    interpolation: Literal["nearest", "trilinear"] = "trilinear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """Splat values into ``image`` (and ``weights``) in place; returns both.
+
+    Reference: image_interpolation_3d.py:104-275.  Trilinear splats use
+    floor/ceil corners, nearest rounds half to even with weight 1; points
+    outside [0, N-1] are dropped and an unknown mode changes nothing.
+    """
+    multichannel = image.ndim == 4
+    d, h, w = image.shape[-3:]
+    values_shape = values.shape[:-1] if multichannel else values.shape
+    if values_shape != coordinates.shape[:-1]:
+        raise ValueError("One coordinate triplet is required for each value in data.")
+    if coordinates.shape[-1] != 3:
+        raise ValueError("Coordinates must be 3D with shape (..., 3).")
+    if image.dtype != values.dtype:
+        raise ValueError("Image and values must have the same dtype.")
+    _lib.require_gpu(image, values)
+    if weights is None:
+        weights = torch.zeros(size=(d, h, w), dtype=torch.float32, device=image.device)
+    if interpolation not in ("nearest", "trilinear"):
+        return image, weights
+    if image.dtype not in _REAL_OF:
+        raise TypeError(f"insert_into_image_3d: unsupported image dtype {image.dtype}")
+    if weights.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"insert_into_image_3d: weights must be float32 or float64, got {weights.dtype}")
+    if not image.is_contiguous() or not weights.is_contiguous():
+        raise ValueError("insert_into_image_3d: image and weights must be contiguous (updated in place)")
+    C = image.shape[0] if multichannel else 1
+    vals = values.reshape(-1, C).contiguous()
+    crd = coordinates.to(device=image.device, dtype=torch.float32).reshape(-1, 3).contiguous()
+    n = crd.shape[0]
+    real_t = _REAL_OF[image.dtype]
+    fn = "tt_splat3d_f32" if real_t == torch.float32 else "tt_splat3d_f64"
+    _lib.call(fn, _lib.ptr(vals), _lib.ptr(crd), n, C, int(image.is_complex()), d, h, w,
+              _lib.TT_INTERP_TRILINEAR if interpolation == "trilinear" else _lib.TT_INTERP_NEAREST,
+              _lib.ptr(image), _lib.ptr(weights), int(weights.dtype == torch.float64),
+              _lib.stream_of(image))
+    return image, weights

@@ -1,0 +1,235 @@
+"""Write the golden input/output vectors of the reference Fourier-slice path.
+
+Run once, on a machine that has the reference sources, with
+``python tests/golden/make_golden.py [REFERENCE_ROOT]`` (default
+``/root/reference``).  It imports the reference's pure-Python packages
+(torch-fourier-slice, torch-image-interpolation, torch-grid-utils) on the CPU
+and stores inputs and outputs as small ``.npz`` fixtures next to this script.
+Nothing at test time reads the reference: the tests load the fixtures.
+
+The fixtures pin the composite float32 behaviour that closed forms and
+invariants leave open (odd box sizes, an image_shape different from the
+volume shape, the fftfreq_max disc, the insertion Friedel mirror, nearest
+rounding ties, torch-autograd gradients, the near-zero conjugation tie-break).
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _import_reference(root):
+    pkgs = os.path.join(root, "packages", "primitives")
+    for name in ("torch-fourier-slice", "torch-image-interpolation", "torch-grid-utils"):
+        sys.path.insert(0, os.path.join(pkgs, name, "src"))
+    import torch_fourier_slice as tfs  # noqa: E402
+    import torch_image_interpolation as tii  # noqa: E402
+
+    return tfs, tii
+
+
+def uniform_rotations(n, seed):
+    """Uniform SO(3) matrices from normalised Gaussian quaternions (float32)."""
+    import torch
+
+    g = torch.Generator().manual_seed(seed)
+    q = torch.randn(n, 4, generator=g, dtype=torch.float64)
+    q = q / q.norm(dim=-1, keepdim=True)
+    w, x, y, z = q.unbind(-1)
+    R = torch.stack(
+        [
+            1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+            2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+            2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y),
+        ],
+        dim=-1,
+    ).reshape(n, 3, 3)
+    return R.to(torch.float32)
+
+
+def zyz_matrix(angles_deg):
+    """Intrinsic ZYZ Euler angles (degrees) -> xyz rotation matrix, float64.
+
+    Equals scipy's Rotation.from_euler("ZYZ", angles, degrees=True).as_matrix().
+    """
+    from scipy.spatial.transform import Rotation
+
+    return Rotation.from_euler("ZYZ", angles_deg, degrees=True).as_matrix()
+
+
+def vol_rfft(v):
+    import torch
+
+    f = torch.fft.fftshift(v, dim=(-3, -2, -1))
+    f = torch.fft.rfftn(f, dim=(-3, -2, -1))
+    return torch.fft.fftshift(f, dim=(-3, -2))
+
+
+def main(root):
+    import torch
+
+    torch.set_num_threads(8)
+    tfs, tii = _import_reference(root)
+    ext = tfs.extract_central_slices_rfft_3d
+    ins = tfs.insert_central_slices_rfft_3d
+
+    def save(name, **arrays):
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
+        print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+    # G1/G2: 64^3 N(0,1) volume, 100 uniform rotations; extract then insert.
+    g = torch.Generator().manual_seed(1)
+    v = torch.randn(64, 64, 64, generator=g)
+    V = vol_rfft(v)
+    R = uniform_rotations(100, 0)
+    S = ext(V, (64, 64, 64), R.clone())
+    Vi, Wi = ins(S, (64, 64, 64), R.clone())
+    save("extract_insert_64", volume_rfft=V.numpy(), rotations=R.numpy(), slices=S.numpy(),
+         ins_volume=Vi.numpy(), ins_weights=Wi.numpy())
+
+    # G3: odd box 33^3.
+    g = torch.Generator().manual_seed(5)
+    v = torch.randn(33, 33, 33, generator=g)
+    V = vol_rfft(v)
+    R = uniform_rotations(20, 6)
+    S = ext(V, (33, 33, 33), R.clone())
+    Vi, Wi = ins(S, (33, 33, 33), R.clone())
+    save("odd_33", volume_rfft=V.numpy(), rotations=R.numpy(), slices=S.numpy(),
+         ins_volume=Vi.numpy(), ins_weights=Wi.numpy())
+
+    # G4: image_shape 4x the volume shape.
+    g = torch.Generator().manual_seed(7)
+    v = torch.randn(16, 16, 16, generator=g)
+    V = vol_rfft(v)
+    R = uniform_rotations(12, 8)
+    S = ext(V, (64, 64, 64), R.clone())
+    save("image_shape_4x", volume_rfft=V.numpy(), rotations=R.numpy(), slices=S.numpy())
+
+    # G5: fftfreq_max = 0.3 at 32^3.
+    g = torch.Generator().manual_seed(9)
+    v = torch.randn(32, 32, 32, generator=g)
+    V = vol_rfft(v)
+    R = uniform_rotations(16, 10)
+    S = ext(V, (32, 32, 32), R.clone(), fftfreq_max=0.3)
+    Vi, Wi = ins(S, (32, 32, 32), R.clone(), fftfreq_max=0.3)
+    save("fftfreq_max_03", volume_rfft=V.numpy(), rotations=R.numpy(), slices=S.numpy(),
+         ins_volume=Vi.numpy(), ins_weights=Wi.numpy())
+
+    # G6: generic sampler, real / complex / multichannel, trilinear / nearest,
+    # coordinates partly out of bounds and exact .5 ties.
+    g = torch.Generator().manual_seed(11)
+    img_r = torch.randn(12, 10, 9, generator=g)
+    img_c = torch.complex(torch.randn(12, 10, 9, generator=g), torch.randn(12, 10, 9, generator=g))
+    img_m = torch.randn(3, 12, 10, 9, generator=g)
+    hi = torch.tensor([12.0, 10.0, 9.0])
+    crd = torch.rand(400, 3, generator=g) * (hi + 1.0) - 1.0
+    ties = torch.randint(0, 8, (100, 3), generator=g).float() + 0.5
+    ints = torch.randint(0, 9, (50, 3), generator=g).float()
+    crd = torch.cat([crd, ties, ints]).reshape(11, 50, 3)
+    out = {}
+    for mode in ("trilinear", "nearest"):
+        out[f"real_{mode}"] = tii.sample_image_3d(img_r, crd, interpolation=mode).numpy()
+        out[f"complex_{mode}"] = tii.sample_image_3d(img_c, crd, interpolation=mode).numpy()
+        out[f"multi_{mode}"] = tii.sample_image_3d(img_m, crd, interpolation=mode).numpy()
+    save("sample_image_3d", image_real=img_r.numpy(), image_complex=img_c.numpy(),
+         image_multi=img_m.numpy(), coordinates=crd.numpy(), **out)
+
+    # G7: generic splat, trilinear / nearest, two accumulating calls.
+    g = torch.Generator().manual_seed(12)
+    crd1 = torch.rand(300, 3, generator=g) * (hi + 1.0) - 1.0
+    crd1 = torch.cat([crd1, ties[:50]])
+    crd2 = torch.rand(200, 3, generator=g) * (hi + 1.0) - 1.0
+    val_r1 = torch.randn(350, generator=g)
+    val_r2 = torch.randn(200, generator=g)
+    val_c1 = torch.complex(torch.randn(350, generator=g), torch.randn(350, generator=g))
+    val_c2 = torch.complex(torch.randn(200, generator=g), torch.randn(200, generator=g))
+    val_m1 = torch.randn(350, 2, generator=g)
+    val_m2 = torch.randn(200, 2, generator=g)
+    out = {}
+    for mode in ("trilinear", "nearest"):
+        im, w = tii.insert_into_image_3d(val_r1, crd1, torch.zeros(12, 10, 9), interpolation=mode)
+        im, w = tii.insert_into_image_3d(val_r2, crd2, im, w, interpolation=mode)
+        out[f"real_{mode}_image"], out[f"real_{mode}_weights"] = im.numpy(), w.numpy()
+        im, w = tii.insert_into_image_3d(val_c1, crd1, torch.zeros(12, 10, 9, dtype=torch.complex64),
+                                         interpolation=mode)
+        im, w = tii.insert_into_image_3d(val_c2, crd2, im, w, interpolation=mode)
+        out[f"complex_{mode}_image"], out[f"complex_{mode}_weights"] = im.numpy(), w.numpy()
+        im, w = tii.insert_into_image_3d(val_m1, crd1, torch.zeros(2, 12, 10, 9), interpolation=mode)
+        im, w = tii.insert_into_image_3d(val_m2, crd2, im, w, interpolation=mode)
+        out[f"multi_{mode}_image"], out[f"multi_{mode}_weights"] = im.numpy(), w.numpy()
+    save("insert_into_image_3d", coords1=crd1.numpy(), coords2=crd2.numpy(),
+         val_r1=val_r1.numpy(), val_r2=val_r2.numpy(), val_c1=val_c1.numpy(),
+         val_c2=val_c2.numpy(), val_m1=val_m1.numpy(), val_m2=val_m2.numpy(), **out)
+
+    # G8: project / backproject of a 24^3 blob volume, pad 2.
+    g = torch.Generator().manual_seed(13)
+    zz, yy, xx = torch.meshgrid(*[torch.arange(24.0)] * 3, indexing="ij")
+    vol = torch.zeros(24, 24, 24)
+    for _ in range(6):
+        cz, cy, cx = (torch.rand(3, generator=g) * 12 + 6).tolist()
+        s = float(torch.rand(1, generator=g) * 2 + 1.5)
+        vol += torch.exp(-((zz - cz) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2) / (2 * s * s))
+    R = uniform_rotations(6, 14)
+    P = tfs.project_3d_to_2d(vol, R.clone())
+    B = tfs.backproject_2d_to_3d(P, R.clone())
+    save("project_backproject_24", volume=vol.numpy(), rotations=R.numpy(), projections=P.numpy(),
+         backprojection=B.numpy())
+
+    # G9: torch-autograd gradients at toy size (float64 leaf rotations).
+    g = torch.Generator().manual_seed(15)
+    v = torch.randn(8, 8, 8, generator=g)
+    V = vol_rfft(v).detach().requires_grad_(True)
+    R64 = uniform_rotations(3, 16).to(torch.float64).requires_grad_(True)
+    T = torch.complex(torch.randn(3, 8, 5, generator=g), torch.randn(3, 8, 5, generator=g))
+    S = ext(V, (8, 8, 8), R64)
+    loss = torch.real(torch.sum(torch.conj(T) * S))
+    loss.backward()
+    ext_gV, ext_gR = V.grad.numpy().copy(), R64.grad.numpy().copy()
+    D_ = torch.complex(torch.randn(3, 8, 5, generator=g), torch.randn(3, 8, 5, generator=g))
+    D_ = D_.requires_grad_(True)
+    R64b = uniform_rotations(3, 17).to(torch.float64).requires_grad_(True)
+    TV = torch.complex(torch.randn(8, 8, 5, generator=g), torch.randn(8, 8, 5, generator=g))
+    TW = torch.randn(8, 8, 5, generator=g)
+    Vi, Wi = ins(D_, (8, 8, 8), R64b)
+    loss = torch.real(torch.sum(torch.conj(TV) * Vi)) + torch.sum(TW * Wi)
+    loss.backward()
+    save("autograd_8", volume_rfft=V.detach().numpy(), ext_rotations=R64.detach().numpy(),
+         ext_target=T.numpy(), ext_slices=S.detach().numpy(), ext_grad_volume=ext_gV,
+         ext_grad_rotations=ext_gR, ins_slices=D_.detach().numpy(),
+         ins_rotations=R64b.detach().numpy(), ins_target_volume=TV.numpy(),
+         ins_target_weights=TW.numpy(), ins_grad_slices=D_.grad.numpy(),
+         ins_grad_rotations=R64b.grad.numpy())
+
+    # G10: in-plane 180-degree pairs with points on the x'=0 boundary, and the
+    # rotations of the reference's insertion Friedel test.
+    pairs = [
+        ([0, 0, 0], [0, 0, 180]), ([0, 0, 0], [180, 0, 0]), ([0, 20, 0], [0, 20, 180]),
+        ([0, 20, 90], [0, 20, 270]), ([0, 60, 0], [0, 60, 180]), ([90, 90, 0], [90, 90, 180]),
+        ([45, 0, 0], [45, 0, 180]), ([45, 90, 0], [45, 90, 180]),
+    ]
+    g = torch.Generator().manual_seed(18)
+    v = torch.randn(32, 32, 32, generator=g)
+    V = vol_rfft(v)
+    left = np.stack([zyz_matrix(a) for a, _ in pairs])
+    right = np.stack([zyz_matrix(b) for _, b in pairs])
+    SL = ext(V, (32, 32, 32), torch.from_numpy(left))
+    SR = ext(V, (32, 32, 32), torch.from_numpy(right))
+    ry90 = np.array([[0, 0, 1], [0, 1, 0], [-1, 0, 0]], dtype=np.float64)
+    rz90 = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]], dtype=np.float64)
+    fried = np.concatenate([np.stack([zyz_matrix([0, 0, 0]), zyz_matrix([0, 0, 180])]),
+                            ry90[None], rz90[None]])
+    Sf = ext(V, (32, 32, 32), torch.from_numpy(fried))
+    Vf, Wf = ins(Sf, (32, 32, 32), torch.from_numpy(fried))
+    save("tiebreak_32", volume_rfft=V.numpy(), left=left, right=right, slices_left=SL.numpy(),
+         slices_right=SR.numpy(), friedel_rotations=fried, friedel_slices=Sf.numpy(),
+         friedel_volume=Vf.numpy(), friedel_weights=Wf.numpy())
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,326 @@
+"""Parity of the gfx950 central-slice kernels against the FP64 oracle and the
+reference's golden vectors.  Tolerances (relative L2): extraction <= 1e-5,
+insertion <= 1e-4 (float atomics reorder the sums), gradients <= 1e-4."""
+
+import numpy as np
+import pytest
+import torch
+from scipy.spatial.transform import Rotation
+
+import teamtomo_amd as tt
+from conftest import golden, max_abs, rel_l2
+from oracle import fourier_slice as O
+from teamtomo_amd.synthetic import blob_volume, uniform_rotations, volume_rfft
+
+pytestmark = pytest.mark.gpu
+
+EXT_TOL = 1e-5
+INS_TOL = 1e-4
+GRAD_TOL = 1e-4
+
+
+def _t(a, device, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    return t if dtype is None else t.to(dtype)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+# ----------------------------------------------------------------- extraction
+def test_extract_config1_vs_oracle_and_golden(device):
+    G = golden("extract_insert_64")
+    V = _t(G["volume_rfft"], device)
+    R = _t(G["rotations"], device)
+    S = _np(tt.extract_central_slices_rfft_3d(V, (64, 64, 64), R))
+    ref = O.extract_central_slices(G["volume_rfft"], (64, 64, 64), O.prepare_rotations(G["rotations"], False, 1e-8))
+    print("extract config1 rel_l2 vs FP64:", rel_l2(S, ref), "max", max_abs(S, ref))
+    assert rel_l2(S, ref) < EXT_TOL
+    assert rel_l2(S, G["slices"]) < EXT_TOL
+
+
+@pytest.mark.parametrize("name,shape,fmax", [
+    ("odd_33", (33, 33, 33), None), ("fftfreq_max_03", (32, 32, 32), 0.3), ("image_shape_4x", (64, 64, 64), None),
+])
+def test_extract_golden_cases(device, name, shape, fmax):
+    G = golden(name)
+    S = _np(tt.extract_central_slices_rfft_3d(_t(G["volume_rfft"], device), shape, _t(G["rotations"], device),
+                                              fftfreq_max=fmax))
+    ref = O.extract_central_slices(G["volume_rfft"], shape, O.prepare_rotations(G["rotations"], False, 1e-8), fmax)
+    assert rel_l2(S, ref) < EXT_TOL
+    assert rel_l2(S, G["slices"]) < EXT_TOL
+    assert np.array_equal(S == 0, ref == 0)
+
+
+def test_extract_conjugate_pairs_and_tiebreak(device):
+    G = golden("tiebreak_32")
+    V = _t(G["volume_rfft"], device)
+    SL = _np(tt.extract_central_slices_rfft_3d(V, (32, 32, 32), _t(G["left"], device)))
+    SR = _np(tt.extract_central_slices_rfft_3d(V, (32, 32, 32), _t(G["right"], device)))
+    scale = np.abs(G["volume_rfft"]).max()
+    assert max_abs(SL, np.conj(SR)) < 1e-6 * scale
+    assert rel_l2(SL, G["slices_left"]) < EXT_TOL
+    # 100 random in-plane 180-degree pairs (reference test :189-232)
+    rng = np.random.default_rng(0)
+    ang = rng.uniform([0, 0, 0], [360, 180, 360], (100, 3))
+    left = Rotation.from_euler("ZYZ", ang, degrees=True).as_matrix()
+    right = left @ Rotation.from_euler("ZYZ", [0, 0, 180], degrees=True).as_matrix()
+    SL = _np(tt.extract_central_slices_rfft_3d(V, (32, 32, 32), _t(left, device)))
+    SR = _np(tt.extract_central_slices_rfft_3d(V, (32, 32, 32), _t(right, device)))
+    assert max_abs(SL, np.conj(SR)) < 1e-6 * scale
+
+
+def test_extract_x0_friedel_line(device):
+    vol = blob_volume(64, 20, seed=3, device=device)
+    V = volume_rfft(vol)
+    S = _np(tt.extract_central_slices_rfft_3d(V, (64, 64, 64), uniform_rotations(10, 3, device=device)))
+    line = S[:, :, 0]
+    assert max_abs(line[:, 1:32], np.conj(line[:, -31:])[:, ::-1]) < 1e-6 * np.abs(line).max()
+
+
+def test_extract_identity_exact(device):
+    V = volume_rfft(torch.randn(32, 32, 32, generator=torch.Generator().manual_seed(2))).to(device)
+    S = tt.extract_central_slices_rfft_3d(V, (32, 32, 32), torch.eye(3, device=device)[None])
+    assert torch.equal(S[0], V[16])
+
+
+def test_extract_shapes_dtypes_and_errors(device):
+    V = volume_rfft(torch.randn(16, 16, 16)).to(device)
+    R = torch.rand(4, 5, 3, 3, device=device)
+    assert tt.extract_central_slices_rfft_3d(V, (16, 16, 16), R).shape == (4, 5, 16, 9)
+    R64 = uniform_rotations(3, 1, device=device, dtype=torch.float64)
+    assert tt.extract_central_slices_rfft_3d(V, (16, 16, 16), R64).dtype == torch.complex64
+    empty = tt.extract_central_slices_rfft_3d(V, (16, 16, 16), torch.zeros(0, 3, 3, device=device))
+    assert empty.shape == (0, 16, 9)
+    Rf = uniform_rotations(3, 2, device=device)
+    Rf[0, 0, 1] = 1e-9
+    before = Rf.clone()
+    tt.extract_central_slices_rfft_3d(V, (16, 16, 16), Rf)
+    assert torch.equal(Rf, before)  # no in-place clamp of the caller's tensor
+    with pytest.raises(AssertionError):
+        tt.extract_central_slices_rfft_3d(V[None], (16, 16, 16), Rf)
+    with pytest.raises(RuntimeError):
+        tt.extract_central_slices_rfft_3d(V.cpu(), (16, 16, 16), Rf.cpu())
+
+
+# ------------------------------------------------------------------ insertion
+def test_insert_config1_vs_oracle_and_golden(device):
+    G = golden("extract_insert_64")
+    Vi, Wi = tt.insert_central_slices_rfft_3d(_t(G["slices"], device), (64, 64, 64), _t(G["rotations"], device))
+    rV, rW = O.insert_central_slices(G["slices"], (64, 64, 64), O.prepare_rotations(G["rotations"], False, None))
+    print("insert config1 rel_l2 vs FP64:", rel_l2(_np(Vi), rV), rel_l2(_np(Wi), rW))
+    assert Vi.dtype == torch.complex64 and Wi.dtype == torch.float32
+    assert rel_l2(_np(Vi), rV) < INS_TOL and rel_l2(_np(Wi), rW) < INS_TOL
+    assert rel_l2(_np(Vi), G["ins_volume"]) < INS_TOL and rel_l2(_np(Wi), G["ins_weights"]) < INS_TOL
+
+
+@pytest.mark.parametrize("name,shape,fmax", [("odd_33", (33, 33, 33), None), ("fftfreq_max_03", (32, 32, 32), 0.3)])
+def test_insert_golden_cases(device, name, shape, fmax):
+    G = golden(name)
+    Vi, Wi = tt.insert_central_slices_rfft_3d(_t(G["slices"], device), shape, _t(G["rotations"], device),
+                                              fftfreq_max=fmax)
+    rV, rW = O.insert_central_slices(G["slices"], shape, O.prepare_rotations(G["rotations"], False, None), fmax)
+    assert rel_l2(_np(Vi), rV) < INS_TOL and rel_l2(_np(Wi), rW) < INS_TOL
+    assert rel_l2(_np(Vi), G["ins_volume"]) < INS_TOL
+
+
+def test_insert_friedel_rotations_golden(device):
+    G = golden("tiebreak_32")
+    R = _t(G["friedel_rotations"], device)
+    Vi, Wi = tt.insert_central_slices_rfft_3d(_t(G["friedel_slices"], device), (32, 32, 32), R)
+    assert rel_l2(_np(Vi), G["friedel_volume"]) < INS_TOL
+    assert max_abs(_np(Wi), G["friedel_weights"]) < 1e-5
+    x0 = _np(Vi)[:, :, 0]
+    zz, yy = np.meshgrid(np.arange(32), np.arange(32), indexing="ij")
+    err = np.abs(x0 - np.conj(x0[(32 - zz) % 32, (32 - yy) % 32]))
+    err[16, 16] = 0
+    assert err.mean() < 1e-5
+
+
+def test_insert_complex128(device):
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal((5, 24, 13)) + 1j * rng.standard_normal((5, 24, 13))
+    R = Rotation.random(5, random_state=3).as_matrix()
+    Vi, Wi = tt.insert_central_slices_rfft_3d(_t(d, device), (24, 24, 24), _t(R, device))
+    assert Vi.dtype == torch.complex128 and Wi.dtype == torch.float64
+    rV, rW = O.insert_central_slices(d, (24, 24, 24), O.prepare_rotations(R, False, None))
+    assert rel_l2(_np(Vi), rV) < 1e-6 and rel_l2(_np(Wi), rW) < 1e-6
+
+
+def test_insert_identity_known_answer(device):
+    N = 16
+    d = torch.randn(1, N, N // 2 + 1, dtype=torch.complex64, generator=torch.Generator().manual_seed(4))
+    Vi, Wi = tt.insert_central_slices_rfft_3d(d.to(device), (N, N, N), torch.eye(3, device=device)[None])
+    rV, rW = O.insert_central_slices(d.numpy(), (N, N, N), O.prepare_rotations(np.eye(3)[None], False, None))
+    assert max_abs(_np(Vi), rV) < 1e-6 and max_abs(_np(Wi), rW) == 0.0
+
+
+# ------------------------------------------------------------------ autograd
+def test_extract_backward_vs_oracle_and_golden(device):
+    G = golden("autograd_8")
+    V = _t(G["volume_rfft"], device).requires_grad_(True)
+    R = _t(G["ext_rotations"], device).requires_grad_(True)
+    S = tt.extract_central_slices_rfft_3d(V, (8, 8, 8), R)
+    loss = torch.real(torch.sum(torch.conj(_t(G["ext_target"], device)) * S))
+    loss.backward()
+    assert rel_l2(_np(V.grad), G["ext_grad_volume"]) < GRAD_TOL
+    assert rel_l2(_np(R.grad), G["ext_grad_rotations"]) < GRAD_TOL
+    assert R.grad.dtype == torch.float64
+
+
+def test_insert_backward_vs_golden(device):
+    G = golden("autograd_8")
+    d = _t(G["ins_slices"], device).requires_grad_(True)
+    R = _t(G["ins_rotations"], device).requires_grad_(True)
+    Vi, Wi = tt.insert_central_slices_rfft_3d(d, (8, 8, 8), R)
+    loss = torch.real(torch.sum(torch.conj(_t(G["ins_target_volume"], device)) * Vi)) + torch.sum(
+        _t(G["ins_target_weights"], device) * Wi)
+    loss.backward()
+    assert rel_l2(_np(d.grad), G["ins_grad_slices"]) < GRAD_TOL
+    assert rel_l2(_np(R.grad), G["ins_grad_rotations"]) < GRAD_TOL
+
+
+def test_extract_adjoint_vs_oracle_config1(device):
+    G = golden("extract_insert_64")
+    rng = np.random.default_rng(5)
+    T = (rng.standard_normal(G["slices"].shape) + 1j * rng.standard_normal(G["slices"].shape)).astype(np.complex64)
+    V = _t(G["volume_rfft"], device).requires_grad_(True)
+    S = tt.extract_central_slices_rfft_3d(V, (64, 64, 64), _t(G["rotations"], device))
+    S.backward(_t(T, device))
+    Rz = O.prepare_rotations(G["rotations"], False, 1e-8)
+    ref = O.extract_adjoint(T, (64, 64, 33), (64, 64, 64), Rz)
+    print("adjoint config1 rel_l2 vs FP64:", rel_l2(_np(V.grad), ref))
+    assert rel_l2(_np(V.grad), ref) < INS_TOL
+
+
+def test_rotation_grad_vs_finite_differences(device):
+    """Kernel rotation gradient vs FP64 central differences of the oracle."""
+    rng = np.random.default_rng(6)
+    vol = O.volume_rfft(rng.standard_normal((16, 16, 16)))
+    R = Rotation.random(4, random_state=6).as_matrix()
+    T = rng.standard_normal((4, 16, 9)) + 1j * rng.standard_normal((4, 16, 9))
+    Rt = _t(R, device).requires_grad_(True)
+    S = tt.extract_central_slices_rfft_3d(_t(vol.astype(np.complex64), device), (16, 16, 16), Rt)
+    torch.real(torch.sum(torch.conj(_t(T.astype(np.complex64), device)) * S)).backward()
+    Rz = O.prepare_rotations(R, False, 1e-8).astype(np.float64)
+    volc = vol.astype(np.complex64).astype(np.complex128)
+    loss = lambda Rm: np.real(np.vdot(T.astype(np.complex64), O.extract_central_slices(volc, (16, 16, 16), Rm)))
+    h = 1e-7
+    fd = np.zeros_like(Rz)
+    for idx in np.ndindex(*Rz.shape):
+        Rp, Rm = Rz.copy(), Rz.copy()
+        Rp[idx] += h
+        Rm[idx] -= h
+        fd[idx] = (loss(Rp) - loss(Rm)) / (2 * h)
+    fd_xyz = O.rotation_grad_to_xyz(fd, R, False, 1e-8)
+    print("rotation grad rel_l2 vs FD:", rel_l2(_np(Rt.grad), fd_xyz))
+    assert rel_l2(_np(Rt.grad), fd_xyz) < GRAD_TOL
+
+
+# ------------------------------------------------------------ generic interp
+@pytest.mark.parametrize("mode", ["trilinear", "nearest"])
+def test_sample_image_3d_golden(device, mode):
+    G = golden("sample_image_3d")
+    for key in ("real", "complex", "multi"):
+        out = _np(tt.sample_image_3d(_t(G["image_" + key], device), _t(G["coordinates"], device), mode))
+        assert out.shape == G[f"{key}_{mode}"].shape
+        assert rel_l2(out, G[f"{key}_{mode}"]) < EXT_TOL
+        if mode == "nearest":
+            assert max_abs(out, G[f"{key}_{mode}"]) == 0.0
+
+
+@pytest.mark.parametrize("mode", ["trilinear", "nearest"])
+def test_insert_into_image_3d_golden(device, mode):
+    G = golden("insert_into_image_3d")
+    for key, kk, shape, dt in (("r", "real", (12, 10, 9), torch.float32), ("c", "complex", (12, 10, 9), torch.complex64),
+                               ("m", "multi", (2, 12, 10, 9), torch.float32)):
+        img = torch.zeros(shape, dtype=dt, device=device)
+        img, w = tt.insert_into_image_3d(_t(G[f"val_{key}1"], device), _t(G["coords1"], device), img, interpolation=mode)
+        img2, w2 = tt.insert_into_image_3d(_t(G[f"val_{key}2"], device), _t(G["coords2"], device), img, w,
+                                           interpolation=mode)
+        assert img2 is img and w2 is w  # in place
+        assert rel_l2(_np(img), G[f"{kk}_{mode}_image"]) < INS_TOL
+        assert rel_l2(_np(w), G[f"{kk}_{mode}_weights"]) < INS_TOL
+
+
+def test_insert_into_image_known_answers(device):
+    img = torch.zeros(28, 28, 28, device=device)
+    img, w = tt.insert_into_image_3d(torch.tensor([5.0], device=device),
+                                     torch.tensor([[10.5, 14.5, 18.5]], device=device), img)
+    assert torch.allclose(img[10:12, 14:16, 18:20], torch.full((2, 2, 2), 5 / 8, device=device))
+    img = torch.zeros(28, 28, 28, device=device)
+    img, w = tt.insert_into_image_3d(torch.tensor([5.0], device=device),
+                                     torch.tensor([[10.7, 14.3, 18.9]], device=device), img, interpolation="nearest")
+    assert img[11, 14, 19] == 5 and img.sum() == 5
+    with pytest.raises(ValueError):
+        tt.insert_into_image_3d(torch.zeros(3, 4, 5, dtype=torch.complex128, device=device),
+                                torch.zeros(3, 4, 5, 3, device=device),
+                                torch.zeros(4, 4, 4, dtype=torch.complex64, device=device))
+    w64 = torch.zeros(4, 4, 4, dtype=torch.float64, device=device)
+    img = torch.zeros(4, 4, 4, dtype=torch.float64, device=device)
+    _, w64 = tt.insert_into_image_3d(torch.ones(3, dtype=torch.float64, device=device),
+                                     torch.full((3, 3), 1.25, device=device), img, w64)
+    assert w64.dtype == torch.float64 and abs(float(w64.sum()) - 3.0) < 1e-12
+
+
+# ------------------------------------------------------------- project / backproject
+def test_project_backproject_golden(device):
+    G = golden("project_backproject_24")
+    P = _np(tt.project_3d_to_2d(_t(G["volume"], device), _t(G["rotations"], device)))
+    assert rel_l2(P, G["projections"]) < EXT_TOL
+    B = _np(tt.backproject_2d_to_3d(_t(G["projections"], device), _t(G["rotations"], device)))
+    assert rel_l2(B, G["backprojection"]) < INS_TOL
+
+
+def test_project_rotation_centre(device):
+    vol = torch.zeros(32, 32, 32, device=device)
+    vol[16, 16, 16] = 1
+    P = tt.project_3d_to_2d(vol, uniform_rotations(100, 7, device=device))
+    idx = P.reshape(100, -1).argmax(dim=-1)
+    assert torch.all(idx == 16 * 32 + 16)
+
+
+def _fsc(a, b):
+    fa, fb = np.fft.fftshift(np.fft.fftn(a)), np.fft.fftshift(np.fft.fftn(b))
+    n = a.shape[0]
+    ax = np.arange(n) - n // 2
+    r = np.sqrt(sum(g ** 2 for g in np.meshgrid(ax, ax, ax, indexing="ij")))
+    shells = np.rint(r).astype(int)
+    out = []
+    for s in range(n // 2 + 1):
+        m = shells == s
+        num = np.real(np.sum(fa[m] * np.conj(fb[m])))
+        den = np.sqrt(np.sum(np.abs(fa[m]) ** 2) * np.sum(np.abs(fb[m]) ** 2))
+        out.append(num / den if den > 0 else 1.0)
+    return np.array(out)
+
+
+def test_projection_backprojection_cycle(device):
+    cube = torch.zeros(32, 32, 32)
+    cube[8:24, 8:24, 8:24] = 1
+    cube[16, 16, 16] = 32
+    R = uniform_rotations(1500, 8, device=device)
+    P = tt.project_3d_to_2d(cube.to(device), R)
+    rec = tt.backproject_2d_to_3d(P, R)
+    f = _fsc(cube.numpy(), _np(rec))
+    assert np.all(f[-10:] > 0.99)
+
+
+def test_backproject_dtypes_and_leading_dims(device):
+    imgs = torch.rand(10, 28, 28, dtype=torch.float64, device=device)
+    R = uniform_rotations(10, 9, device=device, dtype=torch.float64)
+    assert tt.backproject_2d_to_3d(imgs, R).dtype == torch.float64
+    cube = torch.zeros(32, 32, 32, device=device)
+    R = torch.rand(4, 5, 3, 3, device=device)
+    P = tt.project_3d_to_2d(cube, R)
+    assert P.shape == (4, 5, 32, 32)
+    assert tt.backproject_2d_to_3d(P, R).shape == (32, 32, 32)
+
+
+def test_multichannel_shapes(device):
+    vols = torch.rand(4, 10, 10, 10, device=device)
+    R = uniform_rotations(8, 10, device=device)
+    P = tt.project_3d_to_2d_multichannel(vols, R)
+    assert P.shape == (8, 4, 10, 10)
+    assert tt.backproject_2d_to_3d_multichannel(P, R).shape == (4, 10, 10, 10)
