@@ -3,16 +3,16 @@
 One step = extract 4096 central slices from a (256, 256, 129) complex64 rfft
 volume (the gather) and apply the adjoint to 4096 slices of N(0,1) complex64
 gradients (the volume gradient, i.e. the backward of the extraction), exactly
-as ``extract_central_slices_rfft_3d(...).backward(grad)`` runs it: zero the
-gradient volume, gather kernel, scatter kernel.  Inputs are resident in HBM
+as ``extract_central_slices_rfft_3d(...).backward(grad)`` runs it: gather
+kernel, then the scatter kernel writing the gradient volume.  Inputs are resident in HBM
 before timing; synthetic data (seeded blob volume, uniform SO(3) rotations).
 
 Prints ONE JSON line.  Besides the contract fields it reports the roofline of
 the dominant kernel (algorithmic bytes / HIP-event duration), the fused step's
-roofline, the plain-torch restatement of the reference on the same GPU, and
-the FP64 oracle timed on the host CPU on a bounded sample.
+roofline and the FP64 oracle timed on the host CPU on a bounded sample.  The
+plain-torch comparator on the same GPU is timed with --torch.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-torch] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu]
 """
 
 from __future__ import annotations
@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-torch", action="store_true", help="skip the plain-torch comparator")
+    ap.add_argument("--torch", action="store_true",
+                    help="also time the plain-torch comparator (elementwise coordinates, advanced-indexing "
+                         "gather, index_add_ adjoint) on the same GPU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--deterministic", action="store_true")
     return ap.parse_args()
@@ -60,7 +62,10 @@ def make_inputs(device):
 
 
 class Step:
-    """extract + adjoint through the C ABI, with HIP events around each kernel."""
+    """extract + adjoint through the C ABI, with HIP events around each kernel.
+
+    The adjoint writes the gradient volume (TT_FLAG_OVERWRITE), as the
+    autograd backward does, so no separate zero fill is needed."""
 
     def __init__(self, V, R, gS, deterministic=False):
         from teamtomo_amd import _lib
@@ -72,7 +77,7 @@ class Step:
         self.n = self.Rz.shape[0]
         self.out = torch.empty((self.n, N, N // 2 + 1), dtype=torch.complex64, device=V.device)
         self.gV = torch.empty_like(V)
-        self.det = int(deterministic)
+        self.flags = _lib.TT_FLAG_OVERWRITE | (_lib.TT_FLAG_DETERMINISTIC if deterministic else 0)
         self.stream = _lib.stream_of(V)
         self.events = []
 
@@ -86,9 +91,8 @@ class Step:
                L.ptr(self.out), self.stream)
         if timed:
             e[1].record(s)
-        self.gV.zero_()
         L.call("tt_extract_bwd_c64", L.ptr(self.gS), L.ptr(V), N, N, N // 2 + 1, N, N, N, L.ptr(self.Rz),
-               self.n, -1.0, 1e-8, L.ptr(self.gV), None, self.det, self.stream)
+               self.n, -1.0, 1e-8, L.ptr(self.gV), None, self.flags, self.stream)
         if timed:
             e[2].record(s)
             self.events.append(e)
@@ -103,71 +107,64 @@ def algorithmic_bytes():
 
 
 def _torch_coords(image_shape, R, device):
-    """Reference op sequence up to the sampling coordinates
-    (slice_extraction/_extract_central_slices_rfft_3d.py:147-210, _dft_utils.py:5-72)."""
+    """Sampling coordinates of the reference op sequence
+    (slice_extraction/_extract_central_slices_rfft_3d.py:147-210, _dft_utils.py:5-72),
+    in float32 with elementwise ops (the rotation is written out as products and sums)."""
     d, h, w = image_shape
-    R = R.to(torch.float32).clone()
-    R[torch.abs(R) < 1e-8] = 0.0
+    R = R.to(torch.float32)
+    R = torch.where(torch.abs(R) < 1e-8, torch.zeros((), device=device), R)
     R = torch.flip(R, dims=(-2, -1))
-    fy = torch.fft.fftshift(torch.fft.fftfreq(h, device=device))
-    fx = torch.fft.rfftfreq(w, device=device)
-    grid = torch.stack([torch.zeros(h, fx.shape[0], device=device), fy[:, None].expand(h, fx.shape[0]),
-                        fx[None, :].expand(h, fx.shape[0])], dim=-1).reshape(-1, 3)
-    rc = (R[:, None] @ grid[..., None])[..., 0]  # (n, b, 3)
-    conj = rc[..., 2] < -1e-8
-    near = torch.abs(rc[..., 2]) < 1e-8
-    r11, r12 = R[:, 2, 2][:, None].expand_as(near), R[:, 2, 1][:, None].expand_as(near)
+    fy = torch.fft.fftshift(torch.fft.fftfreq(h, device=device))[:, None]
+    fx = torch.fft.rfftfreq(w, device=device)[None, :]
+    q = [R[:, k, 1, None, None] * fy + R[:, k, 2, None, None] * fx for k in range(3)]
+    r11, r12 = R[:, 2, 2, None, None], R[:, 2, 1, None, None]
     tie = torch.where(torch.abs(r11) <= 1e-7, r12 < 0, r11 < 0)
-    conj = conj | (near & tie)
-    rc[conj] *= -1
-    shape = torch.tensor([d, h, w], device=device, dtype=torch.float32)
-    c = torch.empty_like(rc)
-    c[..., :2] = (rc[..., :2] + 0.5) / (1 - 1 / shape[:2]) * (shape[:2] - 1)
-    c[..., 2] = rc[..., 2] * 2 * (w // 2)
+    conj = (q[2] < -1e-8) | ((torch.abs(q[2]) < 1e-8) & tie)
+    sgn = 1.0 - 2.0 * conj.to(torch.float32)
+    c = torch.stack([(sgn * q[0] + 0.5) / (1 - 1 / d) * (d - 1), (sgn * q[1] + 0.5) / (1 - 1 / h) * (h - 1),
+                     sgn * q[2] * 2 * (w // 2)], dim=-1)
     return c, conj
 
 
-def torch_extract(V, image_shape, R):
-    """grid_sample on the stride-0 expanded volume (image_interpolation_3d.py:53-99)."""
-    c, conj = _torch_coords(image_shape, R, V.device)
-    n, b, _ = c.shape
-    cf = c.reshape(-1, 3)
-    img = torch.view_as_real(V).permute(3, 0, 1, 2)[None].expand(cf.shape[0], 2, *V.shape)
-    dims = torch.tensor(V.shape, device=V.device, dtype=torch.float32)
-    grid = torch.flip(cf / (0.5 * dims - 0.5) - 1, dims=(-1,)).reshape(-1, 1, 1, 1, 3)
-    s = torch.nn.functional.grid_sample(img, grid, mode="bilinear", padding_mode="border", align_corners=True)
-    s = torch.view_as_complex(s.reshape(-1, 2).contiguous())
-    inside = torch.all((cf >= 0) & (cf <= dims - 1), dim=-1)
-    s = s * inside
-    s = s.reshape(n, b)
-    s[conj] = torch.conj(s[conj])
-    return s.reshape(n, image_shape[1], image_shape[2] // 2 + 1)
-
-
-def torch_adjoint(gS, gV, image_shape, R):
-    """Adjoint as the reference's splat op sequence: index_put_(accumulate=True)
-    of w * conj-folded gradients at the 8 corners (insertion :289-346)."""
-    c, conj = _torch_coords(image_shape, R, gV.device)
-    g = gS.reshape(c.shape[0], -1).clone()
-    g[conj] = torch.conj(g[conj])
-    cf, g = c.reshape(-1, 3), g.reshape(-1)
-    dims = torch.tensor(gV.shape, device=gV.device, dtype=torch.float32)
-    keep = torch.all((cf >= 0) & (cf <= dims - 1), dim=-1)
-    cf, g = cf[keep], g[keep]
-    lo = torch.floor(cf)
-    t = cf - lo
+def _torch_corners(c, shape):
+    dims = torch.tensor(shape, device=c.device, dtype=torch.float32)
+    inside = torch.all((c >= 0) & (c <= dims - 1), dim=-1)
+    cc = torch.where(inside[..., None], c, torch.zeros((), device=c.device))
+    lo = torch.floor(cc)
+    t = cc - lo
     lo = lo.long()
     for q in range(8):
         dz, dy, dx = q >> 2, (q >> 1) & 1, q & 1
-        w = (t[:, 0] if dz else 1 - t[:, 0]) * (t[:, 1] if dy else 1 - t[:, 1]) * (t[:, 2] if dx else 1 - t[:, 2])
-        iz = torch.clamp(lo[:, 0] + dz, max=gV.shape[0] - 1)
-        iy = torch.clamp(lo[:, 1] + dy, max=gV.shape[1] - 1)
-        ix = torch.clamp(lo[:, 2] + dx, max=gV.shape[2] - 1)
-        gV.index_put_((iz, iy, ix), g * w, accumulate=True)
+        w = (t[..., 0] if dz else 1 - t[..., 0]) * (t[..., 1] if dy else 1 - t[..., 1]) * \
+            (t[..., 2] if dx else 1 - t[..., 2])
+        iz = torch.clamp(lo[..., 0] + dz, max=shape[0] - 1)
+        iy = torch.clamp(lo[..., 1] + dy, max=shape[1] - 1)
+        ix = torch.clamp(lo[..., 2] + dx, max=shape[2] - 1)
+        yield (iz * shape[1] + iy) * shape[2] + ix, w * inside
 
 
-def time_torch_comparator(V, R, gS, steps=1, chunk=512):
-    """Plain-torch restatement of the reference op sequence on the same GPU."""
+def torch_extract(V, image_shape, R):
+    """Plain-torch trilinear gather: 8 advanced-indexing corner reads per sample."""
+    c, conj = _torch_coords(image_shape, R, V.device)
+    Vf = V.reshape(-1)
+    s = torch.zeros(c.shape[:-1], dtype=V.dtype, device=V.device)
+    for idx, w in _torch_corners(c, V.shape):
+        s += w * Vf[idx]
+    return torch.where(conj, torch.conj(s), s)
+
+
+def torch_adjoint(gS, gV, image_shape, R):
+    """Plain-torch adjoint: index_add_ of w * conj-folded gradients at the 8 corners."""
+    c, conj = _torch_coords(image_shape, R, gV.device)
+    g = torch.where(conj, torch.conj(gS), gS)
+    gr = torch.view_as_real(g.resolve_conj()).reshape(-1, 2)
+    out = torch.view_as_real(gV).view(-1, 2)
+    for idx, w in _torch_corners(c, gV.shape):
+        out.index_add_(0, idx.reshape(-1), gr * w.reshape(-1, 1))
+
+
+def time_torch_comparator(V, R, gS, steps=1, chunk=256):
+    """Plain-torch extract + adjoint on the same GPU, chunked over rotations."""
 
     def step():
         out = []
@@ -187,20 +184,23 @@ def time_torch_comparator(V, R, gS, steps=1, chunk=512):
     return N_ROT / dt, dt * 1e3, chunk
 
 
-def cpu_baseline(V, R, gS, n_rot=4):
-    """FP64 oracle (NumPy) on the host, extract + adjoint of n_rot slices at 256^3."""
+def cpu_baseline(V, R, gS, n_rot=64):
+    """FP64 C oracle (OpenMP over rotations) on the host CPU: extract + adjoint of
+    a bounded sample of n_rot of the rotations at 256^3, reported per slice."""
+    from oracle import c_oracle as C
     from oracle import fourier_slice as O
 
     Vn = V.cpu().numpy().astype(np.complex128)
     Rn = O.prepare_rotations(R[:n_rot].cpu().numpy(), False, 1e-8)
     g = gS[:n_rot].cpu().numpy()
+    C.extract(Vn, (N, N, N), Rn[:2])  # load + first-touch outside the timing
     t0 = time.perf_counter()
-    O.extract_central_slices(Vn, (N, N, N), Rn)
-    O.extract_adjoint(g, Vn.shape, (N, N, N), Rn)
+    C.extract(Vn, (N, N, N), Rn)
+    C.extract_adjoint(g, Vn.shape, (N, N, N), Rn)
     dt = time.perf_counter() - t0
-    return {"value": n_rot / dt, "unit": "slices/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 NumPy oracle, extract+adjoint of {n_rot} of the {N_ROT} rotations at {N}^3, "
-                      f"{dt:.1f} s"}
+    return {"value": n_rot / dt, "unit": "slices/s", "cores": C.threads(), "kind": "port",
+            "sample": f"FP64 C oracle (oracle/c/tt_oracle.c, OpenMP), extract+adjoint of {n_rot} of the "
+                      f"{N_ROT} rotations at {N}^3 in {dt:.1f} s"}
 
 
 def main():
@@ -235,7 +235,7 @@ def main():
     ext_b, adj_b = algorithmic_bytes()
     kernels = {
         "extract_fwd_kernel": {"ms": ext_ms, "bytes": ext_b, "GBs": ext_b / ext_ms / 1e6},
-        "adjoint (zero + scatter)": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6},
+        "brick_scatter_kernel (adjoint)": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["GBs"]
@@ -272,7 +272,7 @@ def main():
         "kernels": kernels,
         "wall_s": wall,
     }
-    if not args.no_torch:
+    if args.torch:
         try:
             tv, tms, chunk = time_torch_comparator(V, R, gS)
             line["torch_same_gpu"] = {"value": tv, "unit": "slices/s", "ms_per_step": tms,

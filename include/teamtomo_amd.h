@@ -36,6 +36,11 @@ extern "C" {
 #define TT_INTERP_NEAREST 0
 #define TT_INTERP_TRILINEAR 1
 
+/* `flags` bits of the scatter entry points */
+#define TT_FLAG_DETERMINISTIC 1 /* fixed-point accumulation: bitwise reproducible      */
+#define TT_FLAG_OVERWRITE 2     /* write the outputs instead of adding to them          */
+#define TT_FLAG_PIXEL_ATOMIC 4  /* force the pixel-driven global-atomic scatter         */
+
 /* Message for the last error on the calling thread ("" if none). */
 const char *tt_last_error(void);
 
@@ -60,12 +65,13 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr,
 
 /*
  * Backward of the extraction: the adjoint scatter into grad_volume
- * (accumulated, complex64 (vol_d, vol_h, vol_wr)) and/or the analytic
- * rotation gradient (accumulated into grad_rot, float64 (n_rot, 3, 3) zyx).
- * Either output may be NULL.  `volume` is read only when grad_rot != NULL.
+ * (complex64 (vol_d, vol_h, vol_wr); accumulated, or written with
+ * TT_FLAG_OVERWRITE) and/or the analytic rotation gradient (accumulated into
+ * grad_rot, float64 (n_rot, 3, 3) zyx).  Either output may be NULL.
+ * `volume` is read only when grad_rot != NULL.
  * Replaces torch autograd through grid_sample (the grid_sampler_3d_backward
  * reached from .../image_interpolation_3d.py:69-75).
- * deterministic != 0 selects the fixed-order scatter (bitwise reproducible).
+ * flags: TT_FLAG_* bits.
  */
 int tt_extract_bwd_c64(const void *grad_slices, const void *volume,
                        int vol_d, int vol_h, int vol_wr,
@@ -73,11 +79,12 @@ int tt_extract_bwd_c64(const void *grad_slices, const void *volume,
                        const float *rot_zyx, int64_t n_rot,
                        double fftfreq_max, double x_tolerance,
                        void *grad_volume, double *grad_rot,
-                       int deterministic, void *stream);
+                       int flags, void *stream);
 
 /*
- * Central-slice insertion (the scatter), accumulating into a volume and a
- * weights volume of shape (vol_d, vol_h, vol_w/2+1).
+ * Central-slice insertion (the scatter) into a volume and a weights volume
+ * of shape (vol_d, vol_h, vol_w/2+1), accumulated (or written with
+ * TT_FLAG_OVERWRITE).
  * Replaces insert_central_slices_rfft_3d and _insert_into_3d_dft_linear
  * (torch-fourier-slice/.../slice_insertion/_insert_central_slices_rfft_3d.py:9-106,289-406),
  * including the x=0-plane Friedel mirror.
@@ -87,10 +94,10 @@ int tt_extract_bwd_c64(const void *grad_slices, const void *volume,
  */
 int tt_insert_c64(const void *slices, int vol_d, int vol_h, int vol_w,
                   const float *rot_zyx, int64_t n_rot, double fftfreq_max,
-                  void *volume, float *weights, int deterministic, void *stream);
+                  void *volume, float *weights, int flags, void *stream);
 int tt_insert_c128(const void *slices, int vol_d, int vol_h, int vol_w,
                    const float *rot_zyx, int64_t n_rot, double fftfreq_max,
-                   void *volume, double *weights, int deterministic, void *stream);
+                   void *volume, double *weights, int flags, void *stream);
 
 /*
  * Backward of the insertion: the gradient w.r.t. the slices (the adjoint

@@ -12,7 +12,7 @@ from __future__ import annotations
 import sys
 import types
 
-from ._config import deterministic_enabled, set_deterministic
+from ._config import deterministic_enabled, set_deterministic, set_pixel_atomic
 from .fourier_slice import (
     backproject_2d_to_3d,
     backproject_2d_to_3d_multichannel,
@@ -38,6 +38,7 @@ __all__ = [
     "insert_into_image_3d",
     "set_deterministic",
     "deterministic_enabled",
+    "set_pixel_atomic",
     "install_dropin",
 ]
 

@@ -17,10 +17,14 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
+
+TT_FLAG_DETERMINISTIC = 1
+TT_FLAG_OVERWRITE = 2
+TT_FLAG_PIXEL_ATOMIC = 4
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int

@@ -112,28 +112,54 @@ Plan make_plan(const SliceGeom &g) {
   return p;
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void zero_kernel(T *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = (T)0;
+}
+
 }  // namespace
 
+template <typename T> int zero_fill(T *p, int64_t n, hipStream_t s) {
+  if (n <= 0) return TT_OK;
+  int grid = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(zero_kernel<T>, dim3(grid), dim3(kBlock), 0, s, p, n);
+  return check_launch("zero_kernel");
+}
+template int zero_fill<float>(float *, int64_t, hipStream_t);
+template int zero_fill<double>(double *, int64_t, hipStream_t);
+
 int scatter_extract_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot,
-                            f2v *gV, int deterministic, hipStream_t s) {
-  if (deterministic) return fail_invalid("deterministic scatter is not available in this build");
+                            f2v *gV, int flags, hipStream_t s) {
+  const bool det = flags & TT_FLAG_DETERMINISTIC, overwrite = flags & TT_FLAG_OVERWRITE;
+  if (!(flags & TT_FLAG_PIXEL_ATOMIC))
+    return brick_scatter_adjoint(g, gS, rot, n_rot, gV, det, !overwrite, s);
+  if (det) return fail_invalid("the pixel-atomic scatter is not deterministic");
+  int st;
+  if (overwrite && (st = zero_fill<float>((float *)gV, 2 * (int64_t)g.vD * g.vH * g.vW, s))) return st;
   Plan pl = make_plan(g);
   for (int64_t r0 = 0; r0 < n_rot; r0 += pl.rots_per_launch) {
     int64_t nr = std::min(pl.rots_per_launch, n_rot - r0);
     int64_t nb = (int64_t)pl.blocks_per_rot * nr, nbp = (nb + 7) & ~(int64_t)7;
     hipLaunchKernelGGL(adjoint_atomic_kernel, dim3((unsigned)nbp), dim3(kBlock), 0, s,
                        gS + r0 * g.P, g, rot + r0 * 9, pl.blocks_per_rot, nb, nbp, gV);
-    int st = check_launch("adjoint_atomic_kernel");
-    if (st) return st;
+    if ((st = check_launch("adjoint_atomic_kernel"))) return st;
   }
   return TT_OK;
 }
 
 template <typename T>
 int scatter_insert(const SliceGeom &g, const typename Cx<T>::type *slices, const float *rot,
-                   int64_t n_rot, typename Cx<T>::type *vol, T *w, int deterministic,
-                   hipStream_t s) {
-  if (deterministic) return fail_invalid("deterministic scatter is not available in this build");
+                   int64_t n_rot, typename Cx<T>::type *vol, T *w, int flags, hipStream_t s) {
+  const bool det = flags & TT_FLAG_DETERMINISTIC, overwrite = flags & TT_FLAG_OVERWRITE;
+  if (sizeof(T) == sizeof(float) && !(flags & TT_FLAG_PIXEL_ATOMIC))
+    return brick_scatter_insert(g, (const f2v *)slices, rot, n_rot, (f2v *)vol, (float *)w, det,
+                                !overwrite, s);
+  if (det) return fail_invalid("deterministic insertion is available for complex64 only");
+  int st;
+  const int64_t nv = (int64_t)g.vD * g.vH * g.vW;
+  if (overwrite && ((st = zero_fill<T>((T *)vol, 2 * nv, s)) || (st = zero_fill<T>(w, nv, s))))
+    return st;
   Plan pl = make_plan(g);
   for (int64_t r0 = 0; r0 < n_rot; r0 += pl.rots_per_launch) {
     int64_t nr = std::min(pl.rots_per_launch, n_rot - r0);
@@ -141,8 +167,7 @@ int scatter_insert(const SliceGeom &g, const typename Cx<T>::type *slices, const
     hipLaunchKernelGGL((insert_atomic_kernel<T>), dim3((unsigned)nbp), dim3(kBlock), 0, s,
                        slices + r0 * g.P, g, rot + r0 * 9, pl.blocks_per_rot, nb, nbp,
                        reinterpret_cast<T *>(vol), w);
-    int st = check_launch("insert_atomic_kernel");
-    if (st) return st;
+    if ((st = check_launch("insert_atomic_kernel"))) return st;
   }
   return TT_OK;
 }

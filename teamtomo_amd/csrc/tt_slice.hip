@@ -350,7 +350,7 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 1; }
+int tt_abi_version(void) { return 2; }
 
 int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img_d, int img_h,
                    int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,
@@ -377,7 +377,7 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img
 int tt_extract_bwd_c64(const void *grad_slices, const void *volume, int vol_d, int vol_h,
                        int vol_wr, int img_d, int img_h, int img_w, const float *rot_zyx,
                        int64_t n_rot, double fftfreq_max, double x_tolerance, void *grad_volume,
-                       double *grad_rot, int deterministic, void *stream) {
+                       double *grad_rot, int flags, void *stream) {
   int st = check_common(vol_d, vol_h, vol_wr, img_d, img_h, img_w, n_rot, rot_zyx);
   if (st) return st;
   if (n_rot == 0 || (!grad_volume && !grad_rot)) return TT_OK;
@@ -387,7 +387,7 @@ int tt_extract_bwd_c64(const void *grad_slices, const void *volume, int vol_d, i
   hipStream_t s = (hipStream_t)stream;
   if (grad_volume) {
     st = scatter_extract_adjoint(g, (const f2v *)grad_slices, rot_zyx, n_rot, (f2v *)grad_volume,
-                                 deterministic, s);
+                                 flags, s);
     if (st) return st;
   }
   if (grad_rot) {
@@ -411,7 +411,7 @@ int tt_extract_bwd_c64(const void *grad_slices, const void *volume, int vol_d, i
 template <typename T>
 static int insert_impl(const void *slices, int vol_d, int vol_h, int vol_w, const float *rot_zyx,
                        int64_t n_rot, double fftfreq_max, void *volume, T *weights,
-                       int deterministic, void *stream) {
+                       int flags, void *stream) {
   int vol_wr = vol_w / 2 + 1;
   int st = check_common(vol_d, vol_h, vol_wr, vol_d, vol_h, vol_w, n_rot, rot_zyx);
   if (st) return st;
@@ -419,7 +419,7 @@ static int insert_impl(const void *slices, int vol_d, int vol_h, int vol_w, cons
   if (!slices || !volume || !weights) return fail_invalid("slices / volume / weights pointer is NULL");
   SliceGeom g = make_geom(vol_d, vol_h, vol_w, vol_d, vol_h, vol_wr, fftfreq_max, 1e-8);
   return scatter_insert<T>(g, (const typename Cx<T>::type *)slices, rot_zyx, n_rot,
-                           (typename Cx<T>::type *)volume, weights, deterministic,
+                           (typename Cx<T>::type *)volume, weights, flags,
                            (hipStream_t)stream);
 }
 
@@ -427,16 +427,16 @@ extern "C" {
 
 int tt_insert_c64(const void *slices, int vol_d, int vol_h, int vol_w, const float *rot_zyx,
                   int64_t n_rot, double fftfreq_max, void *volume, float *weights,
-                  int deterministic, void *stream) {
+                  int flags, void *stream) {
   return insert_impl<float>(slices, vol_d, vol_h, vol_w, rot_zyx, n_rot, fftfreq_max, volume,
-                            weights, deterministic, stream);
+                            weights, flags, stream);
 }
 
 int tt_insert_c128(const void *slices, int vol_d, int vol_h, int vol_w, const float *rot_zyx,
                    int64_t n_rot, double fftfreq_max, void *volume, double *weights,
-                   int deterministic, void *stream) {
+                   int flags, void *stream) {
   return insert_impl<double>(slices, vol_d, vol_h, vol_w, rot_zyx, n_rot, fftfreq_max, volume,
-                             weights, deterministic, stream);
+                             weights, flags, stream);
 }
 
 }  // extern "C"

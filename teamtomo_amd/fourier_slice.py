@@ -24,7 +24,7 @@ from __future__ import annotations
 import torch
 import torch.nn.functional as F
 
-from . import _lib
+from . import _config, _lib
 from ._config import deterministic_enabled
 
 __all__ = [
@@ -45,6 +45,16 @@ def _rfft_width(w: int) -> int:
 
 def _fmax_arg(fftfreq_max) -> float:
     return -1.0 if fftfreq_max is None else float(fftfreq_max)
+
+
+def _scatter_flags() -> int:
+    """Scatter outputs are freshly allocated, so they are written, not accumulated."""
+    flags = _lib.TT_FLAG_OVERWRITE
+    if deterministic_enabled():
+        flags |= _lib.TT_FLAG_DETERMINISTIC
+    if _config.pixel_atomic_enabled():
+        flags |= _lib.TT_FLAG_PIXEL_ATOMIC
+    return flags
 
 
 def _prepare_rotations(rotation_matrices, device, zyx_matrices, rot_tolerance):
@@ -87,12 +97,12 @@ class _ExtractFn(torch.autograd.Function):
             return None, None, None, None, None
         g = grad_slices.to(torch.complex64).contiguous()
         D, H, Wr = volume.shape
-        gV = torch.zeros_like(volume) if need_v else None
+        gV = torch.empty_like(volume) if need_v else None
         gR = torch.zeros((R.shape[0], 3, 3), dtype=torch.float64, device=R.device) if need_r else None
         _lib.call(
             "tt_extract_bwd_c64", _lib.ptr(g), _lib.ptr(volume), D, H, Wr, img_d, img_h, img_w,
             _lib.ptr(R), R.shape[0], _fmax_arg(fftfreq_max), float(x_tolerance), _lib.ptr(gV),
-            _lib.ptr(gR), int(deterministic_enabled()), _lib.stream_of(volume),
+            _lib.ptr(gR), _scatter_flags(), _lib.stream_of(volume),
         )
         return gV, (gR.to(R.dtype) if gR is not None else None), None, None, None
 
@@ -160,11 +170,11 @@ class _InsertFn(torch.autograd.Function):
         Wr = _rfft_width(W)
         n = R.shape[0]
         c64 = slices.dtype == torch.complex64
-        vol = torch.zeros((D, H, Wr), dtype=slices.dtype, device=slices.device)
-        wts = torch.zeros((D, H, Wr), dtype=torch.float32 if c64 else torch.float64, device=slices.device)
+        vol = torch.empty((D, H, Wr), dtype=slices.dtype, device=slices.device)
+        wts = torch.empty((D, H, Wr), dtype=torch.float32 if c64 else torch.float64, device=slices.device)
         _lib.call(
             "tt_insert_c64" if c64 else "tt_insert_c128", _lib.ptr(slices), D, H, W, _lib.ptr(R), n,
-            _fmax_arg(fftfreq_max), _lib.ptr(vol), _lib.ptr(wts), int(deterministic_enabled()),
+            _fmax_arg(fftfreq_max), _lib.ptr(vol), _lib.ptr(wts), _scatter_flags(),
             _lib.stream_of(slices),
         )
         ctx.save_for_backward(slices, R)

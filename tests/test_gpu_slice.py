@@ -324,3 +324,82 @@ def test_multichannel_shapes(device):
     P = tt.project_3d_to_2d_multichannel(vols, R)
     assert P.shape == (8, 4, 10, 10)
     assert tt.backproject_2d_to_3d_multichannel(P, R).shape == (4, 10, 10, 10)
+
+
+# ------------------------------------------------------ scatter variants agree
+@pytest.mark.parametrize("shape,image_shape,fmax", [
+    ((48, 48, 48), (48, 48, 48), None), ((33, 33, 33), (33, 33, 33), 0.3), ((20, 20, 20), (64, 64, 64), None),
+])
+def test_brick_scatter_matches_pixel_atomic(device, shape, image_shape, fmax):
+    """Brick-stationary LDS scatter == pixel-driven atomic scatter == FP64 oracle,
+    for the adjoint and the insertion, incl. odd sizes and image_shape != volume."""
+    n = 40
+    R = uniform_rotations(n, 11, device=device)
+    Vr = volume_rfft(torch.randn(*shape, generator=torch.Generator().manual_seed(12))).to(device)
+    Hi, Wri = image_shape[1], image_shape[2] // 2 + 1
+    g = torch.randn(n, Hi, Wri, dtype=torch.complex64, generator=torch.Generator().manual_seed(13)).to(device)
+    outs = []
+    for atomic in (False, True):
+        tt.set_pixel_atomic(atomic)
+        try:
+            V = Vr.clone().requires_grad_(True)
+            tt.extract_central_slices_rfft_3d(V, image_shape, R, fftfreq_max=fmax).backward(g)
+            outs.append(_np(V.grad))
+        finally:
+            tt.set_pixel_atomic(False)
+    ref = O.extract_adjoint(_np(g), tuple(Vr.shape), image_shape, O.prepare_rotations(_np(R), False, 1e-8), fmax)
+    assert rel_l2(outs[0], ref) < INS_TOL and rel_l2(outs[1], ref) < INS_TOL
+    assert rel_l2(outs[0], outs[1]) < INS_TOL
+    if image_shape == shape:
+        d = torch.randn(n, shape[1], shape[2] // 2 + 1, dtype=torch.complex64,
+                        generator=torch.Generator().manual_seed(14)).to(device)
+        res = []
+        for atomic in (False, True):
+            tt.set_pixel_atomic(atomic)
+            try:
+                res.append([_np(x) for x in tt.insert_central_slices_rfft_3d(d, shape, R, fftfreq_max=fmax)])
+            finally:
+                tt.set_pixel_atomic(False)
+        rV, rW = O.insert_central_slices(_np(d), shape, O.prepare_rotations(_np(R), False, None), fmax)
+        for Vi, Wi in res:
+            assert rel_l2(Vi, rV) < INS_TOL and rel_l2(Wi, rW) < INS_TOL
+
+
+def test_deterministic_scatter_is_bitwise_reproducible(device):
+    n = 64
+    R = uniform_rotations(n, 15, device=device)
+    V0 = volume_rfft(torch.randn(40, 40, 40, generator=torch.Generator().manual_seed(16))).to(device)
+    g = torch.randn(n, 40, 21, dtype=torch.complex64, generator=torch.Generator().manual_seed(17)).to(device)
+    tt.set_deterministic(True)
+    try:
+        grads, ins = [], []
+        for _ in range(3):
+            V = V0.clone().requires_grad_(True)
+            tt.extract_central_slices_rfft_3d(V, (40, 40, 40), R).backward(g)
+            grads.append(V.grad.clone())
+            ins.append(tt.insert_central_slices_rfft_3d(g, (40, 40, 40), R))
+    finally:
+        tt.set_deterministic(False)
+    assert all(torch.equal(grads[0], x) for x in grads[1:])
+    assert all(torch.equal(ins[0][0], x[0]) and torch.equal(ins[0][1], x[1]) for x in ins[1:])
+    ref = O.extract_adjoint(_np(g), (40, 40, 21), (40, 40, 40), O.prepare_rotations(_np(R), False, 1e-8))
+    assert rel_l2(_np(grads[0]), ref) < 1e-5
+    rV, rW = O.insert_central_slices(_np(g), (40, 40, 40), O.prepare_rotations(_np(R), False, None))
+    assert rel_l2(_np(ins[0][0]), rV) < 1e-5 and rel_l2(_np(ins[0][1]), rW) < 1e-5
+
+
+def test_scatter_accumulate_flag(device):
+    """Without TT_FLAG_OVERWRITE the C ABI adds into the caller's buffers."""
+    from teamtomo_amd import _lib
+
+    n = 8
+    R = uniform_rotations(n, 18, device=device)
+    Rz = torch.flip(R, dims=(-2, -1)).contiguous()
+    d = torch.randn(n, 24, 13, dtype=torch.complex64, generator=torch.Generator().manual_seed(19)).to(device)
+    base_v = torch.randn(24, 24, 13, dtype=torch.complex64, generator=torch.Generator().manual_seed(20)).to(device)
+    base_w = torch.rand(24, 24, 13, generator=torch.Generator().manual_seed(21)).to(device)
+    vol, w = base_v.clone(), base_w.clone()
+    _lib.call("tt_insert_c64", _lib.ptr(d), 24, 24, 24, _lib.ptr(Rz), n, -1.0, _lib.ptr(vol), _lib.ptr(w), 0,
+              _lib.stream_of(d))
+    Vi, Wi = tt.insert_central_slices_rfft_3d(d, (24, 24, 24), R)
+    assert rel_l2(_np(vol), _np(base_v + Vi)) < 1e-6 and rel_l2(_np(w), _np(base_w + Wi)) < 1e-6

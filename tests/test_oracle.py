@@ -342,3 +342,39 @@ def test_project_errors():
         O.project_3d_to_2d(np.zeros((4, 4, 5)), np.eye(3)[None])
     with pytest.raises(ValueError):
         O.backproject_2d_to_3d(np.zeros((1, 4, 5)), np.eye(3)[None])
+
+
+# ------------------------------------------------------------- the C restatement
+@pytest.fixture(scope="module")
+def C():
+    from oracle import c_oracle
+
+    c_oracle.build()
+    return c_oracle
+
+
+@pytest.mark.parametrize("name,shape,fmax", [
+    ("extract_insert_64", (64, 64, 64), None), ("odd_33", (33, 33, 33), None), ("fftfreq_max_03", (32, 32, 32), 0.3),
+])
+def test_c_oracle_matches_numpy_oracle(C, name, shape, fmax):
+    G = golden(name)
+    Rz = O.prepare_rotations(G["rotations"], False, 1e-8)
+    S = C.extract(G["volume_rfft"], shape, Rz, fmax)
+    assert rel_l2(S, O.extract_central_slices(G["volume_rfft"], shape, Rz, fmax)) < 1e-14
+    assert rel_l2(S, G["slices"]) < GOLD_TOL
+    gA = C.extract_adjoint(G["slices"], G["volume_rfft"].shape, shape, Rz, fmax)
+    assert rel_l2(gA, O.extract_adjoint(G["slices"], G["volume_rfft"].shape, shape, Rz, fmax)) < 1e-14
+    Ri = O.prepare_rotations(G["rotations"], False, None)
+    V, W = C.insert(G["slices"], shape, Ri, fmax)
+    rV, rW = O.insert_central_slices(G["slices"], shape, Ri, fmax)
+    assert rel_l2(V, rV) < 1e-14 and rel_l2(W, rW) < 1e-14
+    assert rel_l2(V, G["ins_volume"]) < GOLD_TOL
+
+
+def test_c_oracle_image_shape_and_friedel(C):
+    G = golden("image_shape_4x")
+    S = C.extract(G["volume_rfft"], (64, 64, 64), O.prepare_rotations(G["rotations"], False, 1e-8))
+    assert rel_l2(S, G["slices"]) < GOLD_TOL
+    G = golden("tiebreak_32")
+    V, W = C.insert(G["friedel_slices"], (32, 32, 32), O.prepare_rotations(G["friedel_rotations"], False, None))
+    assert rel_l2(V, G["friedel_volume"]) < GOLD_TOL and max_abs(W, G["friedel_weights"]) < 1e-5
