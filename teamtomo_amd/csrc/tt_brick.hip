@@ -1,32 +1,43 @@
 // Brick-stationary scatter for gfx950: the extraction adjoint and the
-// central-slice insertion without global atomics.
+// central-slice insertion, without global float atomics.
 //
 // Each workgroup owns one BZ x BY x BX brick of the output volume and keeps
-// its accumulator in LDS.  It (1) tests every rotation's slice plane against
-// the brick (one lane per rotation), (2) for each crossing (rotation, Friedel
-// half) enumerates exactly the slice pixels whose trilinear footprint
-// overlaps the brick -- one lane per slice row computes the row's pixel
-// interval, a wave prefix sum flattens the rows and every lane takes one
-// pixel -- and adds w * value into LDS with ds_add_f32, and (3) writes each
-// voxel once.  A pixel whose footprint straddles a brick face is visited by
-// both bricks, each adding only its own corners, so every voxel sees exactly
-// the contributions of the pixel-driven scatter.
+// its accumulator in LDS as 64-bit fixed point: every contribution w * v is
+// scaled by 2^k (k from max|v|, so |w v 2^k| <= 2^30), rounded to an integer
+// and added with ds_add_u64.  Integer adds commute, so the result is bitwise
+// reproducible run to run, and integer LDS atomics run at close to LDS
+// bandwidth on CDNA4 whereas ds_add_f32 is many times slower.
+//
+// Work of one block:
+//  (1) one lane per rotation tests the slice plane against the brick's
+//      footprint box and, for each crossing (rotation, Friedel half), records
+//      the range of slice rows whose pixels can land in the box;
+//  (2) the items' rows are pooled: one lane per row (rows of several items
+//      share a wave) computes the row's pixel interval from three slab
+//      constraints in brick-local FP32 coordinates, taken relative to the
+//      item's pixel nearest the brick so every offset stays small; rows are
+//      cut into chunks of 4 pixels and a wave prefix sum hands one chunk to
+//      each lane;
+//  (3) a lane walks its chunk (c = c_row + q * s * B); pixels within 1e-3
+//      voxel of a volume face or of the x'=0 Friedel plane take the exact
+//      FP64 path (the snapped coordinates of the gather), the rest go
+//      straight to the 8-corner splat;
+//  (4) the brick is written once.  Bricks in the DC zone, which every slice
+//      crosses, are split over rotation segments; their partial sums meet in
+//      a 64-bit global scratch and a combine pass writes them.
 //
 // Insertion semantics (packages/primitives/torch-fourier-slice/src/
 // torch_fourier_slice/slice_insertion/_insert_central_slices_rfft_3d.py):
 // the direct splat goes to the bricks; the x=0-plane Friedel mirror (:348-404)
-// is applied afterwards from a scratch copy of the plane:
+// is applied afterwards from a copy of the plane:
 //   V[z,y,0] = P[z,y] + conj(P[mz,my])   unless (mz,my) is DC,
 // with (mz,my) = ((D-z)%D, (H-y)%H), because the mirror of every x=0 corner
 // other than DC is added conjugated at that point.
-//
-// Deterministic mode accumulates in 64-bit fixed point (integer adds commute,
-// so the result is bitwise reproducible); the scale 2^k is chosen on the
-// device from max|value| so that no partial sum can overflow.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <string>
 
 #include "../../include/teamtomo_amd.h"
 #include "tt_common.h"
@@ -40,7 +51,12 @@ constexpr int kT = 256;
 constexpr int kWaves = kT / 64;
 constexpr int kRotChunk = kT;
 constexpr int kItemsMax = 2 * kRotChunk;
-constexpr double kMargin = 1e-6;
+constexpr int kChunk = 4;          // pixels per lane-chunk
+constexpr double kMargin = 1e-6;   // slack of the row-interval bounds
+constexpr float kGuard = 1e-3f;    // FP32 guard band (voxels) -> exact FP64 path
+constexpr float kWeightScale = 1073741824.0f;  // 2^30
+
+typedef unsigned long long u64;
 
 __device__ __forceinline__ int64_t logical_block_b(int64_t nb_padded) {
   int64_t b = blockIdx.x;
@@ -48,24 +64,23 @@ __device__ __forceinline__ int64_t logical_block_b(int64_t nb_padded) {
   return (b & 7) * per + (b >> 3);
 }
 
-// Accumulator element for float (fast) and fixed-point (deterministic) modes.
-template <bool DET> struct Acc;
-template <> struct Acc<false> {
-  typedef float T;
-  __device__ static void add(T *p, float v, double) { atomicAdd(p, v); }
-  __device__ static float get(T v, double) { return v; }
-};
-template <> struct Acc<true> {
-  typedef unsigned long long T;
-  __device__ static void add(T *p, float v, double scale) {
-    long long q = (long long)rint((double)v * scale);
-    atomicAdd(p, (unsigned long long)q);
-  }
-  __device__ static float get(T v, double scale) { return (float)((double)(long long)v / scale); }
+// LDS layout of a brick accumulator: row pitch BX+1, plane pitch BY*(BX+1)+1.
+template <int BZ, int BY, int BX> struct Lay {
+  static constexpr int PY = BX + 1;
+  static constexpr int PZ = BY * PY + 1;
+  static constexpr int SIZE = BZ * PZ;
+  __device__ static int at(int z, int y, int x) { return z * PZ + y * PY + x; }
 };
 
-// Per-rotation geometry in array-coordinate space: c = beta + s (A m + B j)
-// with m = i - H/2.
+// Brick grid; bricks of the DC zone are split into rotation segments.
+struct BrickGrid {
+  int nbz, nby, nbx, n_bricks;
+  int zb0, zb1, yb0, yb1, xb1;  // zone: bz in [zb0,zb1], by in [yb0,yb1], bx in [0,xb1]
+  int nzy, nzx, n_zone;         // zone extents (bricks)
+  int nseg;                     // segments per zone brick
+  int n_units, n_units_padded;
+};
+
 struct PlaneGeom {
   double A[3], B[3];
 };
@@ -81,88 +96,123 @@ __device__ __forceinline__ PlaneGeom plane_geom(const SliceGeom &g, const float 
   return p;
 }
 
-// Scale exponent for fixed-point accumulation: every partial sum is bounded
-// by max|v| * n_rot * per_slice (per_slice bounds the summed weights one
-// slice can put on one voxel), kept below 2^62.
-__device__ __forceinline__ double fixed_scale(float maxabs, double bound) {
-  double m = (double)maxabs * bound;
-  if (!(m > 0.0)) return 1.0;
-  int e;
-  frexp(m, &e);  // m < 2^e
-  return ldexp(1.0, 62 - e);
+__device__ __forceinline__ void lds_add(u64 *p, float x) {
+  atomicAdd(p, (u64)(long long)__float2int_rn(x));
 }
 
-template <int BZ, int BY, int BX, bool INSERT, bool DET>
-__global__ __launch_bounds__(kT) void brick_scatter_kernel(
-    const f2v *__restrict__ src, SliceGeom g, const float *__restrict__ rot, int n_rot, int nby,
-    int nbx, int n_bricks, int nb_padded, const float *__restrict__ maxabs, double bound_v,
-    double bound_w, f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
-    float *__restrict__ wplane, int accumulate) {
-  typedef Acc<DET> A;
-  constexpr int NV = BZ * BY * BX;
-  __shared__ typename A::T s_re[NV];
-  __shared__ typename A::T s_im[NV];
-  __shared__ typename A::T s_w[INSERT ? NV : 1];
-  __shared__ int s_item[kItemsMax];
-  __shared__ int s_m0[kItemsMax];
-  __shared__ int s_m1[kItemsMax];
-  __shared__ int s_nitems;
+// Fixed-point value scale 2^(30-e) with max|v| < 2^e, so |w v scale| <= 2^30.
+__device__ __forceinline__ float value_scale(const float *maxabs) {
+  const float m = *maxabs;
+  if (!(m > 0.0f) || !(m < 3.0e38f)) return 1.0f;
+  int e;
+  frexpf(m, &e);
+  return ldexpf(1.0f, 30 - e);
+}
 
-  const int64_t lb = logical_block_b(nb_padded);
-  if (lb >= n_bricks) return;
+template <int BZ, int BY, int BX, bool INSERT>
+__global__ __launch_bounds__(kT) void brick_scatter_kernel(
+    const f2v *__restrict__ src, SliceGeom g, const float *__restrict__ rot, int n_rot,
+    BrickGrid bg, const float *__restrict__ maxabs, f2v *__restrict__ out,
+    float *__restrict__ wout, f2v *__restrict__ plane, float *__restrict__ wplane,
+    u64 *__restrict__ zone_acc, int accumulate) {
+  typedef Lay<BZ, BY, BX> L;
+  constexpr int NV = BZ * BY * BX;
+  constexpr int NS = L::SIZE;
+  __shared__ u64 s_re[NS];
+  __shared__ u64 s_im[NS];
+  __shared__ u64 s_w[INSERT ? NS : 1];
+  // per item (rotation, Friedel half) crossing the brick
+  __shared__ int s_code[kItemsMax];     // 2 * rotation + (half == -1)
+  __shared__ short s_m0[kItemsMax];     // first slice row (m = i - H/2)
+  __shared__ short s_mc[kItemsMax];     // reference pixel (row m, column j)
+  __shared__ short s_jc[kItemsMax];
+  __shared__ int s_rowend[kItemsMax];   // inclusive prefix of row counts
+  __shared__ float s_geo[kItemsMax][9]; // c(ref) - lo, s*A, s*B  (brick-local FP32)
+  __shared__ int s_nitems;
+  __shared__ int s_wsum[kWaves];
+
+  const int64_t u = logical_block_b(bg.n_units_padded);
+  if (u >= bg.n_units) return;
+  // unit -> (brick, rotation segment)
+  int bz, by, bx, seg = 0, nseg = 1;
+  bool in_zone;
+  if (u < bg.n_bricks) {
+    bz = (int)(u / ((int64_t)bg.nby * bg.nbx));
+    const int rem = (int)(u - (int64_t)bz * bg.nby * bg.nbx);
+    by = rem / bg.nbx;
+    bx = rem - by * bg.nbx;
+    in_zone = bz >= bg.zb0 && bz <= bg.zb1 && by >= bg.yb0 && by <= bg.yb1 && bx <= bg.xb1;
+    if (in_zone) nseg = bg.nseg;
+  } else {
+    const int e = (int)(u - bg.n_bricks);
+    const int zb = e / (bg.nseg - 1);
+    seg = 1 + e - zb * (bg.nseg - 1);
+    nseg = bg.nseg;
+    in_zone = true;
+    const int zz = zb / (bg.nzy * bg.nzx), rr = zb - zz * bg.nzy * bg.nzx;
+    bz = bg.zb0 + zz;
+    by = bg.yb0 + rr / bg.nzx;
+    bx = rr % bg.nzx;
+  }
+  const int r_begin = (int)((int64_t)n_rot * seg / nseg);
+  const int r_end = (int)((int64_t)n_rot * (seg + 1) / nseg);
+
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int bz = (int)(lb / ((int64_t)nby * nbx));
-  const int rem = (int)(lb - (int64_t)bz * nby * nbx);
-  const int by = rem / nbx, bx = rem - by * nbx;
   const int lo[3] = {bz * BZ, by * BY, bx * BX};
   const int hi[3] = {min(lo[0] + BZ - 1, g.vD - 1), min(lo[1] + BY - 1, g.vH - 1),
                      min(lo[2] + BX - 1, g.vW - 1)};
-  // extended box of coordinates whose 2x2x2 footprint can touch the brick
-  double L[3], U[3], ctr[3], half[3];
+  const int ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};  // local max index
+  const int top[3] = {g.vD - 1, g.vH - 1, g.vW - 1};
+  // Footprint boxes reaching a volume face need the FP32 guard tests there.
+  const bool face_lo[3] = {lo[0] <= 1, lo[1] <= 1, lo[2] <= 1};
+  const bool face_hi[3] = {hi[0] >= top[0] - 1, hi[1] >= top[1] - 1, hi[2] >= top[2] - 1};
+  const bool any_face = face_lo[0] || face_lo[1] || face_lo[2] || face_hi[0] || face_hi[1] || face_hi[2];
+  double Lb[3], Ub[3], ctr[3], half[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    L[d] = (double)lo[d] - 1.0 - kMargin;
-    U[d] = (double)hi[d] + 1.0 + kMargin;
-    ctr[d] = 0.5 * (L[d] + U[d]);
-    half[d] = 0.5 * (U[d] - L[d]);
+    Lb[d] = (double)lo[d] - 1.0 - kMargin;
+    Ub[d] = (double)hi[d] + 1.0 + kMargin;
+    ctr[d] = 0.5 * (Lb[d] + Ub[d]);
+    half[d] = 0.5 * (Ub[d] - Lb[d]);
   }
   const double beta[3] = {g.bz, g.by, 0.0};
-  double scale_v = 1.0, scale_w = 1.0;
-  if (DET) {
-    scale_v = fixed_scale(*maxabs, bound_v);
-    scale_w = fixed_scale(1.0f, bound_w);
-  }
+  const float vscale = value_scale(maxabs);
+  // brick-local FP32 interval bounds, widened to cover FP32 rounding
+  const float Lf[3] = {-1.0f - 1e-4f, -1.0f - 1e-4f, -1.0f - 1e-4f};
+  const float Uf[3] = {(float)ext[0] + 1.0f + 1e-4f, (float)ext[1] + 1.0f + 1e-4f,
+                       (float)ext[2] + 1.0f + 1e-4f};
 
-  for (int v = tid; v < NV; v += kT) {
+  for (int v = tid; v < NS; v += kT) {
     s_re[v] = 0;
     s_im[v] = 0;
     if (INSERT) s_w[v] = 0;
   }
 
-  for (int r0 = 0; r0 < n_rot; r0 += kRotChunk) {
+  for (int r0 = r_begin; r0 < r_end; r0 += kRotChunk) {
     if (tid == 0) s_nitems = 0;
     __syncthreads();
-    // (1) plane / brick intersection, one lane per rotation
+    // (1) plane / box intersection, one lane per rotation
     const int r = r0 + tid;
-    if (r < n_rot) {
-      PlaneGeom p = plane_geom(g, rot + (int64_t)r * 9);
-      double n0 = p.A[1] * p.B[2] - p.A[2] * p.B[1];
-      double n1 = p.A[2] * p.B[0] - p.A[0] * p.B[2];
-      double n2 = p.A[0] * p.B[1] - p.A[1] * p.B[0];
-      double dist = n0 * (ctr[0] - beta[0]) + n1 * (ctr[1] - beta[1]) + n2 * (ctr[2] - beta[2]);
-      double rad = fabs(n0) * half[0] + fabs(n1) * half[1] + fabs(n2) * half[2];
-      double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    if (r < r_end) {
+      const PlaneGeom p = plane_geom(g, rot + (int64_t)r * 9);
+      const double n0 = p.A[1] * p.B[2] - p.A[2] * p.B[1];
+      const double n1 = p.A[2] * p.B[0] - p.A[0] * p.B[2];
+      const double n2 = p.A[0] * p.B[1] - p.A[1] * p.B[0];
+      const double dist =
+          n0 * (ctr[0] - beta[0]) + n1 * (ctr[1] - beta[1]) + n2 * (ctr[2] - beta[2]);
+      const double rad = fabs(n0) * half[0] + fabs(n1) * half[1] + fabs(n2) * half[2];
+      const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
       if (fabs(dist) <= rad + 1e-9 * nn) {
-        double AA = p.A[0] * p.A[0] + p.A[1] * p.A[1] + p.A[2] * p.A[2];
-        double BB = p.B[0] * p.B[0] + p.B[1] * p.B[1] + p.B[2] * p.B[2];
-        double AB = p.A[0] * p.B[0] + p.A[1] * p.B[1] + p.A[2] * p.B[2];
-        double det = AA * BB - AB * AB;
+        const double AA = p.A[0] * p.A[0] + p.A[1] * p.A[1] + p.A[2] * p.A[2];
+        const double BB = p.B[0] * p.B[0] + p.B[1] * p.B[1] + p.B[2] * p.B[2];
+        const double AB = p.A[0] * p.B[0] + p.A[1] * p.B[1] + p.A[2] * p.B[2];
+        const double det = AA * BB - AB * AB;
         double mc = 0.0, mr = 0.0, jc = 0.0, jr = 0.0;
         if (det > 1e-20 * AA * BB) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            double pm = (BB * p.A[d] - AB * p.B[d]) / det;
-            double pj = (AA * p.B[d] - AB * p.A[d]) / det;
+            const double pm = (BB * p.A[d] - AB * p.B[d]) / det;
+            const double pj = (AA * p.B[d] - AB * p.A[d]) / det;
             mc += pm * (ctr[d] - beta[d]);
             jc += pj * (ctr[d] - beta[d]);
             mr += fabs(pm) * half[d];
@@ -173,113 +223,194 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
         }
         const double mmin = (double)(-g.half_H), mmax = (double)(g.H - 1 - g.half_H);
         for (int sh = 0; sh < 2; ++sh) {
-          double s = sh ? -1.0 : 1.0;
-          double jlo = s * jc - jr, jhi = s * jc + jr;
+          const double s = sh ? -1.0 : 1.0;
+          const double jlo = s * jc - jr, jhi = s * jc + jr;
           if (jhi < -kMargin || jlo > (double)(g.Wr - 1) + kMargin) continue;
-          double mlo = fmax(s * mc - mr, mmin), mhi = fmin(s * mc + mr, mmax);
-          int m0 = (int)ceil(mlo - kMargin), m1 = (int)floor(mhi + kMargin);
+          const double mlo = fmax(s * mc - mr, mmin), mhi = fmin(s * mc + mr, mmax);
+          const int m0 = (int)ceil(mlo - kMargin), m1 = (int)floor(mhi + kMargin);
           if (m0 > m1) continue;
-          int slot = atomicAdd(&s_nitems, 1);
-          s_item[slot] = 2 * r + sh;
-          s_m0[slot] = m0;
-          s_m1[slot] = m1;
+          // reference pixel: the one nearest the projection of the box centre
+          const int mref = (int)rint(fmin(fmax(s * mc, mmin), mmax));
+          const int jref = (int)rint(fmin(fmax(s * jc, 0.0), (double)(g.Wr - 1)));
+          const int slot = atomicAdd(&s_nitems, 1);
+          s_code[slot] = 2 * r + sh;
+          s_m0[slot] = (short)m0;
+          s_mc[slot] = (short)mref;
+          s_jc[slot] = (short)jref;
+          s_rowend[slot] = m1 - m0 + 1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            s_geo[slot][d] =
+                (float)(beta[d] + s * (p.A[d] * (double)mref + p.B[d] * (double)jref) - (double)lo[d]);
+            s_geo[slot][3 + d] = (float)(s * p.A[d]);
+            s_geo[slot][6 + d] = (float)(s * p.B[d]);
+          }
         }
       }
     }
     __syncthreads();
     const int nitems = s_nitems;
-    // (2) each wave walks its items row batch by row batch
-    for (int it = wave; it < nitems; it += kWaves) {
-      const int code = s_item[it];
-      const int ri = code >> 1;
-      const double s = (code & 1) ? -1.0 : 1.0;
-      const int m0 = s_m0[it], m1 = s_m1[it];
-      const float *Rp = rot + (int64_t)ri * 9;
-      PlaneGeom p = plane_geom(g, Rp);
-      Rot rr = load_rot(Rp, g);
-      double C1[3], inv1[3];
+    // inclusive prefix of the items' row counts (two items per thread)
+    {
+      const int a = 2 * tid < nitems ? s_rowend[2 * tid] : 0;
+      const int b = 2 * tid + 1 < nitems ? s_rowend[2 * tid + 1] : 0;
+      int incl = a + b;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        C1[d] = s * p.B[d];
-        inv1[d] = fabs(C1[d]) > 1e-12 ? 1.0 / C1[d] : 0.0;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
       }
-      for (int mb = m0; mb <= m1; mb += 64) {
-        const int m = mb + lane;
-        int jl = 0, len = 0;
-        if (m <= m1) {
-          double jmin = 0.0, jmax = (double)(g.Wr - 1);
+      if (lane == 63) s_wsum[wave] = incl;
+      __syncthreads();
+      int base = 0;
+      for (int w = 0; w < wave; ++w) base += s_wsum[w];
+      incl += base;
+      if (2 * tid < nitems) s_rowend[2 * tid] = incl - b;
+      if (2 * tid + 1 < nitems) s_rowend[2 * tid + 1] = incl;
+    }
+    __syncthreads();
+    const int total_rows = nitems > 0 ? s_rowend[nitems - 1] : 0;
+    // (2) rows of all items, 64 per wave pass
+    for (int rb = wave * 64; rb < total_rows; rb += kT) {
+      const int t = rb + lane;
+      int jl = 0, len = 0, code = 0, i = 0;
+      float st[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
+      if (t < total_rows) {
+        // item = first with rowend > t
+        int it = 0;
+        for (int step = kItemsMax / 2; step > 0; step >>= 1)
+          if (it + step - 1 < nitems && s_rowend[it + step - 1] <= t) it += step;
+        const int excl = it > 0 ? s_rowend[it - 1] : 0;
+        const int m = s_m0[it] + (t - excl);
+        code = s_code[it];
+        i = m + g.half_H;
+        // brick-local FP32 coordinates relative to the item's reference pixel
+        const int jc = s_jc[it];
+        const float dm = (float)(m - s_mc[it]);
+        float cm[3];
+        float dmin = (float)(-jc), dmax = (float)(g.Wr - 1 - jc);  // j - jc range
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            double c0 = beta[d] + s * p.A[d] * (double)m;
-            if (inv1[d] != 0.0) {
-              double t1 = (L[d] - c0) * inv1[d], t2 = (U[d] - c0) * inv1[d];
-              jmin = fmax(jmin, fmin(t1, t2));
-              jmax = fmin(jmax, fmax(t1, t2));
-            } else if (c0 < L[d] || c0 > U[d]) {
-              jmax = -1.0;
+        for (int d = 0; d < 3; ++d) {
+          sb[d] = s_geo[it][6 + d];
+          cm[d] = fmaf(dm, s_geo[it][3 + d], s_geo[it][d]);
+          if (fabsf(sb[d]) > 1e-6f) {
+            const float inv = 1.0f / sb[d];
+            const float t1 = (Lf[d] - cm[d]) * inv, t2 = (Uf[d] - cm[d]) * inv;
+            dmin = fmaxf(dmin, fminf(t1, t2));
+            dmax = fminf(dmax, fmaxf(t1, t2));
+          } else if (cm[d] < Lf[d] || cm[d] > Uf[d]) {
+            dmax = -1e9f;
+          }
+        }
+        dmin = fminf(fmaxf(dmin, -1e6f), 1e6f);  // keep the int conversions in range
+        dmax = fminf(fmaxf(dmax, -1e6f), 1e6f);
+        const int dl = (int)ceilf(dmin), dh = (int)floorf(dmax);
+        jl = jc + dl;
+        len = max(0, dh - dl + 1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) st[d] = fmaf((float)dl, sb[d], cm[d]);
+      }
+      const int nch = (len + kChunk - 1) / kChunk;
+      int incl = nch;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int tt = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += tt;
+      }
+      const int total = __shfl(incl, 63, 64);
+      for (int tb = 0; tb < total; tb += 64) {
+        const int tc = tb + lane;
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+          const int v = __shfl(incl, k + step - 1, 64);
+          if (v <= tc) k += step;
+        }
+        const int r_incl = __shfl(incl, k, 64);
+        const int r_nch = __shfl(nch, k, 64);
+        const int r_jl = __shfl(jl, k, 64);
+        const int r_len = __shfl(len, k, 64);
+        const int r_code = __shfl(code, k, 64);
+        const int r_i = __shfl(i, k, 64);
+        const float s0z = __shfl(st[0], k, 64), s0y = __shfl(st[1], k, 64), s0x = __shfl(st[2], k, 64);
+        const float bz_ = __shfl(sb[0], k, 64), by_ = __shfl(sb[1], k, 64), bx_ = __shfl(sb[2], k, 64);
+        if (tc >= total) continue;
+        const int q0 = (tc - (r_incl - r_nch)) * kChunk;  // first pixel offset in the row
+        const int qn = min(kChunk, r_len - q0);
+        const bool neg = r_code & 1;
+        const int ri = r_code >> 1;
+        for (int q = q0; q < q0 + qn; ++q) {
+          const int j = r_jl + q;
+          float lz = fmaf((float)q, bz_, s0z);
+          float ly = fmaf((float)q, by_, s0y);
+          float lx = fmaf((float)q, bx_, s0x);
+          if (INSERT && j == 0 && r_i < g.half_H) continue;
+          if (!in_disc(g, r_i, j)) continue;
+          if (any_face) {
+            const float gz = lz + (float)lo[0], gy = ly + (float)lo[1], gx = lx + (float)lo[2];
+            bool exact = false, out_vol = false;
+            if (face_lo[2] && gx < kGuard) (gx > -kGuard ? exact : out_vol) = true;
+            if (face_lo[0] && gz < kGuard) (gz > -kGuard ? exact : out_vol) = true;
+            if (face_lo[1] && gy < kGuard) (gy > -kGuard ? exact : out_vol) = true;
+            if (face_hi[0] && gz > (float)top[0] - kGuard) (gz < (float)top[0] + kGuard ? exact : out_vol) = true;
+            if (face_hi[1] && gy > (float)top[1] - kGuard) (gy < (float)top[1] + kGuard ? exact : out_vol) = true;
+            if (face_hi[2] && gx > (float)top[2] - kGuard) (gx < (float)top[2] + kGuard ? exact : out_vol) = true;
+            if (out_vol) continue;  // outside the volume or in the other Friedel half
+            if (exact) {
+              const Rot rr = load_rot(rot + (int64_t)ri * 9, g);
+              const Coord c = slice_coord<INSERT>(g, rr, r_i, j);
+              if (c.conj != neg || !inside(g, c)) continue;
+              lz = (float)(c.z - (double)lo[0]);
+              ly = (float)(c.y - (double)lo[1]);
+              lx = (float)(c.x - (double)lo[2]);
             }
           }
-          jl = (int)ceil(jmin - kMargin);
-          int jh = (int)floor(jmax + kMargin);
-          jl = max(jl, 0);
-          jh = min(jh, g.Wr - 1);
-          len = max(0, jh - jl + 1);
-        }
-        // inclusive wave prefix sum of row lengths
-        int incl = len;
+          // cell (lower corner clamped to N-2 globally, as in the gather)
+          const int z0 = min((int)floorf(lz), top[0] - 1 - lo[0]);
+          const int y0 = min((int)floorf(ly), top[1] - 1 - lo[1]);
+          const int x0 = min((int)floorf(lx), top[2] - 1 - lo[2]);
+          if (z0 > ext[0] || z0 < -1 || y0 > ext[1] || y0 < -1 || x0 > ext[2] || x0 < -1) continue;
+          const float tz = lz - (float)z0, ty = ly - (float)y0, tx = lx - (float)x0;
+          f2v val = src[(int64_t)ri * g.P + (int64_t)r_i * g.Wr + j];
+          if (neg) val.y = -val.y;
+          val *= vscale;
+          const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
+          if (z0 >= 0 && z0 < ext[0] && y0 >= 0 && y0 < ext[1] && x0 >= 0 && x0 < ext[2]) {
+            const int e = L::at(z0, y0, x0);
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          int t = __shfl_up(incl, off, 64);
-          if (lane >= off) incl += t;
-        }
-        const int total = __shfl(incl, 63, 64);
-        for (int tb = 0; tb < total; tb += 64) {
-          const int t = tb + lane;
-          // row k = number of rows whose inclusive prefix is <= t
-          int k = 0;
+            for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
-          for (int step = 32; step > 0; step >>= 1) {
-            int v = __shfl(incl, k + step - 1, 64);
-            if (v <= t) k += step;
-          }
-          const int row_incl = __shfl(incl, k, 64);
-          const int row_jl = __shfl(jl, k, 64);
-          const int row_len = __shfl(len, k, 64);
-          if (t >= total) continue;
-          const int j = row_jl + (t - (row_incl - row_len));
-          const int i = mb + k + g.half_H;
-          Coord c = slice_coord<INSERT>(g, rr, i, j);
-          if (c.conj != (s < 0.0)) continue;
-          if (INSERT && j == 0 && i < g.half_H) continue;
-          if (!in_disc(g, i, j) || !inside(g, c)) continue;
-          Cell k3 = cell_of(g, c);
-          const int z0 = k3.z0, y0 = k3.y0, x0 = k3.x0;
-          if (z0 > hi[0] || z0 + 1 < lo[0] || y0 > hi[1] || y0 + 1 < lo[1] || x0 > hi[2] ||
-              x0 + 1 < lo[2])
-            continue;
-          f2v v = src[(int64_t)ri * g.P + (int64_t)i * g.Wr + j];
-          if (c.conj) v.y = -v.y;
-          const float wz[2] = {1.0f - k3.tz, k3.tz};
-          const float wy[2] = {1.0f - k3.ty, k3.ty};
-          const float wx[2] = {1.0f - k3.tx, k3.tx};
+              for (int dy = 0; dy < 2; ++dy) {
+                const float wzy = wz[dz] * wy[dy];
 #pragma unroll
-          for (int dz = 0; dz < 2; ++dz) {
-            const int z = z0 + dz;
-            if (z < lo[0] || z > hi[0]) continue;
+                for (int dx = 0; dx < 2; ++dx) {
+                  const float w = wzy * wx[dx];
+                  const int o = e + dz * L::PZ + dy * L::PY + dx;
+                  lds_add(&s_re[o], w * val.x);
+                  lds_add(&s_im[o], w * val.y);
+                  if (INSERT) lds_add(&s_w[o], w * kWeightScale);
+                }
+              }
+          } else {
 #pragma unroll
-            for (int dy = 0; dy < 2; ++dy) {
-              const int y = y0 + dy;
-              if (y < lo[1] || y > hi[1]) continue;
-              const float wzy = wz[dz] * wy[dy];
+            for (int dz = 0; dz < 2; ++dz) {
+              const int z = z0 + dz;
+              if (z < 0 || z > ext[0]) continue;
 #pragma unroll
-              for (int dx = 0; dx < 2; ++dx) {
-                const int x = x0 + dx;
-                if (x < lo[2] || x > hi[2]) continue;
-                const float w = wzy * wx[dx];
-                const int e = ((z - lo[0]) * BY + (y - lo[1])) * BX + (x - lo[2]);
-                A::add(&s_re[e], w * v.x, scale_v);
-                A::add(&s_im[e], w * v.y, scale_v);
-                if (INSERT) A::add(&s_w[e], w, scale_w);
+              for (int dy = 0; dy < 2; ++dy) {
+                const int y = y0 + dy;
+                if (y < 0 || y > ext[1]) continue;
+                const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                  const int x = x0 + dx;
+                  if (x < 0 || x > ext[2]) continue;
+                  const float w = wzy * wx[dx];
+                  const int o = L::at(z, y, x);
+                  lds_add(&s_re[o], w * val.x);
+                  lds_add(&s_im[o], w * val.y);
+                  if (INSERT) lds_add(&s_w[o], w * kWeightScale);
+                }
               }
             }
           }
@@ -289,22 +420,67 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
     __syncthreads();
   }
 
-  // (3) flush: every voxel of the brick is written by this block only
+  // (4) flush
   const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
+  const float inv_v = 1.0f / vscale;
+  if (in_zone) {
+    const int zb = ((bz - bg.zb0) * bg.nzy + (by - bg.yb0)) * bg.nzx + bx;
+    u64 *acc = zone_acc + (int64_t)zb * (INSERT ? 3 : 2) * NV;
+    for (int v = tid; v < NV; v += kT) {
+      const int ls = L::at(v / (BY * BX), (v / BX) % BY, v % BX);
+      if (s_re[ls]) atomicAdd(acc + v, s_re[ls]);
+      if (s_im[ls]) atomicAdd(acc + NV + v, s_im[ls]);
+      if (INSERT && s_w[ls]) atomicAdd(acc + 2 * NV + v, s_w[ls]);
+    }
+    return;
+  }
   for (int v = tid; v < NV; v += kT) {
     const int lz = v / (BY * BX), ly = (v / BX) % BY, lx = v % BX;
     const int z = lo[0] + lz, y = lo[1] + ly, x = lo[2] + lx;
     if (z > hi[0] || y > hi[1] || x > hi[2]) continue;
-    f2v val = {A::get(s_re[v], scale_v), A::get(s_im[v], scale_v)};
+    const int ls = L::at(lz, ly, lx);
+    f2v val = {(float)(long long)s_re[ls] * inv_v, (float)(long long)s_im[ls] * inv_v};
+    const float w = INSERT ? (float)(long long)s_w[ls] * (1.0f / kWeightScale) : 0.0f;
     if (INSERT && x == 0) {
       plane[(int64_t)z * g.vH + y] = val;
-      wplane[(int64_t)z * g.vH + y] = A::get(s_w[v], scale_w);
+      wplane[(int64_t)z * g.vH + y] = w;
       continue;
     }
     const int64_t e = (int64_t)z * sz + (int64_t)y * sy + x;
     if (accumulate) val += out[e];
     out[e] = val;
-    if (INSERT) wout[e] = A::get(s_w[v], scale_w) + (accumulate ? wout[e] : 0.0f);
+    if (INSERT) wout[e] = w + (accumulate ? wout[e] : 0.0f);
+  }
+}
+
+// Converts the zone bricks' 64-bit partial sums and writes them like a flush.
+template <int BZ, int BY, int BX, bool INSERT>
+__global__ __launch_bounds__(kT) void zone_combine_kernel(
+    SliceGeom g, BrickGrid bg, const float *__restrict__ maxabs, const u64 *__restrict__ zone_acc,
+    f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
+    float *__restrict__ wplane, int accumulate) {
+  constexpr int NV = BZ * BY * BX;
+  const int zb = blockIdx.x;
+  if (zb >= bg.n_zone) return;
+  const int zz = zb / (bg.nzy * bg.nzx), rr = zb - zz * bg.nzy * bg.nzx;
+  const int lo[3] = {(bg.zb0 + zz) * BZ, (bg.yb0 + rr / bg.nzx) * BY, (rr % bg.nzx) * BX};
+  const float inv_v = 1.0f / value_scale(maxabs);
+  const u64 *acc = zone_acc + (int64_t)zb * (INSERT ? 3 : 2) * NV;
+  const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
+  for (int v = threadIdx.x; v < NV; v += kT) {
+    const int z = lo[0] + v / (BY * BX), y = lo[1] + (v / BX) % BY, x = lo[2] + v % BX;
+    if (z >= g.vD || y >= g.vH || x >= g.vW) continue;
+    f2v val = {(float)(long long)acc[v] * inv_v, (float)(long long)acc[NV + v] * inv_v};
+    const float w = INSERT ? (float)(long long)acc[2 * NV + v] * (1.0f / kWeightScale) : 0.0f;
+    if (INSERT && x == 0) {
+      plane[(int64_t)z * g.vH + y] = val;
+      wplane[(int64_t)z * g.vH + y] = w;
+      continue;
+    }
+    const int64_t e = (int64_t)z * sz + (int64_t)y * sy + x;
+    if (accumulate) val += out[e];
+    out[e] = val;
+    if (INSERT) wout[e] = w + (accumulate ? wout[e] : 0.0f);
   }
 }
 
@@ -320,7 +496,7 @@ __global__ __launch_bounds__(kT) void mirror_plane_kernel(const f2v *__restrict_
   f2v val = plane[idx];
   float w = wplane[idx];
   if (!(mz == D / 2 && my == H / 2)) {
-    f2v m = plane[mz * H + my];
+    const f2v m = plane[mz * H + my];
     val.x += m.x;
     val.y -= m.y;
     w += wplane[mz * H + my];
@@ -339,86 +515,121 @@ __global__ __launch_bounds__(kT) void maxabs_kernel(const f2v *__restrict__ x, i
                                                     unsigned int *__restrict__ res) {
   float m = 0.0f;
   for (int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x; p < n; p += (int64_t)gridDim.x * kT) {
-    f2v v = x[p];
+    const f2v v = x[p];
     m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
   }
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(res, __float_as_uint(m));
 }
 
-__global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
+template <typename T>
+__global__ __launch_bounds__(kT) void zero_words_kernel(T *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    p[i] = (T)0;
+}
 
-template <int BZ, int BY, int BX, bool INSERT, bool DET>
+template <int BZ, int BY, int BX>
+BrickGrid make_grid(const SliceGeom &g, int nseg, int zone_r, int zone_rx) {
+  BrickGrid bg;
+  bg.nbz = (g.vD + BZ - 1) / BZ;
+  bg.nby = (g.vH + BY - 1) / BY;
+  bg.nbx = (g.vW + BX - 1) / BX;
+  bg.n_bricks = bg.nbz * bg.nby * bg.nbx;
+  bg.nseg = std::max(nseg, 1);
+  // zone of bricks around the DC coordinate (bz, by, 0), which all slices cross
+  const int cz = std::min(std::max((int)std::floor(g.bz / BZ), 0), bg.nbz - 1);
+  const int cy = std::min(std::max((int)std::floor(g.by / BY), 0), bg.nby - 1);
+  bg.zb0 = std::max(cz - zone_r - 1, 0);
+  bg.zb1 = std::min(cz + zone_r, bg.nbz - 1);
+  bg.yb0 = std::max(cy - zone_r - 1, 0);
+  bg.yb1 = std::min(cy + zone_r, bg.nby - 1);
+  bg.xb1 = std::min(zone_rx, bg.nbx - 1);
+  if (bg.nseg == 1) {  // no split: an empty zone
+    bg.zb0 = bg.yb0 = 1;
+    bg.zb1 = bg.yb1 = 0;
+    bg.xb1 = -1;
+  }
+  bg.nzy = std::max(bg.yb1 - bg.yb0 + 1, 0);
+  bg.nzx = bg.xb1 + 1;
+  bg.n_zone = std::max(bg.zb1 - bg.zb0 + 1, 0) * bg.nzy * bg.nzx;
+  bg.n_units = bg.n_bricks + bg.n_zone * (bg.nseg - 1);
+  bg.n_units_padded = (bg.n_units + 7) & ~7;
+  return bg;
+}
+
+template <int BZ, int BY, int BX, bool INSERT>
 int launch_brick(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_rot, f2v *out,
                  float *wout, int accumulate, hipStream_t s) {
-  const int nbz = (g.vD + BZ - 1) / BZ, nby = (g.vH + BY - 1) / BY, nbx = (g.vW + BX - 1) / BX;
-  const int nb = nbz * nby * nbx;
-  const int nbp = (nb + 7) & ~7;
-  float *maxabs = nullptr;
-  f2v *plane = nullptr;
-  float *wplane = nullptr;
-  void *scratch = nullptr;
+  constexpr int NV = BZ * BY * BX;
+  // split the DC-zone bricks when there are enough rotations to share
+  const int nseg = n_rot >= 512 ? 8 : (n_rot >= 128 ? 4 : 1);
+  const BrickGrid bg = make_grid<BZ, BY, BX>(g, nseg, 2, 2);
   const size_t plane_elems = (size_t)g.vD * g.vH;
-  size_t bytes = 256;  // maxabs slot
-  if (INSERT) bytes += plane_elems * (sizeof(f2v) + sizeof(float));
-  if (DET || INSERT) {
-    hipError_t e = hipMallocAsync(&scratch, bytes, s);
-    if (e != hipSuccess) {
-      set_error(std::string("hipMallocAsync: ") + hipGetErrorString(e));
-      return TT_ERR_HIP;
-    }
-    maxabs = (float *)scratch;
-    if (INSERT) {
-      plane = (f2v *)((char *)scratch + 256);
-      wplane = (float *)((char *)scratch + 256 + plane_elems * sizeof(f2v));
-    }
+  const size_t zone_words = (size_t)bg.n_zone * (INSERT ? 3 : 2) * NV;
+  const size_t off_plane = 256;
+  const size_t off_wplane = off_plane + (INSERT ? plane_elems * sizeof(f2v) : 0);
+  const size_t off_zone =
+      (off_wplane + (INSERT ? plane_elems * sizeof(float) : 0) + 255) & ~(size_t)255;
+  const size_t bytes = off_zone + zone_words * sizeof(u64);
+  void *scratch = nullptr;
+  hipError_t e = hipMallocAsync(&scratch, bytes, s);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
   }
-  // per-slice bound on the summed trilinear weights one slice puts on one voxel
-  const double step_y = (double)g.ay * g.inv_H, step_x = (double)g.ax * g.inv_W;
-  const double per_px = std::max(step_y, step_x);
-  const double per_slice = std::pow(2.0 / std::max(per_px, 1e-3) + 2.0, 2.0);
-  const double bound = (double)n_rot * per_slice + 1.0;
-  if (DET) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, (unsigned int *)maxabs);
-    int64_t n = n_rot * (int64_t)g.P;
-    int grid = (int)std::min<int64_t>((n + kT - 1) / kT, 4096);
-    hipLaunchKernelGGL(maxabs_kernel, dim3(std::max(grid, 1)), dim3(kT), 0, s, src, n,
-                       (unsigned int *)maxabs);
+  char *base = (char *)scratch;
+  unsigned int *maxabs = (unsigned int *)base;
+  f2v *plane = INSERT ? (f2v *)(base + off_plane) : nullptr;
+  float *wplane = INSERT ? (float *)(base + off_wplane) : nullptr;
+  u64 *zone = zone_words ? (u64 *)(base + off_zone) : nullptr;
+  const int64_t n = n_rot * (int64_t)g.P;
+  hipLaunchKernelGGL(zero_words_kernel<unsigned int>, dim3(1), dim3(kT), 0, s, maxabs, (int64_t)1);
+  hipLaunchKernelGGL(maxabs_kernel,
+                     dim3((int)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 8192))),
+                     dim3(kT), 0, s, src, n, maxabs);
+  if (zone_words)
+    hipLaunchKernelGGL(zero_words_kernel<u64>,
+                       dim3((int)std::min<size_t>((zone_words + kT - 1) / kT, 4096)), dim3(kT), 0,
+                       s, zone, (int64_t)zone_words);
+  int st = check_launch("maxabs / zero");
+  if (!st) {
+    hipLaunchKernelGGL((brick_scatter_kernel<BZ, BY, BX, INSERT>), dim3(bg.n_units_padded),
+                       dim3(kT), 0, s, src, g, rot, (int)n_rot, bg, (const float *)maxabs, out,
+                       wout, plane, wplane, zone, accumulate);
+    st = check_launch("brick_scatter_kernel");
   }
-  hipLaunchKernelGGL((brick_scatter_kernel<BZ, BY, BX, INSERT, DET>), dim3(nbp), dim3(kT), 0, s,
-                     src, g, rot, (int)n_rot, nby, nbx, nb, nbp, maxabs, bound, bound, out, wout,
-                     plane, wplane, accumulate);
-  int st = check_launch("brick_scatter_kernel");
+  if (!st && bg.n_zone) {
+    hipLaunchKernelGGL((zone_combine_kernel<BZ, BY, BX, INSERT>), dim3(bg.n_zone), dim3(kT), 0, s,
+                       g, bg, (const float *)maxabs, (const u64 *)zone, out, wout, plane, wplane,
+                       accumulate);
+    st = check_launch("zone_combine_kernel");
+  }
   if (!st && INSERT) {
-    int n = (int)plane_elems;
-    hipLaunchKernelGGL(mirror_plane_kernel, dim3((n + kT - 1) / kT), dim3(kT), 0, s, plane, wplane,
-                       g.vD, g.vH, g.vW, out, wout, accumulate);
+    const int np = (int)plane_elems;
+    hipLaunchKernelGGL(mirror_plane_kernel, dim3((np + kT - 1) / kT), dim3(kT), 0, s, plane,
+                       wplane, g.vD, g.vH, g.vW, out, wout, accumulate);
     st = check_launch("mirror_plane_kernel");
   }
-  if (scratch) {
-    hipError_t e = hipFreeAsync(scratch, s);
-    if (!st && e != hipSuccess) {
-      set_error(std::string("hipFreeAsync: ") + hipGetErrorString(e));
-      st = TT_ERR_HIP;
-    }
+  e = hipFreeAsync(scratch, s);
+  if (!st && e != hipSuccess) {
+    set_error(std::string("hipFreeAsync: ") + hipGetErrorString(e));
+    st = TT_ERR_HIP;
   }
   return st;
 }
 
 }  // namespace
 
+// The fixed-point brick scatter is bitwise reproducible, so the deterministic
+// flag needs no separate path.
 int brick_scatter_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot,
-                          f2v *gV, int deterministic, int accumulate, hipStream_t s) {
-  if (deterministic)
-    return launch_brick<16, 16, 8, false, true>(g, gS, rot, n_rot, gV, nullptr, accumulate, s);
-  return launch_brick<16, 16, 16, false, false>(g, gS, rot, n_rot, gV, nullptr, accumulate, s);
+                          f2v *gV, int /*deterministic*/, int accumulate, hipStream_t s) {
+  return launch_brick<16, 16, 8, false>(g, gS, rot, n_rot, gV, nullptr, accumulate, s);
 }
 
 int brick_scatter_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64_t n_rot,
-                         f2v *vol, float *w, int deterministic, int accumulate, hipStream_t s) {
-  if (deterministic)
-    return launch_brick<16, 16, 8, true, true>(g, slices, rot, n_rot, vol, w, accumulate, s);
-  return launch_brick<16, 16, 16, true, false>(g, slices, rot, n_rot, vol, w, accumulate, s);
+                         f2v *vol, float *w, int /*deterministic*/, int accumulate, hipStream_t s) {
+  return launch_brick<16, 16, 8, true>(g, slices, rot, n_rot, vol, w, accumulate, s);
 }
 
 }  // namespace tt
