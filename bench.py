@@ -12,7 +12,7 @@ the dominant kernel (algorithmic bytes / HIP-event duration), the fused step's
 roofline and the FP64 oracle timed on the host CPU on a bounded sample.  The
 plain-torch comparator on the same GPU is timed with --torch.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu] [--copy-probe MIB]
 """
 
 from __future__ import annotations
@@ -44,6 +44,9 @@ def parse():
                          "gather, index_add_ adjoint) on the same GPU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--deterministic", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
+                         "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
     return ap.parse_args()
 
 
@@ -209,11 +212,20 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not launched)"}))
         sys.exit(2)
+    device = torch.device("cuda:0")
+    torch.cuda.set_device(device)
+    if args.copy_probe:
+        n = args.copy_probe << 18  # float32 elements
+        src = torch.ones(n, device=device)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": 4 * n, "copies": 5}), flush=True)
+        return
     from teamtomo_amd import _lib
 
     _lib.load()
-    device = torch.device("cuda:0")
-    torch.cuda.set_device(device)
     V, R, gS = make_inputs(device)
     step = Step(V, R, gS, args.deterministic)
     for _ in range(args.warmup):
@@ -235,7 +247,9 @@ def main():
     ext_b, adj_b = algorithmic_bytes()
     kernels = {
         "extract_fwd_kernel": {"ms": ext_ms, "bytes": ext_b, "GBs": ext_b / ext_ms / 1e6},
-        "brick_scatter_kernel (adjoint)": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6},
+        "adjoint": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6,
+                    "launch": "tt_extract_bwd_c64 volume gradient: maxabs_kernel + brick_scatter_kernel + "
+                              "zone_combine_kernel (+ two tiny zero fills)"},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["GBs"]
@@ -266,7 +280,8 @@ def main():
                    "volume": [N, N, N // 2 + 1], "rotations": N_ROT, "parallelism": "single GPU",
                    "deterministic": bool(args.deterministic)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_per_algorithmic_byte": traffic / kernels[dom]["bytes"] if traffic else None},
         "roofline_step": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes": step_bytes},
         "kernels": kernels,
