@@ -44,6 +44,8 @@ def parse():
                          "gather, index_add_ adjoint) on the same GPU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--deterministic", action="store_true")
+    ap.add_argument("--gather", choices=["auto", "pixel", "brick"], default="auto",
+                    help="extraction gather kernel (default: auto)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
@@ -223,9 +225,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_each_way": 4 * n, "copies": 5}), flush=True)
         return
-    from teamtomo_amd import _lib
+    from teamtomo_amd import _lib, set_gather_path
 
     _lib.load()
+    set_gather_path(args.gather)
     V, R, gS = make_inputs(device)
     step = Step(V, R, gS, args.deterministic)
     for _ in range(args.warmup):

@@ -41,11 +41,24 @@ extern "C" {
 #define TT_FLAG_OVERWRITE 2     /* write the outputs instead of adding to them          */
 #define TT_FLAG_PIXEL_ATOMIC 4  /* force the pixel-driven global-atomic scatter         */
 
+/* Extraction gather kernels (process-wide choice, see tt_set_gather_path) */
+#define TT_GATHER_AUTO 0  /* the library's choice (currently the pixel gather)  */
+#define TT_GATHER_PIXEL 1 /* one lane per pixel, corners read through L2/L1      */
+#define TT_GATHER_BRICK 2 /* volume bricks staged in LDS, serving all slices     */
+
 /* Message for the last error on the calling thread ("" if none). */
 const char *tt_last_error(void);
 
 /* ABI version; bumped whenever a signature below changes. */
 int tt_abi_version(void);
+
+/*
+ * Selects the kernel tt_extract_c64 uses (TT_GATHER_AUTO by default).  Both
+ * give the same samples; the choice only moves the cost between L2 traffic
+ * (pixel-major) and LDS staging (brick).  Has no counterpart in the
+ * reference: it is a tuning and testing switch of this library.
+ */
+int tt_set_gather_path(int path);
 
 /*
  * Central-slice extraction (the gather).

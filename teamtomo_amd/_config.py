@@ -28,3 +28,18 @@ def set_pixel_atomic(enabled: bool) -> None:
 
 def pixel_atomic_enabled() -> bool:
     return _pixel_atomic
+
+
+_GATHER_PATHS = {"auto": 0, "pixel": 1, "brick": 2}
+
+
+def set_gather_path(path: str) -> None:
+    """Choose the extraction gather kernel: "auto" (default: the library's
+    choice, currently the pixel gather), "pixel" (one lane per pixel, corners
+    read through the caches) or "brick" (volume bricks staged in LDS, serving
+    every slice crossing them).  Both give bitwise identical samples."""
+    if path not in _GATHER_PATHS:
+        raise ValueError(f"gather path must be one of {sorted(_GATHER_PATHS)}, got {path!r}")
+    from . import _lib
+
+    _lib.call("tt_set_gather_path", _GATHER_PATHS[path])

@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -26,6 +26,10 @@ TT_FLAG_DETERMINISTIC = 1
 TT_FLAG_OVERWRITE = 2
 TT_FLAG_PIXEL_ATOMIC = 4
 
+TT_GATHER_AUTO = 0
+TT_GATHER_PIXEL = 1
+TT_GATHER_BRICK = 2
+
 _p = ctypes.c_void_p
 _i = ctypes.c_int
 _l = ctypes.c_int64
@@ -33,6 +37,7 @@ _d = ctypes.c_double
 
 # name -> argtypes (all return int status)
 _SIGNATURES = {
+    "tt_set_gather_path": [_i],
     "tt_extract_c64": [_p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],
     "tt_extract_bwd_c64": [_p, _p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p, _i, _p],
     "tt_insert_c64": [_p, _i, _i, _i, _p, _l, _d, _p, _p, _i, _p],

@@ -58,12 +58,6 @@ constexpr float kWeightScale = 1073741824.0f;  // 2^30
 
 typedef unsigned long long u64;
 
-__device__ __forceinline__ int64_t logical_block_b(int64_t nb_padded) {
-  int64_t b = blockIdx.x;
-  int64_t per = nb_padded >> 3;
-  return (b & 7) * per + (b >> 3);
-}
-
 // LDS layout of a brick accumulator: row pitch BX+1, plane pitch BY*(BX+1)+1.
 template <int BZ, int BY, int BX> struct Lay {
   static constexpr int PY = BX + 1;
@@ -131,8 +125,14 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
   __shared__ int s_nitems;
   __shared__ int s_wsum[kWaves];
 
-  const int64_t u = logical_block_b(bg.n_units_padded);
-  if (u >= bg.n_units) return;
+  // The dispatcher deals consecutive blocks to the 8 XCDs in turn; in natural
+  // brick order every XCD gets a similar mix of heavy (near DC) and light
+  // bricks.  The extra segments of the DC-zone bricks, the heaviest units,
+  // are dispatched first so that they do not form the tail.
+  const int64_t b = blockIdx.x;
+  if (b >= bg.n_units) return;
+  const int64_t n_extra = bg.n_units - bg.n_bricks;
+  const int64_t u = b < n_extra ? bg.n_bricks + b : b - n_extra;
   // unit -> (brick, rotation segment)
   int bz, by, bx, seg = 0, nseg = 1;
   bool in_zone;

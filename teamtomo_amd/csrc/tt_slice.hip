@@ -16,6 +16,7 @@
 
 #include "../../include/teamtomo_amd.h"
 #include "tt_common.h"
+#include "tt_gather.h"
 #include "tt_scatter.h"
 
 namespace tt {
@@ -350,7 +351,16 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 2; }
+int tt_abi_version(void) { return 3; }
+
+static int g_gather_path = TT_GATHER_AUTO;
+
+int tt_set_gather_path(int path) {
+  if (path != TT_GATHER_AUTO && path != TT_GATHER_PIXEL && path != TT_GATHER_BRICK)
+    return fail_invalid("gather path must be TT_GATHER_AUTO, TT_GATHER_PIXEL or TT_GATHER_BRICK");
+  g_gather_path = path;
+  return TT_OK;
+}
 
 int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img_d, int img_h,
                    int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,
@@ -361,6 +371,9 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img
   if (!volume || !slices) return fail_invalid("volume / slices pointer is NULL");
   SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, x_tolerance);
   hipStream_t s = (hipStream_t)stream;
+  const int path = g_gather_path;
+  if (path == TT_GATHER_BRICK && brick_gather_eligible(g, n_rot))
+    return brick_gather(g, (const f2v *)volume, rot_zyx, n_rot, (f2v *)slices, s);
   int64_t bpr = (g.P + kBlock - 1) / kBlock;
   int64_t rots_per_launch = std::max<int64_t>(1, kMaxBlocksPerLaunch / bpr);
   for (int64_t r0 = 0; r0 < n_rot; r0 += rots_per_launch) {

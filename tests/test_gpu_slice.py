@@ -403,3 +403,75 @@ def test_scatter_accumulate_flag(device):
               _lib.stream_of(d))
     Vi, Wi = tt.insert_central_slices_rfft_3d(d, (24, 24, 24), R)
     assert rel_l2(_np(vol), _np(base_v + Vi)) < 1e-6 and rel_l2(_np(w), _np(base_w + Wi)) < 1e-6
+
+
+# ------------------------------------------------------ brick gather (LDS)
+@pytest.fixture
+def gather_path():
+    yield tt.set_gather_path
+    tt.set_gather_path("auto")
+
+
+@pytest.mark.parametrize("path", ["pixel", "brick"])
+@pytest.mark.parametrize("name,shape,fmax", [
+    ("extract_insert_64", (64, 64, 64), None), ("odd_33", (33, 33, 33), None),
+    ("fftfreq_max_03", (32, 32, 32), 0.3), ("image_shape_4x", (64, 64, 64), None),
+])
+def test_extract_paths_vs_oracle_and_golden(device, gather_path, path, name, shape, fmax):
+    gather_path(path)
+    G = golden(name)
+    S = _np(tt.extract_central_slices_rfft_3d(_t(G["volume_rfft"], device), shape, _t(G["rotations"], device),
+                                              fftfreq_max=fmax))
+    ref = O.extract_central_slices(G["volume_rfft"], shape, O.prepare_rotations(G["rotations"], False, 1e-8), fmax)
+    print(f"{path} {name}: rel_l2 vs FP64 {rel_l2(S, ref):.2e}, vs golden {rel_l2(S, G['slices']):.2e}")
+    assert rel_l2(S, ref) < EXT_TOL
+    assert rel_l2(S, G["slices"]) < EXT_TOL
+    assert np.array_equal(S == 0, ref == 0)
+
+
+def _exact_rotations():
+    m = [np.eye(3), [[0, -1, 0], [1, 0, 0], [0, 0, 1]], [[-1, 0, 0], [0, -1, 0], [0, 0, 1]],
+         [[1, 0, 0], [0, 0, -1], [0, 1, 0]], [[0, 0, 1], [0, 1, 0], [-1, 0, 0]]]
+    return np.asarray(m, dtype=np.float32)
+
+
+@pytest.mark.parametrize("n,image,nrot,fmax", [
+    (32, 32, 300, None), (33, 33, 200, None), (32, 64, 100, None), (48, 48, 300, 0.4), (256, 256, 96, None),
+])
+def test_brick_gather_equals_pixel_gather(device, gather_path, n, image, nrot, fmax):
+    """Both gathers evaluate the same FP64 coordinates, cells and weights, so
+    the samples agree to the last bit; exact-geometry rotations (samples on
+    voxel faces and brick boundaries) are included."""
+    V = volume_rfft(blob_volume(n, 30, seed=n, device=device))
+    R = torch.cat([_t(_exact_rotations(), device), uniform_rotations(nrot, n, device=device)])
+    gather_path("pixel")
+    a = _np(tt.extract_central_slices_rfft_3d(V, (image,) * 3, R, fftfreq_max=fmax))
+    gather_path("brick")
+    b = _np(tt.extract_central_slices_rfft_3d(V, (image,) * 3, R, fftfreq_max=fmax))
+    diff = max_abs(a, b)
+    print(f"n={n} image={image} nrot={nrot}: max |brick - pixel| = {diff:.3e}, bitwise {np.array_equal(a, b)}")
+    assert diff <= 1e-6 * np.abs(a).max()
+    assert np.array_equal(a == 0, b == 0)
+
+
+def test_brick_gather_full_size_matches_pixel_and_oracle_subset(device, gather_path):
+    """256^3 x 4096 rotations (the benchmark workload): brick and pixel gathers
+    agree everywhere; 8 slices are checked against the C FP64 oracle."""
+    from oracle import c_oracle as C
+
+    V = volume_rfft(blob_volume(256, 200, seed=2, device=device))
+    R = uniform_rotations(4096, 0, device=device)
+    gather_path("brick")
+    b = tt.extract_central_slices_rfft_3d(V, (256, 256, 256), R)
+    gather_path("pixel")
+    a = tt.extract_central_slices_rfft_3d(V, (256, 256, 256), R)
+    diff = float((a - b).abs().max())
+    scale = float(a.abs().max())
+    print(f"full size: max |brick - pixel| = {diff:.3e} (scale {scale:.3e})")
+    assert diff <= 1e-6 * scale
+    idx = [0, 1, 517, 1024, 2047, 3000, 4094, 4095]
+    Rn = O.prepare_rotations(R[idx].cpu().numpy(), False, 1e-8)
+    ref = C.extract(V.cpu().numpy(), (256, 256, 256), Rn)
+    sub = _np(b[idx])
+    print("full size subset rel_l2 vs FP64:", rel_l2(sub, ref))
+    assert rel_l2(sub, ref) < EXT_TOL
