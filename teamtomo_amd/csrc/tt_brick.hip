@@ -47,23 +47,24 @@ namespace tt {
 
 namespace {
 
-constexpr int kT = 256;
-constexpr int kWaves = kT / 64;
-constexpr int kRotChunk = kT;
+constexpr int kT = 256;            // threads of the helper kernels
+constexpr int kRotChunk = 64;      // rotations tested per pass, one lane each
 constexpr int kItemsMax = 2 * kRotChunk;
-constexpr int kChunk = 4;          // pixels per lane-chunk
 constexpr double kMargin = 1e-6;   // slack of the row-interval bounds
 constexpr float kGuard = 1e-3f;    // FP32 guard band (voxels) -> exact FP64 path
 constexpr float kWeightScale = 1073741824.0f;  // 2^30
 
 typedef unsigned long long u64;
 
-// LDS layout of a brick accumulator: row pitch BX+1, plane pitch BY*(BX+1)+1.
+// LDS layout of a brick accumulator with a one-voxel margin on every side:
+// local coordinates run from -1 to B.  Corners of a cell that straddles the
+// brick face land in the margin, which is never flushed, so every pixel adds
+// all 8 corners without per-corner tests (no divergence between lanes).
 template <int BZ, int BY, int BX> struct Lay {
-  static constexpr int PY = BX + 1;
-  static constexpr int PZ = BY * PY + 1;
-  static constexpr int SIZE = BZ * PZ;
-  __device__ static int at(int z, int y, int x) { return z * PZ + y * PY + x; }
+  static constexpr int PY = BX + 2;
+  static constexpr int PZ = (BY + 2) * PY;
+  static constexpr int SIZE = (BZ + 2) * PZ;
+  __device__ static int at(int z, int y, int x) { return (z + 1) * PZ + (y + 1) * PY + (x + 1); }
 };
 
 // Brick grid; bricks of the DC zone are split into rotation segments.
@@ -90,8 +91,47 @@ __device__ __forceinline__ PlaneGeom plane_geom(const SliceGeom &g, const float 
   return p;
 }
 
+// Per-rotation plane data shared by every brick: the slice axes A (rows)
+// and B (columns) in array units, the plane normal, and the rows of the
+// pseudo-inverse that map a point of the plane back to (row, column).
+struct PlanePre {
+  double A[3], B[3], n[3], pm[3], pj[3];
+  double nn;  // |n|; negative for a degenerate matrix (every pixel a candidate)
+};
+
+__global__ __launch_bounds__(kT) void plane_prep_kernel(SliceGeom g, const float *__restrict__ rot,
+                                                        int n_rot, PlanePre *__restrict__ pre) {
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= n_rot) return;
+  const PlaneGeom p = plane_geom(g, rot + (int64_t)r * 9);
+  PlanePre o;
+  for (int d = 0; d < 3; ++d) {
+    o.A[d] = p.A[d];
+    o.B[d] = p.B[d];
+  }
+  o.n[0] = p.A[1] * p.B[2] - p.A[2] * p.B[1];
+  o.n[1] = p.A[2] * p.B[0] - p.A[0] * p.B[2];
+  o.n[2] = p.A[0] * p.B[1] - p.A[1] * p.B[0];
+  o.nn = sqrt(o.n[0] * o.n[0] + o.n[1] * o.n[1] + o.n[2] * o.n[2]);
+  const double AA = p.A[0] * p.A[0] + p.A[1] * p.A[1] + p.A[2] * p.A[2];
+  const double BB = p.B[0] * p.B[0] + p.B[1] * p.B[1] + p.B[2] * p.B[2];
+  const double AB = p.A[0] * p.B[0] + p.A[1] * p.B[1] + p.A[2] * p.B[2];
+  const double det = AA * BB - AB * AB;
+  if (det > 1e-20 * AA * BB) {
+    for (int d = 0; d < 3; ++d) {
+      o.pm[d] = (BB * p.A[d] - AB * p.B[d]) / det;
+      o.pj[d] = (AA * p.B[d] - AB * p.A[d]) / det;
+    }
+  } else {
+    for (int d = 0; d < 3; ++d) o.pm[d] = o.pj[d] = 0.0;
+    o.nn = -1.0;
+  }
+  pre[r] = o;
+}
+
+// Adds round-toward-zero fixed point x; |x| <= 2^30 by the choice of scale.
 __device__ __forceinline__ void lds_add(u64 *p, float x) {
-  atomicAdd(p, (u64)(long long)__float2int_rn(x));
+  atomicAdd(p, (u64)(long long)(int)x);
 }
 
 // Fixed-point value scale 2^(30-e) with max|v| < 2^e, so |w v scale| <= 2^30.
@@ -103,15 +143,34 @@ __device__ __forceinline__ float value_scale(const float *maxabs) {
   return ldexpf(1.0f, 30 - e);
 }
 
+// Threads per brick block: the adjoint's LDS (u64 re/im accumulators, items,
+// row tables) allows two 512-thread blocks per CU; the insertion's extra
+// weight accumulator allows two 256-thread blocks.
+template <bool INSERT> struct ScatterCfg {
+  static constexpr int T = INSERT ? 256 : 512;
+  static constexpr int W = T / 64;
+  static constexpr int MIN_WAVES_PER_SIMD = INSERT ? 2 : 4;  // two blocks per CU
+};
+
+// One slice row's candidate pixels in brick-local FP32 coordinates:
+// pixel q of the row (column jl + q) sits at st + q * sb.
+struct RowRec {
+  int jl, excl, i, code;  // first column, pixels before this row, slice row, item code
+  float st0, st1, st2;
+  float sb0, sb1, sb2;
+};
+
 template <int BZ, int BY, int BX, bool INSERT>
-__global__ __launch_bounds__(kT) void brick_scatter_kernel(
+__global__ __launch_bounds__(ScatterCfg<INSERT>::T, ScatterCfg<INSERT>::MIN_WAVES_PER_SIMD) void brick_scatter_kernel(
     const f2v *__restrict__ src, SliceGeom g, const float *__restrict__ rot, int n_rot,
     BrickGrid bg, const float *__restrict__ maxabs, f2v *__restrict__ out,
     float *__restrict__ wout, f2v *__restrict__ plane, float *__restrict__ wplane,
-    u64 *__restrict__ zone_acc, int accumulate) {
+    u64 *__restrict__ zone_acc, const PlanePre *__restrict__ pre, int accumulate) {
   typedef Lay<BZ, BY, BX> L;
   constexpr int NV = BZ * BY * BX;
   constexpr int NS = L::SIZE;
+  constexpr int kBT = ScatterCfg<INSERT>::T, kBW = ScatterCfg<INSERT>::W;
+  static_assert(kItemsMax <= kBT && kRotChunk <= kBT, "one item and one rotation per thread");
   __shared__ u64 s_re[NS];
   __shared__ u64 s_im[NS];
   __shared__ u64 s_w[INSERT ? NS : 1];
@@ -121,9 +180,10 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
   __shared__ short s_mc[kItemsMax];     // reference pixel (row m, column j)
   __shared__ short s_jc[kItemsMax];
   __shared__ int s_rowend[kItemsMax];   // inclusive prefix of row counts
-  __shared__ float s_geo[kItemsMax][9]; // c(ref) - lo, s*A, s*B  (brick-local FP32)
+  __shared__ float s_geo[kItemsMax][12]; // c(ref) - lo, s*A, s*B, 1/(s*B) (brick-local FP32)
+  __shared__ RowRec s_rt[kBW][64];     // per wave: the rows of its current pass
   __shared__ int s_nitems;
-  __shared__ int s_wsum[kWaves];
+  __shared__ int s_wsum[kBW];
 
   // The dispatcher deals consecutive blocks to the 8 XCDs in turn; in natural
   // brick order every XCD gets a similar mix of heavy (near DC) and light
@@ -182,7 +242,7 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
   const float Uf[3] = {(float)ext[0] + 1.0f + 1e-4f, (float)ext[1] + 1.0f + 1e-4f,
                        (float)ext[2] + 1.0f + 1e-4f};
 
-  for (int v = tid; v < NS; v += kT) {
+  for (int v = tid; v < NS; v += kBT) {
     s_re[v] = 0;
     s_im[v] = 0;
     if (INSERT) s_w[v] = 0;
@@ -193,30 +253,20 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
     __syncthreads();
     // (1) plane / box intersection, one lane per rotation
     const int r = r0 + tid;
-    if (r < r_end) {
-      const PlaneGeom p = plane_geom(g, rot + (int64_t)r * 9);
-      const double n0 = p.A[1] * p.B[2] - p.A[2] * p.B[1];
-      const double n1 = p.A[2] * p.B[0] - p.A[0] * p.B[2];
-      const double n2 = p.A[0] * p.B[1] - p.A[1] * p.B[0];
-      const double dist =
-          n0 * (ctr[0] - beta[0]) + n1 * (ctr[1] - beta[1]) + n2 * (ctr[2] - beta[2]);
-      const double rad = fabs(n0) * half[0] + fabs(n1) * half[1] + fabs(n2) * half[2];
-      const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-      if (fabs(dist) <= rad + 1e-9 * nn) {
-        const double AA = p.A[0] * p.A[0] + p.A[1] * p.A[1] + p.A[2] * p.A[2];
-        const double BB = p.B[0] * p.B[0] + p.B[1] * p.B[1] + p.B[2] * p.B[2];
-        const double AB = p.A[0] * p.B[0] + p.A[1] * p.B[1] + p.A[2] * p.B[2];
-        const double det = AA * BB - AB * AB;
+    if (tid < kRotChunk && r < r_end) {
+      const PlanePre &p = pre[r];
+      const double dist = p.n[0] * (ctr[0] - beta[0]) + p.n[1] * (ctr[1] - beta[1]) +
+                          p.n[2] * (ctr[2] - beta[2]);
+      const double rad = fabs(p.n[0]) * half[0] + fabs(p.n[1]) * half[1] + fabs(p.n[2]) * half[2];
+      if (p.nn < 0.0 || fabs(dist) <= rad + 1e-9 * p.nn) {
         double mc = 0.0, mr = 0.0, jc = 0.0, jr = 0.0;
-        if (det > 1e-20 * AA * BB) {
+        if (p.nn >= 0.0) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            const double pm = (BB * p.A[d] - AB * p.B[d]) / det;
-            const double pj = (AA * p.B[d] - AB * p.A[d]) / det;
-            mc += pm * (ctr[d] - beta[d]);
-            jc += pj * (ctr[d] - beta[d]);
-            mr += fabs(pm) * half[d];
-            jr += fabs(pj) * half[d];
+            mc += p.pm[d] * (ctr[d] - beta[d]);
+            jc += p.pj[d] * (ctr[d] - beta[d]);
+            mr += fabs(p.pm[d]) * half[d];
+            jr += fabs(p.pj[d]) * half[d];
           }
         } else {  // degenerate (non-rotation) matrix: search every row and column
           mr = jr = 1e300;
@@ -243,18 +293,18 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
             s_geo[slot][d] =
                 (float)(beta[d] + s * (p.A[d] * (double)mref + p.B[d] * (double)jref) - (double)lo[d]);
             s_geo[slot][3 + d] = (float)(s * p.A[d]);
-            s_geo[slot][6 + d] = (float)(s * p.B[d]);
+            const float sb = (float)(s * p.B[d]);
+            s_geo[slot][6 + d] = sb;
+            s_geo[slot][9 + d] = fabsf(sb) > 1e-6f ? 1.0f / sb : 0.0f;
           }
         }
       }
     }
     __syncthreads();
     const int nitems = s_nitems;
-    // inclusive prefix of the items' row counts (two items per thread)
+    // inclusive prefix of the items' row counts (one item per thread)
     {
-      const int a = 2 * tid < nitems ? s_rowend[2 * tid] : 0;
-      const int b = 2 * tid + 1 < nitems ? s_rowend[2 * tid + 1] : 0;
-      int incl = a + b;
+      int incl = tid < nitems ? s_rowend[tid] : 0;
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
         const int t = __shfl_up(incl, off, 64);
@@ -265,21 +315,29 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
       int base = 0;
       for (int w = 0; w < wave; ++w) base += s_wsum[w];
       incl += base;
-      if (2 * tid < nitems) s_rowend[2 * tid] = incl - b;
-      if (2 * tid + 1 < nitems) s_rowend[2 * tid + 1] = incl;
+      if (tid < nitems) s_rowend[tid] = incl;
     }
     __syncthreads();
     const int total_rows = nitems > 0 ? s_rowend[nitems - 1] : 0;
     // (2) rows of all items, 64 per wave pass
-    for (int rb = wave * 64; rb < total_rows; rb += kT) {
+    for (int rb = wave * 64; rb < total_rows; rb += kBT) {
       const int t = rb + lane;
       int jl = 0, len = 0, code = 0, i = 0;
       float st[3] = {0.0f, 0.0f, 0.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
       if (t < total_rows) {
-        // item = first with rowend > t
-        int it = 0;
+        // item of row t: the first whose inclusive row end exceeds t.  The
+        // items of one 64-row pass are few and consecutive: find the first
+        // one (uniform search), then walk them (uniform loop).
+        int it0 = 0;
         for (int step = kItemsMax / 2; step > 0; step >>= 1)
-          if (it + step - 1 < nitems && s_rowend[it + step - 1] <= t) it += step;
+          if (it0 + step - 1 < nitems && s_rowend[it0 + step - 1] <= rb) it0 += step;
+        it0 = __builtin_amdgcn_readfirstlane(it0);
+        int it = it0;
+        for (int k = it0; k < nitems; ++k) {
+          const int end = s_rowend[k];
+          if (end <= t) it = k + 1;
+          if (end >= rb + 64) break;
+        }
         const int excl = it > 0 ? s_rowend[it - 1] : 0;
         const int m = s_m0[it] + (t - excl);
         code = s_code[it];
@@ -293,8 +351,8 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
         for (int d = 0; d < 3; ++d) {
           sb[d] = s_geo[it][6 + d];
           cm[d] = fmaf(dm, s_geo[it][3 + d], s_geo[it][d]);
-          if (fabsf(sb[d]) > 1e-6f) {
-            const float inv = 1.0f / sb[d];
+          const float inv = s_geo[it][9 + d];
+          if (inv != 0.0f) {
             const float t1 = (Lf[d] - cm[d]) * inv, t2 = (Uf[d] - cm[d]) * inv;
             dmin = fmaxf(dmin, fminf(t1, t2));
             dmax = fminf(dmax, fmaxf(t1, t2));
@@ -310,112 +368,103 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
 #pragma unroll
         for (int d = 0; d < 3; ++d) st[d] = fmaf((float)dl, sb[d], cm[d]);
       }
-      const int nch = (len + kChunk - 1) / kChunk;
-      int incl = nch;
+      // (3) the rows' pixels spread over the lanes, consecutive lanes taking
+      // consecutive pixels of a row (coalesced value loads)
+      int incl = len;
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
         const int tt = __shfl_up(incl, off, 64);
         if (lane >= off) incl += tt;
       }
       const int total = __shfl(incl, 63, 64);
-      for (int tb = 0; tb < total; tb += 64) {
-        const int tc = tb + lane;
+      if (len > 0) {
+        RowRec rec;
+        rec.jl = jl;
+        rec.excl = incl - len;
+        rec.i = i;
+        rec.code = code;
+        rec.st0 = st[0];
+        rec.st1 = st[1];
+        rec.st2 = st[2];
+        rec.sb0 = sb[0];
+        rec.sb1 = sb[1];
+        rec.sb2 = sb[2];
+        s_rt[wave][lane] = rec;
+      }
+      __builtin_amdgcn_wave_barrier();
+      // Branch-free per pixel: a rejected pixel (past the row ends, outside
+      // the disc, the volume or the brick) adds zero at the margin corner, so
+      // the lanes of a wave never diverge on the common path.
+      for (int p0 = 0; p0 < total; p0 += 64) {
+        const int p = p0 + lane;
         int k = 0;
 #pragma unroll
         for (int step = 32; step > 0; step >>= 1) {
           const int v = __shfl(incl, k + step - 1, 64);
-          if (v <= tc) k += step;
+          if (v <= p) k += step;
         }
-        const int r_incl = __shfl(incl, k, 64);
-        const int r_nch = __shfl(nch, k, 64);
-        const int r_jl = __shfl(jl, k, 64);
-        const int r_len = __shfl(len, k, 64);
-        const int r_code = __shfl(code, k, 64);
-        const int r_i = __shfl(i, k, 64);
-        const float s0z = __shfl(st[0], k, 64), s0y = __shfl(st[1], k, 64), s0x = __shfl(st[2], k, 64);
-        const float bz_ = __shfl(sb[0], k, 64), by_ = __shfl(sb[1], k, 64), bx_ = __shfl(sb[2], k, 64);
-        if (tc >= total) continue;
-        const int q0 = (tc - (r_incl - r_nch)) * kChunk;  // first pixel offset in the row
-        const int qn = min(kChunk, r_len - q0);
-        const bool neg = r_code & 1;
-        const int ri = r_code >> 1;
-        for (int q = q0; q < q0 + qn; ++q) {
-          const int j = r_jl + q;
-          float lz = fmaf((float)q, bz_, s0z);
-          float ly = fmaf((float)q, by_, s0y);
-          float lx = fmaf((float)q, bx_, s0x);
-          if (INSERT && j == 0 && r_i < g.half_H) continue;
-          if (!in_disc(g, r_i, j)) continue;
-          if (any_face) {
-            const float gz = lz + (float)lo[0], gy = ly + (float)lo[1], gx = lx + (float)lo[2];
-            bool exact = false, out_vol = false;
-            if (face_lo[2] && gx < kGuard) (gx > -kGuard ? exact : out_vol) = true;
-            if (face_lo[0] && gz < kGuard) (gz > -kGuard ? exact : out_vol) = true;
-            if (face_lo[1] && gy < kGuard) (gy > -kGuard ? exact : out_vol) = true;
-            if (face_hi[0] && gz > (float)top[0] - kGuard) (gz < (float)top[0] + kGuard ? exact : out_vol) = true;
-            if (face_hi[1] && gy > (float)top[1] - kGuard) (gy < (float)top[1] + kGuard ? exact : out_vol) = true;
-            if (face_hi[2] && gx > (float)top[2] - kGuard) (gx < (float)top[2] + kGuard ? exact : out_vol) = true;
-            if (out_vol) continue;  // outside the volume or in the other Friedel half
-            if (exact) {
-              const Rot rr = load_rot(rot + (int64_t)ri * 9, g);
-              const Coord c = slice_coord<INSERT>(g, rr, r_i, j);
-              if (c.conj != neg || !inside(g, c)) continue;
-              lz = (float)(c.z - (double)lo[0]);
-              ly = (float)(c.y - (double)lo[1]);
-              lx = (float)(c.x - (double)lo[2]);
-            }
-          }
-          // cell (lower corner clamped to N-2 globally, as in the gather)
-          const int z0 = min((int)floorf(lz), top[0] - 1 - lo[0]);
-          const int y0 = min((int)floorf(ly), top[1] - 1 - lo[1]);
-          const int x0 = min((int)floorf(lx), top[2] - 1 - lo[2]);
-          if (z0 > ext[0] || z0 < -1 || y0 > ext[1] || y0 < -1 || x0 > ext[2] || x0 < -1) continue;
-          const float tz = lz - (float)z0, ty = ly - (float)y0, tx = lx - (float)x0;
-          f2v val = src[(int64_t)ri * g.P + (int64_t)r_i * g.Wr + j];
-          if (neg) val.y = -val.y;
-          val *= vscale;
-          const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
-          if (z0 >= 0 && z0 < ext[0] && y0 >= 0 && y0 < ext[1] && x0 >= 0 && x0 < ext[2]) {
-            const int e = L::at(z0, y0, x0);
-#pragma unroll
-            for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-              for (int dy = 0; dy < 2; ++dy) {
-                const float wzy = wz[dz] * wy[dy];
-#pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                  const float w = wzy * wx[dx];
-                  const int o = e + dz * L::PZ + dy * L::PY + dx;
-                  lds_add(&s_re[o], w * val.x);
-                  lds_add(&s_im[o], w * val.y);
-                  if (INSERT) lds_add(&s_w[o], w * kWeightScale);
-                }
-              }
-          } else {
-#pragma unroll
-            for (int dz = 0; dz < 2; ++dz) {
-              const int z = z0 + dz;
-              if (z < 0 || z > ext[0]) continue;
-#pragma unroll
-              for (int dy = 0; dy < 2; ++dy) {
-                const int y = y0 + dy;
-                if (y < 0 || y > ext[1]) continue;
-                const float wzy = wz[dz] * wy[dy];
-#pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                  const int x = x0 + dx;
-                  if (x < 0 || x > ext[2]) continue;
-                  const float w = wzy * wx[dx];
-                  const int o = L::at(z, y, x);
-                  lds_add(&s_re[o], w * val.x);
-                  lds_add(&s_im[o], w * val.y);
-                  if (INSERT) lds_add(&s_w[o], w * kWeightScale);
-                }
-              }
-            }
+        const RowRec rec = s_rt[wave][k];
+        const int q = p - rec.excl;
+        const int j = rec.jl + q;
+        const int r_i = rec.i;
+        const int ri = rec.code >> 1;
+        const bool neg = rec.code & 1;
+        bool ok = p < total;
+        if (INSERT) ok = ok && !(j == 0 && r_i < g.half_H);
+        if (g.use_mask) ok = ok && in_disc(g, r_i, j);
+        float lz = fmaf((float)q, rec.sb0, rec.st0);
+        float ly = fmaf((float)q, rec.sb1, rec.st1);
+        float lx = fmaf((float)q, rec.sb2, rec.st2);
+        if (any_face) {
+          const float gz = lz + (float)lo[0], gy = ly + (float)lo[1], gx = lx + (float)lo[2];
+          bool exact = false, out_vol = false;
+          if (face_lo[2] && gx < kGuard) (gx > -kGuard ? exact : out_vol) = true;
+          if (face_lo[0] && gz < kGuard) (gz > -kGuard ? exact : out_vol) = true;
+          if (face_lo[1] && gy < kGuard) (gy > -kGuard ? exact : out_vol) = true;
+          if (face_hi[0] && gz > (float)top[0] - kGuard) (gz < (float)top[0] + kGuard ? exact : out_vol) = true;
+          if (face_hi[1] && gy > (float)top[1] - kGuard) (gy < (float)top[1] + kGuard ? exact : out_vol) = true;
+          if (face_hi[2] && gx > (float)top[2] - kGuard) (gx < (float)top[2] + kGuard ? exact : out_vol) = true;
+          ok = ok && !out_vol;  // outside the volume or in the other Friedel half
+          if (ok && exact) {
+            const Rot rr = load_rot(rot + (int64_t)ri * 9, g);
+            const Coord c = slice_coord<INSERT>(g, rr, r_i, j);
+            ok = c.conj == neg && inside(g, c);
+            lz = (float)(c.z - (double)lo[0]);
+            ly = (float)(c.y - (double)lo[1]);
+            lx = (float)(c.x - (double)lo[2]);
           }
         }
+        // cell (lower corner clamped to N-2 globally, as in the gather)
+        int z0 = min((int)floorf(lz), top[0] - 1 - lo[0]);
+        int y0 = min((int)floorf(ly), top[1] - 1 - lo[1]);
+        int x0 = min((int)floorf(lx), top[2] - 1 - lo[2]);
+        ok = ok && z0 >= -1 && z0 <= ext[0] && y0 >= -1 && y0 <= ext[1] && x0 >= -1 && x0 <= ext[2];
+        const float tz = lz - (float)z0, ty = ly - (float)y0, tx = lx - (float)x0;
+        if (!ok) z0 = y0 = x0 = -1;
+        const int64_t src_off = ok ? (int64_t)ri * g.P + (int64_t)r_i * g.Wr + j : 0;
+        f2v val = src[src_off];
+        val.y = neg ? -val.y : val.y;
+        const float vs = ok ? vscale : 0.0f;
+        val *= vs;
+        const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
+        const float wsc = ok ? kWeightScale : 0.0f;
+        const int e = L::at(z0, y0, x0);
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy) {
+            const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+              const float w = wzy * wx[dx];
+              const int o = e + dz * L::PZ + dy * L::PY + dx;
+              lds_add(&s_re[o], w * val.x);
+              lds_add(&s_im[o], w * val.y);
+              if (INSERT) lds_add(&s_w[o], w * wsc);
+            }
+          }
       }
+      __builtin_amdgcn_wave_barrier();  // the next pass rewrites this wave's row table
     }
     __syncthreads();
   }
@@ -426,7 +475,7 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
   if (in_zone) {
     const int zb = ((bz - bg.zb0) * bg.nzy + (by - bg.yb0)) * bg.nzx + bx;
     u64 *acc = zone_acc + (int64_t)zb * (INSERT ? 3 : 2) * NV;
-    for (int v = tid; v < NV; v += kT) {
+    for (int v = tid; v < NV; v += kBT) {
       const int ls = L::at(v / (BY * BX), (v / BX) % BY, v % BX);
       if (s_re[ls]) atomicAdd(acc + v, s_re[ls]);
       if (s_im[ls]) atomicAdd(acc + NV + v, s_im[ls]);
@@ -434,7 +483,7 @@ __global__ __launch_bounds__(kT) void brick_scatter_kernel(
     }
     return;
   }
-  for (int v = tid; v < NV; v += kT) {
+  for (int v = tid; v < NV; v += kBT) {
     const int lz = v / (BY * BX), ly = (v / BX) % BY, lx = v % BX;
     const int z = lo[0] + lz, y = lo[1] + ly, x = lo[2] + lx;
     if (z > hi[0] || y > hi[1] || x > hi[2]) continue;
@@ -511,8 +560,37 @@ __global__ __launch_bounds__(kT) void mirror_plane_kernel(const f2v *__restrict_
 }
 
 // max(|re|, |im|) over n complex values into *res (float bits, res >= 0).
+// Reads 16 bytes (two complex values) per lane and load, four loads in
+// flight per lane; an odd tail element is read by thread 0.
 __global__ __launch_bounds__(kT) void maxabs_kernel(const f2v *__restrict__ x, int64_t n,
                                                     unsigned int *__restrict__ res) {
+  const float4 *x4 = reinterpret_cast<const float4 *>(x);
+  const int64_t n4 = n >> 1;
+  const int64_t stride = (int64_t)gridDim.x * kT;
+  float m = 0.0f;
+  int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x;
+  for (; p + 3 * stride < n4; p += 4 * stride) {
+    const float4 a = x4[p], b = x4[p + stride], c = x4[p + 2 * stride], d = x4[p + 3 * stride];
+    m = fmaxf(m, fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))),
+                       fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fmaxf(fabsf(b.z), fabsf(b.w)))));
+    m = fmaxf(m, fmaxf(fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(fabsf(c.z), fabsf(c.w))),
+                       fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fmaxf(fabsf(d.z), fabsf(d.w)))));
+  }
+  for (; p < n4; p += stride) {
+    const float4 a = x4[p];
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))));
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const f2v v = x[n - 1];
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(res, __float_as_uint(m));
+}
+
+// Scalar fallback for a source that is not 16-byte aligned.
+__global__ __launch_bounds__(kT) void maxabs_scalar_kernel(const f2v *__restrict__ x, int64_t n,
+                                                           unsigned int *__restrict__ res) {
   float m = 0.0f;
   for (int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x; p < n; p += (int64_t)gridDim.x * kT) {
     const f2v v = x[p];
@@ -570,7 +648,8 @@ int launch_brick(const SliceGeom &g, const f2v *src, const float *rot, int64_t n
   const size_t off_wplane = off_plane + (INSERT ? plane_elems * sizeof(f2v) : 0);
   const size_t off_zone =
       (off_wplane + (INSERT ? plane_elems * sizeof(float) : 0) + 255) & ~(size_t)255;
-  const size_t bytes = off_zone + zone_words * sizeof(u64);
+  const size_t off_pre = (off_zone + zone_words * sizeof(u64) + 255) & ~(size_t)255;
+  const size_t bytes = off_pre + (size_t)n_rot * sizeof(PlanePre);
   void *scratch = nullptr;
   hipError_t e = hipMallocAsync(&scratch, bytes, s);
   if (e != hipSuccess) {
@@ -582,20 +661,24 @@ int launch_brick(const SliceGeom &g, const f2v *src, const float *rot, int64_t n
   f2v *plane = INSERT ? (f2v *)(base + off_plane) : nullptr;
   float *wplane = INSERT ? (float *)(base + off_wplane) : nullptr;
   u64 *zone = zone_words ? (u64 *)(base + off_zone) : nullptr;
+  PlanePre *pre = (PlanePre *)(base + off_pre);
   const int64_t n = n_rot * (int64_t)g.P;
   hipLaunchKernelGGL(zero_words_kernel<unsigned int>, dim3(1), dim3(kT), 0, s, maxabs, (int64_t)1);
-  hipLaunchKernelGGL(maxabs_kernel,
-                     dim3((int)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 8192))),
-                     dim3(kT), 0, s, src, n, maxabs);
+  if (((uintptr_t)src & 15) == 0)
+    hipLaunchKernelGGL(maxabs_kernel, dim3(2048), dim3(kT), 0, s, src, n, maxabs);
+  else
+    hipLaunchKernelGGL(maxabs_scalar_kernel, dim3(2048), dim3(kT), 0, s, src, n, maxabs);
   if (zone_words)
     hipLaunchKernelGGL(zero_words_kernel<u64>,
                        dim3((int)std::min<size_t>((zone_words + kT - 1) / kT, 4096)), dim3(kT), 0,
                        s, zone, (int64_t)zone_words);
-  int st = check_launch("maxabs / zero");
+  hipLaunchKernelGGL(plane_prep_kernel, dim3((unsigned)((n_rot + kT - 1) / kT)), dim3(kT), 0, s, g, rot,
+                     (int)n_rot, pre);
+  int st = check_launch("maxabs / zero / plane_prep");
   if (!st) {
     hipLaunchKernelGGL((brick_scatter_kernel<BZ, BY, BX, INSERT>), dim3(bg.n_units_padded),
-                       dim3(kT), 0, s, src, g, rot, (int)n_rot, bg, (const float *)maxabs, out,
-                       wout, plane, wplane, zone, accumulate);
+                       dim3(ScatterCfg<INSERT>::T), 0, s, src, g, rot, (int)n_rot, bg, (const float *)maxabs, out,
+                       wout, plane, wplane, zone, pre, accumulate);
     st = check_launch("brick_scatter_kernel");
   }
   if (!st && bg.n_zone) {
