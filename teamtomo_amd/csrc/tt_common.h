@@ -5,10 +5,17 @@
 // frequency (0, fy, fx) with fy = (i - H//2)/H, fx = j/W
 // (_grids/_central_slice_fftfreq_grid.py:6-32); rotated frequencies map to
 // array coordinates c = (f + 0.5)(N-1)/(1-1/N) on full axes and
-// c = 2 f (Wr-1) on the half axis (_dft_utils.py:59-67).  Coordinates are
-// evaluated in FP64 (full-rate on CDNA4 vector ALUs) and snapped to integers
-// within 1e-9 voxel, so exact-geometry rotations land exactly on voxels and
-// faces as the reference's float32 arithmetic does.
+// c = 2 f (Wr-1) on the half axis (_dft_utils.py:59-67).
+//
+// Canonical coordinates.  Every kernel (pixel gather, brick gather and
+// scatter, zero pass, backward passes) evaluates the coordinate of pixel
+// (i, j) under rotation R and Friedel sign s with the same FP64 operations:
+//   U_d = a_d R[d][1],  V_d = (a_d R[d][2]) / W,  fy = (i - H//2) / H
+//   c_d = fma(j, s V_d, fma(s U_d, fy, b_d)),   b = (bz, by, 0)
+// and snaps values within 1e-9 of an integer to that integer, so that
+// exact-geometry rotations land exactly on voxels and faces as the
+// reference's float32 arithmetic does, and so that a brick kernel and the
+// pixel kernels agree bit for bit on which cell (and brick) owns a pixel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,6 +48,7 @@ struct SliceGeom {
   float fmax;             // float32(fftfreq_max)
   float inv_H32, inv_W32; // float32(1/H), float32(1/W) as torch.fft.fftfreq uses
   double xtol, xtol10;    // float32(x_tol), float32(10 x_tol) as doubles
+  double tolx;            // xtol * ax: the x' tolerance in half-axis array units
   float invWr_f;          // 1/Wr for the flat-index split
 };
 
@@ -78,45 +86,56 @@ struct Coord {
   bool conj;
 };
 
-// Rotation loaded once per block (wave-uniform -> scalar registers).
+// Canonical per-rotation constants (wave-uniform when one rotation per block).
 struct Rot {
-  double r01, r02, r11, r12, r21, r22;  // columns y, x of the zyx matrix
-  bool tie;                             // extraction conj decision on the x'=0 plane
+  double U[3], V[3];  // a_d R_zyx[d][1] and (a_d R_zyx[d][2]) / W, d = z, y, x
+  bool tie;           // extraction conj decision on the x'=0 plane
 };
 
 __device__ __forceinline__ Rot load_rot(const float *__restrict__ R, const SliceGeom &g) {
   Rot r;
-  r.r01 = (double)R[1];
-  r.r02 = (double)R[2];
-  r.r11 = (double)R[4];
-  r.r12 = (double)R[5];
-  r.r21 = (double)R[7];
-  r.r22 = (double)R[8];
+  const double a[3] = {g.az, g.ay, g.ax};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    r.U[d] = __dmul_rn(a[d], (double)R[3 * d + 1]);
+    r.V[d] = __dmul_rn(__dmul_rn(a[d], (double)R[3 * d + 2]), g.inv_W);
+  }
   // _determine_near_zero_conjugate_mask (:53-79): the sign of R_zyx[2,2]
   // decides, or that of R_zyx[2,1] when |R_zyx[2,2]| <= 10 tol.
-  r.tie = (fabs(r.r22) <= g.xtol10) ? (r.r21 < 0.0) : (r.r22 < 0.0);
+  const double r21 = (double)R[7], r22 = (double)R[8];
+  r.tie = (fabs(r22) <= g.xtol10) ? (r21 < 0.0) : (r22 < 0.0);
   return r;
 }
 
-// Rotated, Friedel-folded array coordinate of slice pixel (i, j).
+__device__ __forceinline__ double canon_fy(const SliceGeom &g, int i) {
+  return __dmul_rn((double)(i - g.half_H), g.inv_H);
+}
+
+// x coordinate for s = +1 (a_x x'), whose sign decides the Friedel half.
+__device__ __forceinline__ double canon_cx1(const Rot &r, double fy, int j) {
+  return __fma_rn((double)j, r.V[2], __dmul_rn(r.U[2], fy));
+}
+
 // INSERT: conjugate iff x' < 0 (insertion :78-80).  Otherwise the extraction
 // rule: x' < -tol, or |x'| < tol and the per-rotation tie (extraction :193-203).
 template <bool INSERT>
+__device__ __forceinline__ bool canon_conj(const SliceGeom &g, bool tie, double cx1) {
+  if (INSERT) return cx1 < 0.0;
+  return (cx1 < -g.tolx) || (fabs(cx1) < g.tolx && tie);
+}
+
+// Rotated, Friedel-folded, snapped array coordinate of slice pixel (i, j).
+template <bool INSERT>
 __device__ __forceinline__ Coord slice_coord(const SliceGeom &g, const Rot &r, int i, int j) {
-  double fy = (double)(i - g.half_H) * g.inv_H;
-  double fx = (double)j * g.inv_W;
-  double qz = fma(r.r01, fy, r.r02 * fx);
-  double qy = fma(r.r11, fy, r.r12 * fx);
-  double qx = fma(r.r21, fy, r.r22 * fx);
+  const double fy = canon_fy(g, i);
+  const double jd = (double)j;
+  const double cx1 = canon_cx1(r, fy, j);
   Coord c;
-  if (INSERT)
-    c.conj = qx < 0.0;
-  else
-    c.conj = (qx < -g.xtol) || (fabs(qx) < g.xtol && r.tie);
-  double s = c.conj ? -1.0 : 1.0;
-  c.z = snap(fma(s * qz, g.az, g.bz));
-  c.y = snap(fma(s * qy, g.ay, g.by));
-  c.x = snap(s * qx * g.ax);
+  c.conj = canon_conj<INSERT>(g, r.tie, cx1);
+  const double s = c.conj ? -1.0 : 1.0;
+  c.z = snap(__fma_rn(jd, s * r.V[0], __fma_rn(s * r.U[0], fy, g.bz)));
+  c.y = snap(__fma_rn(jd, s * r.V[1], __fma_rn(s * r.U[1], fy, g.by)));
+  c.x = snap(c.conj ? -cx1 : cx1);
   return c;
 }
 
@@ -142,6 +161,27 @@ __device__ __forceinline__ Cell cell_of(const SliceGeom &g, const Coord &c) {
   k.ty = (float)(c.y - (double)k.y0);
   k.tx = (float)(c.x - (double)k.x0);
   return k;
+}
+
+// Trilinear interpolation of 8 complex corners, x then y then z, with the
+// products and fused adds pinned so every gather kernel rounds alike.
+__device__ __forceinline__ float lerp1(float a, float b, float u, float t) {
+  return __fmaf_rn(b, t, __fmul_rn(a, u));
+}
+
+__device__ __forceinline__ f2v trilerp(f2v v000, f2v v001, f2v v010, f2v v011, f2v v100, f2v v101,
+                                       f2v v110, f2v v111, float tz, float ty, float tx) {
+  const float ux = 1.0f - tx, uy = 1.0f - ty, uz = 1.0f - tz;
+  const float r00 = lerp1(v000.x, v001.x, ux, tx), i00 = lerp1(v000.y, v001.y, ux, tx);
+  const float r01 = lerp1(v010.x, v011.x, ux, tx), i01 = lerp1(v010.y, v011.y, ux, tx);
+  const float r10 = lerp1(v100.x, v101.x, ux, tx), i10 = lerp1(v100.y, v101.y, ux, tx);
+  const float r11 = lerp1(v110.x, v111.x, ux, tx), i11 = lerp1(v110.y, v111.y, ux, tx);
+  const float r0 = lerp1(r00, r01, uy, ty), i0 = lerp1(i00, i01, uy, ty);
+  const float r1 = lerp1(r10, r11, uy, ty), i1 = lerp1(i10, i11, uy, ty);
+  f2v out;
+  out.x = lerp1(r0, r1, uz, tz);
+  out.y = lerp1(i0, i1, uz, tz);
+  return out;
 }
 
 // Host-side geometry setup shared by extraction and insertion.

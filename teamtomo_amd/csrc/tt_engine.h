@@ -1,0 +1,23 @@
+// Brick engine: brick-stationary central-slice gather and scatter (complex64).
+#pragma once
+
+#include "tt_common.h"
+
+namespace tt {
+
+// slices = extraction of `vol` at the rotations; writes every pixel of the
+// n_rot slices (a zero pass for pixels outside the volume or the disc, then
+// the brick gather for the rest).
+int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                   hipStream_t s);
+
+// gV (+)= A^T gS, A the extraction of geometry g.  LDS float accumulation per
+// brick, bricks flushed with global float atomics (order not fixed).
+int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot, f2v *gV,
+                   int accumulate, hipStream_t s);
+
+// (vol, w) (+)= insertion of `slices`, including the x=0-plane Friedel mirror.
+int engine_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64_t n_rot, f2v *vol,
+                  float *w, int accumulate, hipStream_t s);
+
+}  // namespace tt

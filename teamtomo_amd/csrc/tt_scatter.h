@@ -14,12 +14,6 @@ template <typename T>
 int scatter_insert(const SliceGeom &g, const typename Cx<T>::type *slices, const float *rot,
                    int64_t n_rot, typename Cx<T>::type *vol, T *w, int flags, hipStream_t s);
 
-// Brick-stationary LDS scatter (complex64): no global atomics.
-int brick_scatter_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot,
-                          f2v *gV, int deterministic, int accumulate, hipStream_t s);
-int brick_scatter_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64_t n_rot,
-                         f2v *vol, float *w, int deterministic, int accumulate, hipStream_t s);
-
 // Fill n elements of type T with zeros (a kernel, so it can be graph-captured).
 template <typename T> int zero_fill(T *p, int64_t n, hipStream_t s);
 

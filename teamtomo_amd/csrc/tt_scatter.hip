@@ -15,6 +15,7 @@
 
 #include "../../include/teamtomo_amd.h"
 #include "tt_common.h"
+#include "tt_engine.h"
 #include "tt_scatter.h"
 
 namespace tt {
@@ -132,8 +133,8 @@ template int zero_fill<double>(double *, int64_t, hipStream_t);
 int scatter_extract_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot,
                             f2v *gV, int flags, hipStream_t s) {
   const bool det = flags & TT_FLAG_DETERMINISTIC, overwrite = flags & TT_FLAG_OVERWRITE;
-  if (!(flags & TT_FLAG_PIXEL_ATOMIC))
-    return brick_scatter_adjoint(g, gS, rot, n_rot, gV, det, !overwrite, s);
+  // The brick engine accumulates in fixed point: deterministic as it stands.
+  if (!(flags & TT_FLAG_PIXEL_ATOMIC)) return engine_adjoint(g, gS, rot, n_rot, gV, !overwrite, s);
   if (det) return fail_invalid("the pixel-atomic scatter is not deterministic");
   int st;
   if (overwrite && (st = zero_fill<float>((float *)gV, 2 * (int64_t)g.vD * g.vH * g.vW, s))) return st;
@@ -153,8 +154,7 @@ int scatter_insert(const SliceGeom &g, const typename Cx<T>::type *slices, const
                    int64_t n_rot, typename Cx<T>::type *vol, T *w, int flags, hipStream_t s) {
   const bool det = flags & TT_FLAG_DETERMINISTIC, overwrite = flags & TT_FLAG_OVERWRITE;
   if (sizeof(T) == sizeof(float) && !(flags & TT_FLAG_PIXEL_ATOMIC))
-    return brick_scatter_insert(g, (const f2v *)slices, rot, n_rot, (f2v *)vol, (float *)w, det,
-                                !overwrite, s);
+    return engine_insert(g, (const f2v *)slices, rot, n_rot, (f2v *)vol, (float *)w, !overwrite, s);
   if (det) return fail_invalid("deterministic insertion is available for complex64 only");
   int st;
   const int64_t nv = (int64_t)g.vD * g.vH * g.vW;

@@ -16,7 +16,7 @@
 
 #include "../../include/teamtomo_amd.h"
 #include "tt_common.h"
-#include "tt_gather.h"
+#include "tt_engine.h"
 #include "tt_scatter.h"
 
 namespace tt {
@@ -62,6 +62,7 @@ SliceGeom make_geom(int img_d, int img_h, int img_w, int vol_d, int vol_h, int v
   g.inv_W32 = (float)(1.0 / (double)img_w);
   g.xtol = (double)(float)x_tolerance;
   g.xtol10 = (double)(float)(x_tolerance * 10.0);
+  g.tolx = g.xtol * g.ax;
   g.invWr_f = 1.0f / (float)g.Wr;
   return g;
 }
@@ -103,16 +104,9 @@ __global__ __launch_bounds__(kBlock) void extract_fwd_kernel(
     f4a8 a01 = *reinterpret_cast<const f4a8 *>(base + sy);
     f4a8 a10 = *reinterpret_cast<const f4a8 *>(base + sz);
     f4a8 a11 = *reinterpret_cast<const f4a8 *>(base + sz + sy);
-    float ux = 1.0f - k.tx, uy = 1.0f - k.ty, uz = 1.0f - k.tz;
-    // lerp along x, then y, then z
-    float r00 = a00.x * ux + a00.z * k.tx, i00 = a00.y * ux + a00.w * k.tx;
-    float r01 = a01.x * ux + a01.z * k.tx, i01 = a01.y * ux + a01.w * k.tx;
-    float r10 = a10.x * ux + a10.z * k.tx, i10 = a10.y * ux + a10.w * k.tx;
-    float r11 = a11.x * ux + a11.z * k.tx, i11 = a11.y * ux + a11.w * k.tx;
-    float r0 = r00 * uy + r01 * k.ty, i0 = i00 * uy + i01 * k.ty;
-    float r1 = r10 * uy + r11 * k.ty, i1 = i10 * uy + i11 * k.ty;
-    res.x = r0 * uz + r1 * k.tz;
-    res.y = i0 * uz + i1 * k.tz;
+    res = trilerp(f2v{a00.x, a00.y}, f2v{a00.z, a00.w}, f2v{a01.x, a01.y}, f2v{a01.z, a01.w},
+                  f2v{a10.x, a10.y}, f2v{a10.z, a10.w}, f2v{a11.x, a11.y}, f2v{a11.z, a11.w}, k.tz,
+                  k.ty, k.tx);
     if (c.conj) res.y = -res.y;
   }
   out[r_idx * g.P + p] = res;
@@ -371,9 +365,8 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img
   if (!volume || !slices) return fail_invalid("volume / slices pointer is NULL");
   SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, x_tolerance);
   hipStream_t s = (hipStream_t)stream;
-  const int path = g_gather_path;
-  if (path == TT_GATHER_BRICK && brick_gather_eligible(g, n_rot))
-    return brick_gather(g, (const f2v *)volume, rot_zyx, n_rot, (f2v *)slices, s);
+  if (g_gather_path != TT_GATHER_PIXEL)
+    return engine_extract(g, (const f2v *)volume, rot_zyx, n_rot, (f2v *)slices, s);
   int64_t bpr = (g.P + kBlock - 1) / kBlock;
   int64_t rots_per_launch = std::max<int64_t>(1, kMaxBlocksPerLaunch / bpr);
   for (int64_t r0 = 0; r0 < n_rot; r0 += rots_per_launch) {
