@@ -57,22 +57,21 @@ namespace {
 enum Mode { kGather = 0, kAdjoint = 1, kInsert = 2 };
 
 constexpr double kBoxMargin = 1e-6;  // widening of the candidate box (voxels)
-constexpr int kDiscRot = 32;         // rotations tested per discovery pass (<= 64 items)
-constexpr int kItems = 2 * kDiscRot;
-constexpr int kRows = 32;            // rows per row pass
 
 // Per-rotation plane data, shared by all bricks.
 struct PlaneRec {
-  double U[3], V[3];    // canonical constants (tt_common.h)
-  double iV[3];         // 1 / V_d, 0 where |V_d| <= 1e-30 (constant along a row)
-  double n[3];          // normal of span(A, B), A = U / H (per row), B = V (per column)
-  double pm[3], pj[3];  // rows of the pseudo-inverse: (m, j) of a point of the plane
-  double nn;            // |n|; negative for a degenerate matrix (no pseudo-inverse)
+  double U[3], V[3];  // canonical constants (tt_common.h)
+  double iV[3];       // 1 / V_d, 0 where |V_d| <= 1e-30 (constant along a row)
   int tie, pad;
 };
 
+// Plane test data (double4): the normal of span(A, B), A = U / H (per row),
+// B = V (per column), in array units, and its length (0 for a degenerate matrix).
+typedef double4 PlaneTest;
+
 __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float *__restrict__ rot,
-                                                        int n_rot, PlaneRec *__restrict__ out) {
+                                                        int n_rot, PlaneRec *__restrict__ out,
+                                                        double4 *__restrict__ test) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= n_rot) return;
   const Rot rr = load_rot(rot + (int64_t)r * 9, g);
@@ -86,28 +85,15 @@ __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float
     A[d] = rr.U[d] * g.inv_H;
     B[d] = rr.V[d];
   }
-  p.n[0] = A[1] * B[2] - A[2] * B[1];
-  p.n[1] = A[2] * B[0] - A[0] * B[2];
-  p.n[2] = A[0] * B[1] - A[1] * B[0];
-  p.nn = sqrt(p.n[0] * p.n[0] + p.n[1] * p.n[1] + p.n[2] * p.n[2]);
-  const double AA = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
-  const double BB = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
-  const double AB = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
-  const double det = AA * BB - AB * AB;
-  if (det > 1e-20 * AA * BB && AA > 0.0 && BB > 0.0) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      p.pm[d] = (BB * A[d] - AB * B[d]) / det;
-      p.pj[d] = (AA * B[d] - AB * A[d]) / det;
-    }
-  } else {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) p.pm[d] = p.pj[d] = 0.0;
-    p.nn = -1.0;
-  }
   p.tie = rr.tie ? 1 : 0;
   p.pad = 0;
   out[r] = p;
+  double4 t;
+  t.x = A[1] * B[2] - A[2] * B[1];
+  t.y = A[2] * B[0] - A[0] * B[2];
+  t.z = A[0] * B[1] - A[1] * B[0];
+  t.w = sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
+  test[r] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -163,29 +149,40 @@ template <int BZ, int BY, int BX, int MODE>
 struct Cfg {
   static constexpr int NW = 8;                          // waves per block
   static constexpr int T = NW * 64;
-  static constexpr int CH = (MODE == kGather) ? 4 : 2;  // pixels per chunk
-  // gather: cells lo..hc and their corners (B + 1 voxels per axis);
-  // scatter: owned voxels lo..hi with a one-voxel margin (B + 2 per axis)
-  static constexpr int OFF = (MODE == kGather) ? 0 : 1;
-  static constexpr int SX = BX + 1 + OFF, SY = BY + 1 + OFF, SZ = BZ + 1 + OFF;
+  static constexpr int CH = 2;  // pixels per chunk
+  // tile: the brick's cells lo..lo+B-1 and their corners, B + 1 voxels per axis
+  static constexpr int SX = BX + 1, SY = BY + 1, SZ = BZ + 1;
   static constexpr int NT = SX * SY * SZ;
   static constexpr int NACC = (MODE == kGather) ? 0 : ((MODE == kInsert) ? 3 : 2);
-  static constexpr int NV = BZ * BY * BX;  // owned voxels of a scatter brick
+  // upper shell of a scatter tile (voxels l_z = BZ, or l_y = BY, or l_x = BX),
+  // which belongs to the neighbouring bricks: Z face, then Y face, then X face
+  static constexpr int NSH = SY * SX + BZ * SX + BZ * BY;
 };
+
+template <int BZ, int BY, int BX>
+__device__ __forceinline__ int shell_index(int lz, int ly, int lx) {
+  constexpr int SX = BX + 1, SY = BY + 1;
+  if (lz == BZ) return ly * SX + lx;
+  if (ly == BY) return SY * SX + lz * SX + lx;
+  return SY * SX + BZ * SX + lz * BY + ly;
+}
+
+constexpr int kSegCap = 256;    // chunks listed per window
 
 struct ItemRec {
-  int r;     // rotation (relative to the launch)
-  int m0;    // first row, as m = i - H//2
-  int nrow;  // number of rows
-  int neg;   // Friedel half: 0 -> s = +1, 1 -> s = -1
+  int rn;          // rotation (relative to the launch) | Friedel half (s = -1) << 30
+  short m0, nrow;  // first row (as m = i - H//2) and number of rows
 };
 
+// One slice row's candidate pixels in tile-local FP32 coordinates: the
+// candidate q (column jl + q) sits at st + q * sb.  The FP32 values are
+// rounded from the canonical FP64 ones (|st| < ~30, so within ~3e-6 voxel);
+// pixels within 1e-5 voxel of an integer coordinate re-evaluate the exact
+// canonical FP64 coordinate instead.
 struct RowRec {
-  double C[3];     // canonical row constants fma(s U_d, fy, b_d) (x: s U_x fy)
-  double sV[3];    // s V_d
-  long long base;  // r * P + i * Wr
-  int jl;          // first candidate column
-  int info;        // i | neg << 16 | tie << 17
+  float st[3], sb[3];
+  int ji;        // first candidate column | row i << 16
+  unsigned rnt;  // rotation | Friedel half (s = -1) << 30 | tie << 31
 };
 
 // Fixed-point scale of the scatter: values are multiplied by
@@ -194,14 +191,22 @@ struct RowRec {
 // larger (or non-finite) value raises *flag and the call is redone with the
 // exact maximum (headroom 0).
 struct ScaleArgs {
-  const unsigned *maxbits;  // float bits of max|v| (sampled or exact)
+  const unsigned *maxbits;  // float bits of max|v| (sampled or exact), kMaxSlots slots
   unsigned *flag;           // set when a value exceeds the limit
   int headroom;             // bits of headroom above the sampled maximum
   int guard;                // 1: run only if *flag is set (the exact redo)
 };
 
+// The block maxima are combined with atomicMax into kMaxSlots words kSlotStride
+// words apart (separate cache lines, so the atomics do not queue on one
+// address); the maximum is the largest slot.
+constexpr int kMaxSlots = 32, kSlotStride = 32;
+
 __device__ __forceinline__ void scale_of(const ScaleArgs &sa, float &scale, float &limit) {
-  const float m = __uint_as_float(*sa.maxbits);
+  unsigned mb = 0u;
+#pragma unroll
+  for (int k = 0; k < kMaxSlots; ++k) mb = max(mb, sa.maxbits[k * kSlotStride]);
+  const float m = __uint_as_float(mb);
   int e = 0;
   if (m > 0.0f && m < 3.0e38f) frexpf(m, &e);
   scale = ldexpf(1.0f, 30 - sa.headroom - e);
@@ -209,31 +214,216 @@ __device__ __forceinline__ void scale_of(const ScaleArgs &sa, float &scale, floa
 }
 
 constexpr float kWScale = 1073741824.0f;  // 2^30: fixed-point scale of the weights
+constexpr float kNear = 1e-5f;            // FP32 guard band around integer coordinates
 
 typedef unsigned long long u64;
 
-__device__ __forceinline__ void lds_add_fixed(u64 *p, float x) {
-  const int q = __float2int_rn(x);
-  atomicAdd(p, (u64)(long long)q);
+__device__ __forceinline__ int cvt_floor_i32(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));  // floor and convert in one VALU op
+  return r;
+}
+
+// Adds round(w * v) (half up) as a 64-bit fixed-point value (|w v| < 2^31).
+__device__ __forceinline__ void lds_add_fixed(u64 *p, float w, float v) {
+  atomicAdd(p, (u64)(long long)cvt_floor_i32(__fmaf_rn(w, v, 0.5f)));
+}
+
+// The exact canonical FP64 cell of pixel (i, j) under rotation r and Friedel
+// half `neg`, for pixels whose FP32 tile coordinate lies within kNear of an
+// integer: snap, faces and the Friedel half decide exactly as in every other
+// kernel.  Returns false if the pixel belongs to the other half.
+template <bool INSERT>
+__device__ __forceinline__ bool exact_cell(const SliceGeom &g, const PlaneRec *__restrict__ pre, int r,
+                                           bool neg, bool tie, int i, int j, int to0, int to1, int to2,
+                                           int (&cell)[3], float (&frac)[3]) {
+  const PlaneRec &p = pre[r];
+  const double s = neg ? -1.0 : 1.0;
+  const double fyd = canon_fy(g, i), jd = (double)j;
+  const double cx1 = __fma_rn(jd, p.V[2], __dmul_rn(p.U[2], fyd));
+  const bool ok = canon_conj<INSERT>(g, tie, cx1) == neg;
+  double c[3];
+  c[0] = snap(__fma_rn(jd, s * p.V[0], __fma_rn(s * p.U[0], fyd, g.bz)));
+  c[1] = snap(__fma_rn(jd, s * p.V[1], __fma_rn(s * p.U[1], fyd, g.by)));
+  c[2] = snap(s * cx1);
+  const int top[3] = {g.vD - 1, g.vH - 1, g.vW - 1};
+  const int to[3] = {to0, to1, to2};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double fl = floor(c[d]);
+    int k = (int)fl;
+    double t = c[d] - fl;
+    if (k == top[d] && t == 0.0) {  // c = N-1 exactly: cell N-2 with fraction 1
+      k = top[d] - 1;
+      t = 1.0;
+    }
+    cell[d] = k - to[d];
+    frac[d] = (float)t;
+  }
+  return ok;
+}
+
+// Decoded brick geometry shared by the scatter flush and combine kernels.
+struct BrickBox {
+  int lo[3], ext[3];  // first cell and local index of the last cell per axis
+  bool last[3];       // the brick holds cell N-2: its upper voxel layer is the volume face
+};
+
+template <int BZ, int BY, int BX>
+__device__ __forceinline__ BrickBox brick_box(const SliceGeom &g, const BrickGrid &bg, int brick) {
+  BrickBox b;
+  const int bz = brick / (bg.nby * bg.nbx);
+  const int brem = brick - bz * bg.nby * bg.nbx;
+  const int by = brem / bg.nbx, bx = brem - by * bg.nbx;
+  const int B[3] = {BZ, BY, BX}, N[3] = {g.vD, g.vH, g.vW}, bi[3] = {bz, by, bx};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = bi[d] * B[d];
+    const int ch = min(b.lo[d] + B[d] - 1, N[d] - 2);
+    b.ext[d] = ch - b.lo[d];
+    b.last[d] = ch == N[d] - 2;
+  }
+  return b;
+}
+
+// Writes a scatter brick's tile (NACC fixed-point planes of NT words):
+// voxels it owns (local l in [0, ext], plus ext + 1 on axes where the brick
+// is last) are converted and stored (the insertion's x = 0 plane into
+// plane/wplane for the Friedel mirror pass); the upper shell (l = B on an
+// axis where the brick is not last) belongs to the next brick and goes to
+// the halo buffer as fixed-point partial sums.
+template <int BZ, int BY, int BX, int MODE>
+__device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astride, int v,
+                                            const BrickBox &bb, const SliceGeom &g, int brick,
+                                            float inv_v, int accumulate, f2v *__restrict__ out,
+                                            float *__restrict__ wout, f2v *__restrict__ plane,
+                                            float *__restrict__ wplane, u64 *__restrict__ halo) {
+  typedef Cfg<BZ, BY, BX, MODE> C;
+  constexpr int SX = C::SX, SY = C::SY, NA = C::NACC, NSH = C::NSH;
+  const int lz = v / (SY * SX), ly = (v / SX) % SY, lx = v % SX;
+  if (lz > bb.ext[0] + 1 || ly > bb.ext[1] + 1 || lx > bb.ext[2] + 1) return;
+  const bool shell = (lz == bb.ext[0] + 1 && !bb.last[0]) || (ly == bb.ext[1] + 1 && !bb.last[1]) ||
+                     (lx == bb.ext[2] + 1 && !bb.last[2]);
+  if (shell) {
+    u64 *hb = halo + (int64_t)brick * NA * NSH;
+    const int si = shell_index<BZ, BY, BX>(lz, ly, lx);
+#pragma unroll
+    for (int k = 0; k < NA; ++k) hb[k * NSH + si] = a[k * astride];
+    return;
+  }
+  const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
+  const int gz = bb.lo[0] + lz, gy = bb.lo[1] + ly, gx = bb.lo[2] + lx;
+  f2v val = {(float)(long long)a[0] * inv_v, (float)(long long)a[astride] * inv_v};
+  if (MODE == kInsert) {
+    const float w = (float)(long long)a[2 * astride] * (1.0f / kWScale);
+    if (gx == 0) {  // the x = 0 plane goes through the Friedel mirror pass
+      plane[(int64_t)gz * g.vH + gy] = val;
+      wplane[(int64_t)gz * g.vH + gy] = w;
+      return;
+    }
+    const int64_t e = gz * sz + gy * sy + gx;
+    wout[e] = accumulate ? wout[e] + w : w;
+    out[e] = accumulate ? out[e] + val : val;
+  } else {
+    const int64_t e = gz * sz + gy * sy + gx;
+    out[e] = accumulate ? out[e] + val : val;
+  }
+}
+
+template <int BZ, int BY, int BX, int MODE>
+__device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, const BrickGrid &bg,
+                           int brick, float vscale, int accumulate, f2v *__restrict__ out,
+                           float *__restrict__ wout, f2v *__restrict__ plane,
+                           float *__restrict__ wplane, u64 *__restrict__ halo, int tid, int nthreads) {
+  constexpr int NT = Cfg<BZ, BY, BX, MODE>::NT;
+  const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
+  const float inv_v = 1.0f / vscale;
+  for (int v = tid; v < NT; v += nthreads)
+    flush_voxel<BZ, BY, BX, MODE>(acc + v, NT, v, bb, g, brick, inv_v, accumulate, out, wout, plane,
+                                  wplane, halo);
+}
+
+// Rows m (= i - H//2) of the slice, for Friedel sign s, whose pixels can have
+// coordinates in the box [Lb, Ub]: the projection onto m of the polygon
+//   {(m, j) : Lb <= beta + s (A m + V j) <= Ub, 0 <= j <= Wr - 1},  A = U / H,
+// by Fourier-Motzkin elimination of j (each pair of a lower and an upper
+// bound on j gives a bound on m).  The bounds use the hardware reciprocal
+// (relative error far below 1e-6) and are widened by 1e-3 (relative) plus
+// 1e-2 rows, so the range always holds every row with a pixel in the box;
+// rows it adds in excess have empty column intervals.  False if empty.
+__device__ __forceinline__ bool item_rows(const PlaneRec &p, double s, const double Lb[3],
+                                          const double Ub[3], const double beta[3], double inv_H,
+                                          int Wr, double mmin, double mmax, int &m0, int &m1) {
+  // constraints alpha m + b j <= gamma: lower bounds on j (b < 0), upper (b > 0)
+  double la[4], lb[4], lg[4], ua[4], ubv[4], ug[4];
+  double lo = mmin, hi = mmax;
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double A = s * p.U[d] * inv_H, B = s * p.V[d];
+    const double gu = Ub[d] - beta[d], gl = beta[d] - Lb[d];  // A m + B j <= gu; -A m - B j <= gl
+    const bool pos = B > 0.0;
+    ua[d] = pos ? A : -A;
+    ubv[d] = pos ? B : -B;
+    ug[d] = pos ? gu : gl;
+    la[d] = pos ? -A : A;
+    lb[d] = pos ? -B : B;
+    lg[d] = pos ? gl : gu;
+    // B == 0: the row's coordinate along d does not depend on j
+    const bool dir = B == 0.0;
+    const double rA = __builtin_amdgcn_rcp(A);
+    const double q1 = gu * rA, q2 = -gl * rA;  // A m <= gu, -A m <= gl
+    hi = (dir && A > 0.0) ? fmin(hi, q1) : hi;
+    lo = (dir && A > 0.0) ? fmax(lo, q2) : lo;
+    lo = (dir && A < 0.0) ? fmax(lo, q1) : lo;
+    hi = (dir && A < 0.0) ? fmin(hi, q2) : hi;
+    ok = ok && !(dir && A == 0.0 && (gu < 0.0 || gl < 0.0));
+  }
+  la[3] = 0.0; lb[3] = -1.0; lg[3] = 0.0;                   // j >= 0
+  ua[3] = 0.0; ubv[3] = 1.0; ug[3] = (double)(Wr - 1);      // j <= Wr - 1
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      // a feasible j exists iff m (ua lb - la ub) >= ug lb - lg ub  (lb < 0 < ub)
+      const double kap = ua[b] * lb[a] - la[a] * ubv[b];
+      const double lam = ug[b] * lb[a] - lg[a] * ubv[b];
+      const double tol = 1e-12 * (fabs(ua[b] * lb[a]) + fabs(la[a] * ubv[b]));
+      const bool valid = lb[a] < 0.0 && ubv[b] > 0.0;
+      const double q = lam * __builtin_amdgcn_rcp(kap);
+      lo = (valid && kap > tol) ? fmax(lo, q) : lo;
+      hi = (valid && kap < -tol) ? fmin(hi, q) : hi;
+    }
+  lo -= 1e-3 * fabs(lo) + 1e-2;
+  hi += 1e-3 * fabs(hi) + 1e-2;
+  lo = fmax(lo, mmin);
+  hi = fmin(hi, mmax);
+  if (!ok || !(lo <= hi)) return false;
+  m0 = (int)ceil(lo);
+  m1 = (int)floor(hi);
+  return m0 <= m1;
 }
 
 // ---------------------------------------------------------------------------
 // The brick kernel (gather, adjoint or insertion)
 // ---------------------------------------------------------------------------
-template <int BZ, int BY, int BX, int MODE>
-__global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T)) void brick_kernel(
+template <int BZ, int BY, int BX, int MODE, bool MASK>
+__global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
-    const Unit *__restrict__ units, int n_rot, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ segbuf, ScaleArgs sa,
+    const double4 *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
+    f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
+    float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
     int accumulate) {
   typedef Cfg<BZ, BY, BX, MODE> C;
-  constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT, OFF = C::OFF;
+  constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
   // gather: NT complex voxels; scatter: NACC fixed-point planes of NT words
   constexpr int NWORDS = SCATTER ? C::NACC * NT : NT;
   __shared__ u64 s_tile[NWORDS];
-  __shared__ ItemRec s_item[NW][kItems];
-  __shared__ RowRec s_row[NW][kRows];
+  __shared__ ItemRec s_item[NW][128];
+  __shared__ int s_cross[NW][128];  // rotations whose plane meets the brick, pending
+  __shared__ RowRec s_row[NW][64];
+  __shared__ int s_seg[NW][kSegCap];  // row | pixel count << 6 | first pixel << 9
 
   if (SCATTER && sa.guard && *sa.flag == 0u) return;  // exact redo not needed
   const Unit un = units[blockIdx.x];
@@ -244,28 +434,10 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T)) void brick_kernel(
   const int brem = brick - bz * bg.nby * bg.nbx;
   const int by = brem / bg.nbx, bx = brem - by * bg.nbx;
   const int lo0 = bz * BZ, lo1 = by * BY, lo2 = bx * BX;
-  // candidate cells [cl, ch]: gather, the brick's own cells; scatter, every
-  // cell with a corner among the owned voxels lo..hi
-  int cl0, cl1, cl2, ch0, ch1, ch2;
-  if (SCATTER) {
-    cl0 = max(lo0 - 1, 0);
-    cl1 = max(lo1 - 1, 0);
-    cl2 = max(lo2 - 1, 0);
-    ch0 = min(lo0 + BZ - 1, g.vD - 2);
-    ch1 = min(lo1 + BY - 1, g.vH - 2);
-    ch2 = min(lo2 + BX - 1, g.vW - 2);
-  } else {
-    cl0 = lo0;
-    cl1 = lo1;
-    cl2 = lo2;
-    ch0 = min(lo0 + BZ - 1, g.vD - 2);
-    ch1 = min(lo1 + BY - 1, g.vH - 2);
-    ch2 = min(lo2 + BX - 1, g.vW - 2);
-  }
-  const unsigned ext0 = (unsigned)(ch0 - cl0), ext1 = (unsigned)(ch1 - cl1), ext2 = (unsigned)(ch2 - cl2);
-  const int to0 = lo0 - OFF, to1 = lo1 - OFF, to2 = lo2 - OFF;  // tile origin (voxels)
-  const bool top0 = ch0 == g.vD - 2, top1 = ch1 == g.vH - 2, top2 = ch2 == g.vW - 2;
-  const bool conj_check = cl2 == 0;  // only bricks at x = 0 see both Friedel halves
+  // the brick's cells lo..ch (cells run 0..N-2 per axis)
+  const int ch0 = min(lo0 + BZ - 1, g.vD - 2), ch1 = min(lo1 + BY - 1, g.vH - 2),
+            ch2 = min(lo2 + BX - 1, g.vW - 2);
+  const unsigned ext0 = (unsigned)(ch0 - lo0), ext1 = (unsigned)(ch1 - lo1), ext2 = (unsigned)(ch2 - lo2);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   float vscale = 1.0f, vlimit = 3.0e38f;
@@ -286,9 +458,10 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T)) void brick_kernel(
   }
   __syncthreads();
 
-  const double Lb[3] = {(double)cl0 - kBoxMargin, (double)cl1 - kBoxMargin, (double)cl2 - kBoxMargin};
+  const double Lb[3] = {(double)lo0 - kBoxMargin, (double)lo1 - kBoxMargin, (double)lo2 - kBoxMargin};
   const double Ub[3] = {(double)(ch0 + 1) + kBoxMargin, (double)(ch1 + 1) + kBoxMargin,
                         (double)(ch2 + 1) + kBoxMargin};
+  const double tor[3] = {(double)lo0, (double)lo1, (double)lo2};
   const double beta[3] = {g.bz, g.by, 0.0};
   double ctr[3], half[3];
 #pragma unroll
@@ -304,93 +477,108 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T)) void brick_kernel(
   const int r_end = (int)(ub + (ue - ub) * (wave + 1) / NW);
 
   ItemRec *items = s_item[wave];
+  int *cross = s_cross[wave];
   RowRec *rows = s_row[wave];
-  const bool have_cells = nseg_raw > 0 && ch0 >= cl0 && ch1 >= cl1 && ch2 >= cl2;
+  int *segs = s_seg[wave];
+  const bool have_cells = nseg_raw > 0 && ch0 >= lo0 && ch1 >= lo1 && ch2 >= lo2;
+  const double cb0 = ctr[0] - beta[0], cb1 = ctr[1] - beta[1], cb2 = ctr[2] - beta[2];
 
-  for (int rd = r_beg; have_cells && rd < r_end; rd += kDiscRot) {
-    // (1) discovery: lanes 0..kDiscRot-1 test one rotation each
-    bool h0 = false, h1 = false;
-    int m0a = 0, m1a = -1, m0b = 0, m1b = -1;
-    const int r = rd + lane;
-    if (lane < kDiscRot && r < r_end) {
-      const PlaneRec &p = pre[r];
-      const double dist = p.n[0] * (ctr[0] - beta[0]) + p.n[1] * (ctr[1] - beta[1]) +
-                          p.n[2] * (ctr[2] - beta[2]);
-      const double rad = fabs(p.n[0]) * half[0] + fabs(p.n[1]) * half[1] + fabs(p.n[2]) * half[2];
-      if (p.nn < 0.0 || fabs(dist) <= rad + 1e-9 * p.nn) {
-        double mc = 0.0, mr = 1e300, jc = 0.0, jr = 1e300;
-        if (p.nn >= 0.0) {
-          mr = jr = 0.0;
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            mc += p.pm[d] * (ctr[d] - beta[d]);
-            jc += p.pj[d] * (ctr[d] - beta[d]);
-            mr += fabs(p.pm[d]) * half[d];
-            jr += fabs(p.pj[d]) * half[d];
-          }
-          mr += 1e-6 * (1.0 + fabs(mc));
-          jr += 1e-6 * (1.0 + fabs(jc));
-        }
-#pragma unroll
-        for (int sh = 0; sh < 2; ++sh) {
-          const double s = sh ? -1.0 : 1.0;
-          if (s * jc + jr < 0.0 || s * jc - jr > (double)(g.Wr - 1)) continue;
-          const double mlo = fmax(s * mc - mr, mmin), mhi = fmin(s * mc + mr, mmax);
-          if (mlo > mhi) continue;
-          const int m0 = (int)ceil(mlo), m1 = (int)floor(mhi);
-          if (m0 > m1) continue;
-          if (sh == 0) {
-            h0 = true;
-            m0a = m0;
-            m1a = m1;
-          } else {
-            h1 = true;
-            m0b = m0;
-            m1b = m1;
-          }
-        }
+  int rd = r_beg, ncross = 0;
+  while (have_cells) {
+    // (1a) plane test, 64 rotations per pass: rotations whose plane meets the
+    // brick's coordinate box are listed (at most 127 pending)
+    while (ncross < 64 && rd < r_end) {
+      const int r = rd + lane;
+      bool hit = false;
+      if (r < r_end) {
+        const double4 pt = ptest[r];  // one 32-byte load: normal, |normal|
+        const double dist = pt.x * cb0 + pt.y * cb1 + pt.z * cb2;
+        const double rad = fabs(pt.x) * half[0] + fabs(pt.y) * half[1] + fabs(pt.z) * half[2];
+        hit = (pt.w <= 0.0) | (fabs(dist) <= rad + 1e-9 * pt.w);
       }
+      const unsigned long long bm = __ballot(hit);
+      if (hit) cross[ncross + rank_below(bm)] = r;
+      ncross += __popcll(bm);
+      rd += 64;
+    }
+    if (ncross == 0) break;
+    __builtin_amdgcn_wave_barrier();
+    // (1b) items: per listed rotation and Friedel half, the exact range of
+    // slice rows whose pixels can reach the box
+    const int take = min(ncross, 64);
+    bool h0 = false, h1 = false;
+    int m0a = 0, m1a = -1, m0b = 0, m1b = -1, r = 0;
+    if (lane < take) {
+      r = cross[lane];
+      const PlaneRec &p = pre[r];
+      h0 = item_rows(p, 1.0, Lb, Ub, beta, g.inv_H, g.Wr, mmin, mmax, m0a, m1a);
+      h1 = item_rows(p, -1.0, Lb, Ub, beta, g.inv_H, g.Wr, mmin, mmax, m0b, m1b);
+    }
+    {  // keep the rest of the list
+      const int rest = ncross - take;
+      const int v = lane < rest ? cross[take + lane] : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < rest) cross[lane] = v;
+      ncross = rest;
     }
     const unsigned long long b0 = __ballot(h0), b1 = __ballot(h1);
     const int n0 = __popcll(b0);
     const int nitems = n0 + __popcll(b1);
-    if (h0) items[rank_below(b0)] = ItemRec{r, m0a, m1a - m0a + 1, 0};
-    if (h1) items[n0 + rank_below(b1)] = ItemRec{r, m0b, m1b - m0b + 1, 1};
+    if (h0) items[rank_below(b0)] = ItemRec{r, (short)m0a, (short)(m1a - m0a + 1)};
+    if (h1) items[n0 + rank_below(b1)] = ItemRec{r | (1 << 30), (short)m0b, (short)(m1b - m0b + 1)};
     __builtin_amdgcn_wave_barrier();
     if (nitems == 0) continue;
 
-    // (2) rows of all items: lane = item for the prefix
-    const int nrow_l = lane < nitems ? items[lane].nrow : 0;
-    const int rincl = wave_incl_scan(nrow_l);
-    const int total_rows = __shfl(rincl, 63, 64);
-    for (int rb = 0; rb < total_rows; rb += kRows) {
-      int len = 0;
+    // (2) inclusive prefix of the items' row counts, held in registers:
+    // item k's end in lane k (ia) or lane k - 64 (ib)
+    const int ra = lane < nitems ? (int)items[lane].nrow : 0;
+    const int rb_ = lane + 64 < nitems ? (int)items[lane + 64].nrow : 0;
+    const int ia = wave_incl_scan(ra);
+    const int ib = wave_incl_scan(rb_) + __builtin_amdgcn_readlane(ia, 63);
+    const int total_rows = __builtin_amdgcn_readlane(ib, 63);
+
+    for (int rb = 0; rb < total_rows; rb += 64) {
+      // item of each lane's row: the few items ending inside this pass are
+      // walked with wave-uniform reads of the prefix
       const int t = rb + lane;
-      // item of row t: searched with every lane active (cross-lane reads)
-      const int k = find_segment<64>(rincl, min(t, total_rows - 1));
-      const int excl_raw = __shfl(rincl, k > 0 ? k - 1 : 0, 64);
-      if (lane < kRows && t < total_rows) {
+      const int kf = __popcll(__ballot(lane < nitems && ia <= rb)) +
+                     __popcll(__ballot(lane + 64 < nitems && ib <= rb));
+      int k = kf;
+      int excl = 0;
+      if (kf > 0) excl = kf - 1 < 64 ? __builtin_amdgcn_readlane(ia, kf - 1)
+                                     : __builtin_amdgcn_readlane(ib, kf - 65);
+      for (int kk = kf; kk < nitems; ++kk) {
+        const int e = kk < 64 ? __builtin_amdgcn_readlane(ia, kk) : __builtin_amdgcn_readlane(ib, kk - 64);
+        if (e > rb + 63) break;
+        if (e <= t) {
+          k = kk + 1;
+          excl = e;
+        }
+      }
+      // one lane per row: canonical row constants and the candidate columns
+      int len = 0;
+      if (t < total_rows) {
         const ItemRec it = items[k];
-        const int excl = k > 0 ? excl_raw : 0;
-        const int m = it.m0 + (t - excl);
+        const int m = (int)it.m0 + (t - excl);
         const int i = m + g.half_H;
-        const double s = it.neg ? -1.0 : 1.0;
-        const PlaneRec &p = pre[it.r];
+        const int ir = it.rn & 0x3fffffff, ineg = (it.rn >> 30) & 1;
+        const double s = ineg ? -1.0 : 1.0;
+        const PlaneRec &p = pre[ir];
         const double fy = canon_fy(g, i);
-        RowRec rec;
-        rec.C[0] = __fma_rn(s * p.U[0], fy, g.bz);
-        rec.C[1] = __fma_rn(s * p.U[1], fy, g.by);
-        rec.C[2] = s * __dmul_rn(p.U[2], fy);
+        double Cr[3], sV[3];
+        Cr[0] = __fma_rn(s * p.U[0], fy, g.bz);
+        Cr[1] = __fma_rn(s * p.U[1], fy, g.by);
+        Cr[2] = s * __dmul_rn(p.U[2], fy);
         double jlo = 0.0, jhi = (double)(g.Wr - 1);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          rec.sV[d] = s * p.V[d];
+          sV[d] = s * p.V[d];
           if (p.iV[d] != 0.0) {
             const double inv = s * p.iV[d];
-            const double t1 = (Lb[d] - rec.C[d]) * inv, t2 = (Ub[d] - rec.C[d]) * inv;
+            const double t1 = (Lb[d] - Cr[d]) * inv, t2 = (Ub[d] - Cr[d]) * inv;
             jlo = fmax(jlo, fmin(t1, t2));
             jhi = fmin(jhi, fmax(t1, t2));
-          } else if (rec.C[d] < Lb[d] || rec.C[d] > Ub[d]) {
+          } else if (Cr[d] < Lb[d] || Cr[d] > Ub[d]) {
             jhi = -1.0;
           }
         }
@@ -403,96 +591,134 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T)) void brick_kernel(
         }
         if (MODE == kInsert && i < g.half_H) jl = max(jl, 1);  // x = 0, y < 0 is skipped
         len = max(0, jh - jl + 1);
-        rec.base = (long long)it.r * g.P + (long long)i * g.Wr;
-        rec.jl = jl;
-        rec.info = i | (it.neg << 16) | (p.tie << 17);
+        RowRec rec;
+        const double jld = (double)jl;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          rec.st[d] = (float)(__fma_rn(jld, sV[d], Cr[d]) - tor[d]);
+          rec.sb[d] = (float)sV[d];
+        }
+        rec.ji = jl | (i << 16);
+        rec.rnt = (unsigned)ir | ((unsigned)ineg << 30) | ((unsigned)p.tie << 31);
         rows[lane] = rec;
       }
       const int nch = (len + CH - 1) / CH;
       const int cincl = wave_incl_scan(nch);
+      const int cexcl = cincl - nch;
       const int total_ch = __shfl(cincl, 63, 64);
-      __builtin_amdgcn_wave_barrier();
 
-      // (3) chunks of CH pixels, one per lane
-      for (int pc = 0; pc < total_ch; pc += 64) {
-        const int pidx = pc + lane;
-        const int k = find_segment<kRows>(cincl, pidx);
-        const int cexcl_raw = __shfl(cincl, k > 0 ? k - 1 : 0, 64);
-        const int cexcl = k > 0 ? cexcl_raw : 0;
-        const int len_k = __shfl(len, k, 64);
-        const bool live = pidx < total_ch;
-        const int q0 = (pidx - cexcl) * CH;
-        const RowRec rec = rows[k];
-        const int i = rec.info & 0xffff;
-        const bool neg = (rec.info >> 16) & 1;
-        const bool tie = (rec.info >> 17) & 1;
+      // (3) the rows' chunks of CH pixels are listed (windows of kSegCap),
+      // then taken one per lane, consecutive lanes on consecutive chunks
+      for (int w0 = 0; w0 < total_ch; w0 += kSegCap) {
+        const int c0 = max(cexcl, w0), c1 = min(cexcl + nch, w0 + kSegCap);
+        for (int c = c0; c < c1; ++c) {
+          const int q0 = (c - cexcl) * CH;
+          segs[c - w0] = lane | (min(CH, len - q0) << 6) | (q0 << 9);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int wn = min(total_ch - w0, kSegCap);
+        for (int pc = 0; pc < wn; pc += 64) {
+          const bool live = pc + lane < wn;
+          const int sgv = segs[live ? pc + lane : 0];
+          const int k = sgv & 63, cnt = live ? (sgv >> 6) & 7 : 0, q0 = sgv >> 9;
+          const RowRec rec = rows[k];
+          const int jl = rec.ji & 0xffff, i = rec.ji >> 16;
+          const int rr = (int)(rec.rnt & 0x3fffffffu);
+          const bool neg = (rec.rnt >> 30) & 1;
+          const int64_t base = (int64_t)rr * g.P + (int64_t)i * g.Wr + jl + q0;
+          // FP32 tile coordinates of the chunk's pixels
+          int c3[CH][3];
+          float t3[CH][3];
+          bool own[CH];
+          bool anynear = false;
 #pragma unroll
-        for (int q = 0; q < CH; ++q) {
-          const int jj = q0 + q;
-          const int j = rec.jl + jj;
-          bool own = live && jj < len_k;
-          const double jd = (double)j;
-          double cz = __fma_rn(jd, rec.sV[0], rec.C[0]);
-          double cy = __fma_rn(jd, rec.sV[1], rec.C[1]);
-          double cx = __fma_rn(jd, rec.sV[2], rec.C[2]);
-          if (conj_check) {
-            const double cx1 = neg ? -cx : cx;
-            own = own && (canon_conj<MODE == kInsert>(g, tie, cx1) == neg);
+          for (int q = 0; q < CH; ++q) {
+            own[q] = q < cnt;
+            const float qf = (float)(q0 + q);
+            float tmin = 1.0f, tmax = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const float l = fmaf(qf, rec.sb[d], rec.st[d]);
+              const float fl = floorf(l);
+              t3[q][d] = l - fl;
+              c3[q][d] = (int)fl;
+              tmin = fminf(tmin, t3[q][d]);
+              tmax = fmaxf(tmax, t3[q][d]);
+            }
+            anynear = anynear || (own[q] && (tmin < kNear || tmax > 1.0f - kNear));
           }
-          double fz = floor(cz), fy_ = floor(cy), fx = floor(cx);
-          double tz = cz - fz, ty = cy - fy_, tx = cx - fx;
-          const bool near = (tz < 2e-9) | (tz > 1.0 - 2e-9) | (ty < 2e-9) | (ty > 1.0 - 2e-9) |
-                            (tx < 2e-9) | (tx > 1.0 - 2e-9);
-          if (near) {  // rare: the canonical snap can move the coordinate
-            cz = snap(cz);
-            cy = snap(cy);
-            cx = snap(cx);
-            fz = floor(cz);
-            fy_ = floor(cy);
-            fx = floor(cx);
-            tz = cz - fz;
-            ty = cy - fy_;
-            tx = cx - fx;
+          if (__ballot(anynear)) {  // rare, wave-uniform: exact FP64 cells near integers
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+              const float tmin = fminf(fminf(t3[q][0], t3[q][1]), t3[q][2]);
+              const float tmax = fmaxf(fmaxf(t3[q][0], t3[q][1]), t3[q][2]);
+              if (own[q] && (tmin < kNear || tmax > 1.0f - kNear))
+                own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (rec.rnt >> 31) & 1, i, jl + q0 + q,
+                                                     lo0, lo1, lo2, c3[q], t3[q]);
+            }
           }
-          int z0 = (int)fz, y0 = (int)fy_, x0 = (int)fx;
-          // c = N-1 exactly: the cell is clamped to N-2 with fraction 1
-          if (top0 && z0 == g.vD - 1 && tz == 0.0) { z0 = g.vD - 2; tz = 1.0; }
-          if (top1 && y0 == g.vH - 1 && ty == 0.0) { y0 = g.vH - 2; ty = 1.0; }
-          if (top2 && x0 == g.vW - 1 && tx == 0.0) { x0 = g.vW - 2; tx = 1.0; }
-          own = own && (unsigned)(z0 - cl0) <= ext0 && (unsigned)(y0 - cl1) <= ext1 &&
-                (unsigned)(x0 - cl2) <= ext2;
-          if (g.use_mask) own = own && in_disc(g, i, j);
-          const int e = own ? ((z0 - to0) * SY + (y0 - to1)) * SX + (x0 - to2) : 0;
-          const float ftz = (float)tz, fty = (float)ty, ftx = (float)tx;
+          int e[CH];
+#pragma unroll
+          for (int q = 0; q < CH; ++q) {
+            own[q] = own[q] && (unsigned)c3[q][0] <= ext0 && (unsigned)c3[q][1] <= ext1 &&
+                     (unsigned)c3[q][2] <= ext2;
+            if (MASK) own[q] = own[q] && in_disc(g, i, jl + q0 + q);
+            e[q] = own[q] ? (c3[q][0] * SY + c3[q][1]) * SX + c3[q][2] : 0;
+          }
           if (MODE == kGather) {
-            const f2v *b = reinterpret_cast<const f2v *>(s_tile) + e;
-            f2v val = trilerp(b[0], b[1], b[SX], b[SX + 1], b[SY * SX], b[SY * SX + 1],
-                              b[SY * SX + SX], b[SY * SX + SX + 1], ftz, fty, ftx);
-            if (neg) val.y = -val.y;
-            if (own) out[rec.base + j] = val;
-          } else if (own) {
-            f2v v = src[rec.base + j];
-            if (neg) v.y = -v.y;
-            bad = bad || !(fabsf(v.x) < vlimit && fabsf(v.y) < vlimit);
-            v.x *= vscale;
-            v.y *= vscale;
-            const float wz[2] = {1.0f - ftz, ftz}, wy[2] = {1.0f - fty, fty}, wx[2] = {1.0f - ftx, ftx};
+            const f2v *tv = reinterpret_cast<const f2v *>(s_tile);
+            f2v cv[CH][8];
 #pragma unroll
-            for (int dz = 0; dz < 2; ++dz)
+            for (int q = 0; q < CH; ++q) {
+              const f2v *b = tv + e[q];
+              cv[q][0] = b[0];
+              cv[q][1] = b[1];
+              cv[q][2] = b[SX];
+              cv[q][3] = b[SX + 1];
+              cv[q][4] = b[SY * SX];
+              cv[q][5] = b[SY * SX + 1];
+              cv[q][6] = b[SY * SX + SX];
+              cv[q][7] = b[SY * SX + SX + 1];
+            }
 #pragma unroll
-              for (int dy = 0; dy < 2; ++dy) {
-                const float wzy = wz[dz] * wy[dy];
+            for (int q = 0; q < CH; ++q) {
+              f2v val = trilerp(cv[q][0], cv[q][1], cv[q][2], cv[q][3], cv[q][4], cv[q][5], cv[q][6],
+                                cv[q][7], t3[q][0], t3[q][1], t3[q][2]);
+              if (neg) val.y = -val.y;
+              if (own[q]) out[base + q] = val;
+            }
+          } else {
+            f2v vv[CH];
 #pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                  const float w = wzy * wx[dx];
-                  const int o = e + dz * SY * SX + dy * SX + dx;
-                  lds_add_fixed(&s_tile[o], w * v.x);
-                  lds_add_fixed(&s_tile[NT + o], w * v.y);
-                  if (MODE == kInsert) lds_add_fixed(&s_tile[2 * NT + o], w * kWScale);
+            for (int q = 0; q < CH; ++q) vv[q] = src[own[q] ? base + q : base];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+              if (!own[q]) continue;
+              f2v v = vv[q];
+              if (neg) v.y = -v.y;
+              bad = bad || !(fabsf(v.x) < vlimit && fabsf(v.y) < vlimit);
+              v.x *= vscale;
+              v.y *= vscale;
+              const float tz = t3[q][0], ty = t3[q][1], tx = t3[q][2];
+              const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
+#pragma unroll
+              for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy) {
+                  const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+                  for (int dx = 0; dx < 2; ++dx) {
+                    const float w = wzy * wx[dx];
+                    const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                    lds_add_fixed(&s_tile[o], w, v.x);
+                    lds_add_fixed(&s_tile[NT + o], w, v.y);
+                    if (MODE == kInsert) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                  }
                 }
-              }
+            }
           }
         }
+        __builtin_amdgcn_wave_barrier();  // the next window rewrites the list
       }
       __builtin_amdgcn_wave_barrier();  // the next row pass rewrites this wave's row table
     }
@@ -502,105 +728,162 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T)) void brick_kernel(
   if (!SCATTER) return;
   if (__ballot(bad) && lane == 0) atomicOr(sa.flag, 1u);
   __syncthreads();
-  // flush the owned voxels lo..hi (tile coordinates 1..B per axis)
-  const int hi0 = min(lo0 + BZ - 1, g.vD - 1), hi1 = min(lo1 + BY - 1, g.vH - 1),
-            hi2 = min(lo2 + BX - 1, g.vW - 1);
-  const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
-  const float inv_v = 1.0f / vscale;
-  constexpr int NV = C::NV;
-  for (int v = tid; v < NV; v += C::T) {
-    const int lz = v / (BY * BX), ly = (v / BX) % BY, lx = v % BX;
-    const int gz = lo0 + lz, gy = lo1 + ly, gx = lo2 + lx;
-    if (gz > hi0 || gy > hi1 || gx > hi2) continue;
-    const int o = ((lz + 1) * SY + (ly + 1)) * SX + (lx + 1);
-    if (nseg > 1) {  // partial sums of a rotation segment: combined later
-      u64 *dst = segbuf + ((int64_t)un.slot + seg) * (C::NACC * NV);
-      dst[v] = s_tile[o];
-      dst[NV + v] = s_tile[NT + o];
-      if (MODE == kInsert) dst[2 * NV + v] = s_tile[2 * NT + o];
-      continue;
-    }
-    f2v val = {(float)(long long)s_tile[o] * inv_v, (float)(long long)s_tile[NT + o] * inv_v};
-    if (MODE == kInsert) {
-      const float w = (float)(long long)s_tile[2 * NT + o] * (1.0f / kWScale);
-      if (gx == 0) {  // the x = 0 plane goes through the Friedel mirror pass
-        plane[(int64_t)gz * g.vH + gy] = val;
-        wplane[(int64_t)gz * g.vH + gy] = w;
-        continue;
-      }
-      const int64_t e = gz * sz + gy * sy + gx;
-      wout[e] = accumulate ? wout[e] + w : w;
-      out[e] = accumulate ? out[e] + val : val;
-    } else {
-      const int64_t e = gz * sz + gy * sy + gx;
-      out[e] = accumulate ? out[e] + val : val;
-    }
+  if (nseg > 1) {  // partial sums of a rotation segment: combined by split_combine_kernel
+    u64 *dst = segbuf + ((int64_t)un.slot + seg) * (C::NACC * NT);
+    for (int v = tid; v < C::NACC * NT; v += C::T) dst[v] = s_tile[v];
+    return;
   }
+  flush_tile<BZ, BY, BX, MODE>(s_tile, g, bg, brick, vscale, accumulate, out, wout, plane, wplane,
+                               halo, tid, C::T);
 }
 
-// Sums the rotation segments of the split bricks (integer sums: the result
-// does not depend on the order) and writes them like the brick flush.
+// Sums the rotation segments of a split brick (integer sums: the result does
+// not depend on the order) and flushes the tile like an unsplit brick; one
+// thread per tile voxel, blockIdx.x over the split bricks.
 template <int BZ, int BY, int BX, int MODE>
-__global__ __launch_bounds__(256) void segment_combine_kernel(
+__global__ __launch_bounds__(256) void split_combine_kernel(
     SliceGeom g, BrickGrid bg, const SplitRec *__restrict__ splits, const u64 *__restrict__ segbuf,
     ScaleArgs sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane) {
+    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo) {
   typedef Cfg<BZ, BY, BX, MODE> C;
-  constexpr int NV = C::NV, NA = C::NACC;
+  constexpr int NT = C::NT, NA = C::NACC;
   if (sa.guard && *sa.flag == 0u) return;
   const SplitRec sp = splits[blockIdx.x];
-  const int bz = sp.brick / (bg.nby * bg.nbx);
-  const int brem = sp.brick - bz * bg.nby * bg.nbx;
-  const int by = brem / bg.nbx, bx = brem - by * bg.nbx;
-  const int lo0 = bz * BZ, lo1 = by * BY, lo2 = bx * BX;
+  const int v = blockIdx.y * 256 + threadIdx.x;
+  if (v >= NT) return;
+  u64 acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = 0ull;
+  for (int sgi = 0; sgi < sp.nseg; ++sgi) {
+    const u64 *b = segbuf + ((int64_t)sp.slot + sgi) * (NA * NT);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] += b[a * NT + v];
+  }
+  float vscale, vlimit;
+  scale_of(sa, vscale, vlimit);
+  const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, sp.brick);
+  flush_voxel<BZ, BY, BX, MODE>(acc, 1, v, bb, g, sp.brick, 1.0f / vscale, accumulate, out, wout, plane,
+                                wplane, halo);
+}
+
+// Adds the halo partial sums of the lower neighbours to the voxels of a
+// brick's lower shell (l = 0 on an axis where the brick has a neighbour
+// below): a voxel on a face, edge or corner receives 1, 3 or 7 halos.
+template <int BZ, int BY, int BX, int MODE>
+__global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGrid bg,
+                                                           const u64 *__restrict__ halo, ScaleArgs sa,
+                                                           f2v *__restrict__ out,
+                                                           float *__restrict__ wout,
+                                                           f2v *__restrict__ plane,
+                                                           float *__restrict__ wplane) {
+  typedef Cfg<BZ, BY, BX, MODE> C;
+  constexpr int SX = C::SX, SY = C::SY, NA = C::NACC, NSH = C::NSH;
+  if (sa.guard && *sa.flag == 0u) return;
+  const int brick = blockIdx.x;
+  const int u = blockIdx.y * 256 + threadIdx.x;
+  if (u >= NSH) return;
+  const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
+  const int B[3] = {BZ, BY, BX};
+  const int bstride[3] = {bg.nby * bg.nbx, bg.nbx, 1};
   float vscale, vlimit;
   scale_of(sa, vscale, vlimit);
   const float inv_v = 1.0f / vscale;
   const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
-  for (int v = threadIdx.x; v < NV; v += 256) {
-    const int lz = v / (BY * BX), ly = (v / BX) % BY, lx = v % BX;
-    const int gz = lo0 + lz, gy = lo1 + ly, gx = lo2 + lx;
-    if (gz >= g.vD || gy >= g.vH || gx >= g.vW) continue;
-    u64 acc[3] = {0ull, 0ull, 0ull};
-    for (int sgi = 0; sgi < sp.nseg; ++sgi) {
-      const u64 *b = segbuf + ((int64_t)sp.slot + sgi) * (NA * NV);
-#pragma unroll
-      for (int a = 0; a < NA; ++a) acc[a] += b[a * NV + v];
+  {
+    // lower shell in the same order as the upper one: Z face, Y face, X face
+    int l[3];
+    if (u < SY * SX) {
+      l[0] = 0;
+      l[1] = u / SX;
+      l[2] = u % SX;
+    } else if (u < SY * SX + BZ * SX) {
+      const int w = u - SY * SX;
+      l[0] = 1 + w / SX;
+      l[1] = 0;
+      l[2] = w % SX;
+    } else {
+      const int w = u - SY * SX - BZ * SX;
+      l[0] = 1 + w / BY;
+      l[1] = 1 + w % BY;
+      l[2] = 0;
     }
-    f2v val = {(float)(long long)acc[0] * inv_v, (float)(long long)acc[1] * inv_v};
+    bool owned = true;
+    int amask = 0;  // axes on which the voxel receives a lower neighbour's halo
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      owned = owned && l[d] <= bb.ext[d] + (bb.last[d] ? 1 : 0);
+      if (l[d] == 0 && bb.lo[d] > 0) amask |= 1 << d;
+    }
+    if (!owned || amask == 0) return;
+    u64 acc[3] = {0ull, 0ull, 0ull};
+    // the nonempty subsets of amask, unrolled so that every load is in flight at once
+#pragma unroll
+    for (int sbits = 1; sbits < 8; ++sbits) {
+      if ((sbits & amask) != sbits) continue;
+      int nb = brick, lp[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const bool in = (sbits >> d) & 1;
+        nb -= in ? bstride[d] : 0;
+        lp[d] = l[d] + (in ? B[d] : 0);
+      }
+      const int si = shell_index<BZ, BY, BX>(lp[0], lp[1], lp[2]);
+      const u64 *hb = halo + (int64_t)nb * NA * NSH;
+#pragma unroll
+      for (int a = 0; a < NA; ++a) acc[a] += hb[a * NSH + si];
+    }
+    const int gz = bb.lo[0] + l[0], gy = bb.lo[1] + l[1], gx = bb.lo[2] + l[2];
+    const f2v val = {(float)(long long)acc[0] * inv_v, (float)(long long)acc[1] * inv_v};
     if (MODE == kInsert) {
       const float w = (float)(long long)acc[2] * (1.0f / kWScale);
       if (gx == 0) {
-        plane[(int64_t)gz * g.vH + gy] = val;
-        wplane[(int64_t)gz * g.vH + gy] = w;
-        continue;
+        const int64_t pe = (int64_t)gz * g.vH + gy;
+        plane[pe] += val;
+        wplane[pe] += w;
+        return;
       }
       const int64_t e = gz * sz + gy * sy + gx;
-      wout[e] = accumulate ? wout[e] + w : w;
-      out[e] = accumulate ? out[e] + val : val;
+      wout[e] += w;
+      out[e] += val;
     } else {
       const int64_t e = gz * sz + gy * sy + gx;
-      out[e] = accumulate ? out[e] + val : val;
+      out[e] += val;
     }
   }
 }
 
-// max(|re|, |im|) of one 128-byte line in every 64 (or of every value when
-// stride_lines == 1), as float bits into *res.
-__global__ __launch_bounds__(256) void maxabs_lines_kernel(const f2v *__restrict__ x, int64_t n,
-                                                           int64_t stride_lines, const unsigned *guard,
-                                                           unsigned *__restrict__ res) {
+// max(|re|, |im|) as float bits into *res, over chunks of 8192 consecutive
+// values (64 KiB) taken every `stride_chunks` chunks: 1 reads everything, 64
+// a 1/64 sample of contiguous blocks.  Four loads in flight per lane; one
+// atomic per block, into slot blockIdx.x % kMaxSlots.
+__global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restrict__ x, int64_t n,
+                                                            int64_t stride_chunks,
+                                                            const unsigned *guard,
+                                                            unsigned *__restrict__ res) {
+  __shared__ float s_m[4];
   if (guard && *guard == 0u) return;
   const int64_t lanes = (int64_t)gridDim.x * 256;
   float m = 0.0f;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;; t += lanes) {
-    const int64_t e = (t >> 4) * stride_lines * 16 + (t & 15);
-    if (e >= n) break;
-    const f2v v = x[e];
-    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  for (int64_t t0 = (int64_t)blockIdx.x * 256 + threadIdx.x;; t0 += 4 * lanes) {
+    f2v v[4];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t t = t0 + u * lanes;
+      const int64_t e = (t >> 13) * stride_chunks * 8192 + (t & 8191);
+      v[u] = e < n ? x[e] : f2v{0.0f, 0.0f};
+      any = any || e < n;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) m = fmaxf(m, fmaxf(fabsf(v[u].x), fabsf(v[u].y)));
+    if (!any) break;
   }
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(res, __float_as_uint(m));
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(res + (blockIdx.x % kMaxSlots) * kSlotStride,
+              __float_as_uint(fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]))));
 }
 
 __global__ void set_word_kernel(unsigned *p, unsigned v) {
@@ -788,7 +1071,7 @@ constexpr int kTargetSlots = 256 * 2 * 4;  // CUs x resident blocks x units per 
 // dropped (gather) or kept as empty units that write zeros (scatter).
 int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n_rot, PlanEntry &pe) {
   int dev = 0;
-  hipGetDevice(&dev);
+  (void)hipGetDevice(&dev);
   int64_t nq = 64;
   while (nq < n_rot) nq <<= 1;
   float fm = g.use_mask ? g.fmax : -1.0f;
@@ -802,10 +1085,9 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
     pe = it->second;
     return TT_OK;
   }
-  // gather bricks tile the cells 0..N-2, scatter bricks the voxels 0..N-1
-  const int sz_ = scatter ? 0 : 1;
-  const int nbz = (g.vD - sz_ + BZ - 1) / BZ, nby = (g.vH - sz_ + BY - 1) / BY,
-            nbx = (g.vW - sz_ + BX - 1) / BX;
+  // bricks tile the cells 0..N-2 of each axis
+  const int nbz = (g.vD - 1 + BZ - 1) / BZ, nby = (g.vH - 1 + BY - 1) / BY,
+            nbx = (g.vW - 1 + BX - 1) / BX;
   // largest distance of a slice sample from DC (array units), plus margin
   const double amax = std::max(g.az, std::max(g.ay, g.ax));
   const double fy_max = 0.5 + 1.0 / g.H, fx_max = 0.5 + 1.0 / (2.0 * (g.Wr - 1) + 1.0);
@@ -832,7 +1114,7 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
         double dmin2 = 0.0, dc2 = 0.0;
         for (int d = 0; d < 3; ++d) {
           const int lo = b3[d] * Bd[d];
-          const double clo = scatter ? std::max(lo - 1, 0) : lo;
+          const double clo = lo;
           const double chi = std::min(lo + Bd[d] - 1, Nv[d] - 2) + 1.0;
           const double q = std::max(0.0, std::max(clo - c[d], c[d] - chi));
           dmin2 += q * q;
@@ -911,26 +1193,26 @@ struct Scratch {
     return TT_OK;
   }
   ~Scratch() {
-    if (p) hipFreeAsync(p, s);
+    if (p) (void)hipFreeAsync(p, s);
   }
 };
 
 constexpr int kGBZ = 16, kGBY = 16, kGBX = 8;  // gather brick (cells)
-constexpr int kABZ = 16, kABY = 16, kABX = 8;  // adjoint brick (owned voxels)
-constexpr int kIBZ = 16, kIBY = 8, kIBX = 8;   // insertion brick (owned voxels)
+constexpr int kABZ = 16, kABY = 16, kABX = 8;  // adjoint brick (cells)
+constexpr int kIBZ = 16, kIBY = 8, kIBX = 8;   // insertion brick (cells)
 
-BrickGrid grid_of(const SliceGeom &g, int BZ, int BY, int BX, bool scatter) {
-  const int sz_ = scatter ? 0 : 1;
+BrickGrid grid_of(const SliceGeom &g, int BZ, int BY, int BX) {
   BrickGrid bg;
-  bg.nbz = (g.vD - sz_ + BZ - 1) / BZ;
-  bg.nby = (g.vH - sz_ + BY - 1) / BY;
-  bg.nbx = (g.vW - sz_ + BX - 1) / BX;
+  bg.nbz = (g.vD - 1 + BZ - 1) / BZ;
+  bg.nby = (g.vH - 1 + BY - 1) / BY;
+  bg.nbx = (g.vW - 1 + BX - 1) / BX;
   return bg;
 }
 
 int check_engine(const SliceGeom &g, int64_t n_rot) {
   if (n_rot >= ((int64_t)1 << 30)) return fail_invalid("too many rotations for one call (< 2^30)");
   if (g.H >= 32768) return fail_invalid("slice height must be < 32768");
+  if (g.Wr >= 65536) return fail_invalid("slice width must be < 131070");
   return TT_OK;
 }
 
@@ -941,10 +1223,13 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   typedef Cfg<BZ, BY, BX, MODE> C;
   PlanEntry pe;
   if (int st = get_plan(g, BZ, BY, BX, true, n_rot, pe)) return st;
-  const BrickGrid bg = grid_of(g, BZ, BY, BX, true);
+  const BrickGrid bg = grid_of(g, BZ, BY, BX);
+  const int64_t n_bricks = (int64_t)bg.nbz * bg.nby * bg.nbx;
   const size_t np = (size_t)g.vD * g.vH;
-  // scratch: words (max bits sampled, flag, max bits exact) | plane | wplane | PlaneRec | segments
-  size_t off = 256;
+  // scratch: words (max slots sampled, max slots exact, flag) | plane | wplane |
+  // PlaneRec | PlaneTest | rotation-segment slots | halo shells
+  constexpr int kSetWords = kMaxSlots * kSlotStride;
+  size_t off = ((2 * kSetWords + 1) * sizeof(unsigned) + 255) & ~(size_t)255;
   const size_t off_plane = off;
   if (MODE == kInsert) off += np * sizeof(f2v);
   const size_t off_wplane = off;
@@ -954,8 +1239,14 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   const size_t off_pre = off;
   off += (size_t)std::max<int64_t>(n_rot, 1) * sizeof(PlaneRec);
   off = (off + 255) & ~(size_t)255;
+  const size_t off_test = off;
+  off += (size_t)std::max<int64_t>(n_rot, 1) * sizeof(PlaneTest);
+  off = (off + 255) & ~(size_t)255;
   const size_t off_seg = off;
-  off += (size_t)pe.n_slots * C::NACC * C::NV * sizeof(u64);
+  off += (size_t)pe.n_slots * C::NACC * C::NT * sizeof(u64);
+  off = (off + 255) & ~(size_t)255;
+  const size_t off_halo = off;
+  off += (size_t)n_bricks * C::NACC * C::NSH * sizeof(u64);
   Scratch sc(s);
   if (int st = sc.alloc(off)) return st;
   char *base = (char *)sc.p;
@@ -963,43 +1254,57 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   f2v *plane = MODE == kInsert ? (f2v *)(base + off_plane) : nullptr;
   float *wplane = MODE == kInsert ? (float *)(base + off_wplane) : nullptr;
   PlaneRec *pre = (PlaneRec *)(base + off_pre);
+  PlaneTest *ptest = (PlaneTest *)(base + off_test);
   u64 *segbuf = (u64 *)(base + off_seg);
+  u64 *halo = (u64 *)(base + off_halo);
   if (int st = fill_zero<unsigned>(words, (int64_t)(zero_end / sizeof(unsigned)), s)) return st;
   const int64_t n = n_rot * (int64_t)g.P;
   // accumulate: exact max (the outputs cannot be rewritten); otherwise a
   // sampled max with 8 bits of headroom and an exact redo if it was exceeded
   const bool sampled = !accumulate;
-  const int mgrid = (int)std::min<int64_t>(std::max<int64_t>(n / 4096, 1), 4096);
+  const int mgrid = (int)std::min<int64_t>(std::max<int64_t>(n / 4096, 1), 2048);
   if (n_rot > 0) {
-    hipLaunchKernelGGL(maxabs_lines_kernel, dim3(mgrid), dim3(256), 0, s, src, n,
+    // sampled: one 8192-value chunk in 64, about four values per lane
+    const int64_t nsamp = (n / (64 * 8192) + 1) * 8192;
+    const int sgrid = (int)std::min<int64_t>((nsamp + 1023) / 1024, 2048);
+    hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(sampled ? sgrid : mgrid), dim3(256), 0, s, src, n,
                        (int64_t)(sampled ? 64 : 1), (const unsigned *)nullptr, words);
     hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g,
-                       rot, (int)n_rot, pre);
-    if (int st = check_launch("maxabs_lines / plane_rec")) return st;
+                       rot, (int)n_rot, pre, ptest);
+    if (int st = check_launch("maxabs_chunks / plane_rec")) return st;
   }
   for (int pass = 0; pass < (sampled ? 2 : 1); ++pass) {
     ScaleArgs sa;
-    sa.maxbits = words + (pass ? 2 : 0);
-    sa.flag = words + 1;
+    sa.maxbits = words + (pass ? kSetWords : 0);
+    sa.flag = words + 2 * kSetWords;
     sa.headroom = (sampled && pass == 0) ? 8 : 0;
     sa.guard = pass;
     if (pass == 1 && n_rot > 0) {
-      hipLaunchKernelGGL(maxabs_lines_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
-                         (const unsigned *)(words + 1), words + 2);
-      if (int st = check_launch("maxabs_lines_kernel")) return st;
+      hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
+                         (const unsigned *)(words + 2 * kSetWords), words + kSetWords);
+      if (int st = check_launch("maxabs_chunks_kernel")) return st;
     }
     if (pe.n_units > 0) {
-      hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE>), dim3((unsigned)pe.n_units), dim3(C::T), 0,
-                         s, src, g, bg, (const PlaneRec *)pre, (const Unit *)pe.units, (int)n_rot, out,
-                         wout, plane, wplane, segbuf, sa, accumulate);
+      if (g.use_mask)
+        hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, true>), dim3((unsigned)pe.n_units), dim3(C::T),
+                           0, s, src, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units, (int)n_rot,
+                           out, wout, plane, wplane, segbuf, halo, sa, accumulate);
+      else
+        hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, false>), dim3((unsigned)pe.n_units), dim3(C::T),
+                           0, s, src, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units, (int)n_rot,
+                           out, wout, plane, wplane, segbuf, halo, sa, accumulate);
       if (int st = check_launch("brick_kernel")) return st;
     }
     if (pe.n_splits > 0) {
-      hipLaunchKernelGGL((segment_combine_kernel<BZ, BY, BX, MODE>), dim3((unsigned)pe.n_splits),
-                         dim3(256), 0, s, g, bg, (const SplitRec *)pe.splits, (const u64 *)segbuf, sa,
-                         accumulate, out, wout, plane, wplane);
-      if (int st = check_launch("segment_combine_kernel")) return st;
+      hipLaunchKernelGGL((split_combine_kernel<BZ, BY, BX, MODE>),
+                         dim3((unsigned)pe.n_splits, (unsigned)((C::NT + 255) / 256)), dim3(256), 0, s, g, bg, (const SplitRec *)pe.splits, (const u64 *)segbuf, sa, accumulate,
+                         out, wout, plane, wplane, halo);
+      if (int st = check_launch("split_combine_kernel")) return st;
     }
+    hipLaunchKernelGGL((halo_combine_kernel<BZ, BY, BX, MODE>),
+                       dim3((unsigned)n_bricks, (unsigned)((C::NSH + 255) / 256)), dim3(256), 0, s,
+                       g, bg, (const u64 *)halo, sa, out, wout, plane, wplane);
+    if (int st = check_launch("halo_combine_kernel")) return st;
   }
   if (MODE == kInsert) {
     hipLaunchKernelGGL(mirror_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, plane, wplane,
@@ -1024,17 +1329,26 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
   if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
   if (pe.n_units == 0) return TT_OK;
   Scratch sc(s);
-  if (int st = sc.alloc((size_t)n_rot * sizeof(PlaneRec))) return st;
+  const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
+  if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
   PlaneRec *pre = (PlaneRec *)sc.p;
+  PlaneTest *ptest = (PlaneTest *)((char *)sc.p + off_test);
   hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
-                     (int)n_rot, pre);
+                     (int)n_rot, pre, ptest);
   if (int st = check_launch("plane_rec_kernel")) return st;
-  const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX, false);
+  const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
   ScaleArgs sa = {nullptr, nullptr, 0, 0};
-  hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather>), dim3((unsigned)pe.n_units),
-                     dim3(Cfg<kGBZ, kGBY, kGBX, kGather>::T), 0, s, vol, g, bg, (const PlaneRec *)pre,
-                     (const Unit *)pe.units, (int)n_rot, out, (float *)nullptr, (f2v *)nullptr,
-                     (float *)nullptr, (u64 *)nullptr, sa, 0);
+  constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
+  if (g.use_mask)
+    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)pe.n_units), dim3(T),
+                       0, s, vol, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units, (int)n_rot, out,
+                       (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
+                       (u64 *)nullptr, sa, 0);
+  else
+    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)pe.n_units),
+                       dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units,
+                       (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
+                       (u64 *)nullptr, (u64 *)nullptr, sa, 0);
   return check_launch("brick_kernel");
 }
 
