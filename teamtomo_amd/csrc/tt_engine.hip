@@ -58,20 +58,24 @@ enum Mode { kGather = 0, kAdjoint = 1, kInsert = 2 };
 
 constexpr double kBoxMargin = 1e-6;  // widening of the candidate box (voxels)
 
-// Per-rotation plane data, shared by all bricks.
+// Per-rotation plane data, shared by all bricks.  The slice plane of pixel
+// (m, j), m = i - H//2, is x = beta + m A + j B with A = U / H, B = V (array
+// units); (As, Bs) is the dual basis of (A, B) in the plane, so that a point x
+// of the plane has m = (x - beta) . As and j = (x - beta) . Bs.
 struct PlaneRec {
-  double U[3], V[3];  // canonical constants (tt_common.h)
-  double iV[3];       // 1 / V_d, 0 where |V_d| <= 1e-30 (constant along a row)
-  int tie, pad;
+  double U[3], V[3];    // canonical constants (tt_common.h)
+  double As[3], Bs[3];  // dual basis (0 for a degenerate plane)
+  int tie;              // extraction conj decision on the x' = 0 plane
+  int full;             // degenerate plane: every row and column is a candidate
 };
 
-// Plane test data (double4): the normal of span(A, B), A = U / H (per row),
-// B = V (per column), in array units, and its length (0 for a degenerate matrix).
-typedef double4 PlaneTest;
+// Plane test data: the unit normal of span(A, B) in FP32; w < 0 marks a
+// degenerate plane (every brick is a candidate).
+typedef float4 PlaneTest;
 
 __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float *__restrict__ rot,
                                                         int n_rot, PlaneRec *__restrict__ out,
-                                                        double4 *__restrict__ test) {
+                                                        PlaneTest *__restrict__ test) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= n_rot) return;
   const Rot rr = load_rot(rot + (int64_t)r * 9, g);
@@ -81,18 +85,32 @@ __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float
   for (int d = 0; d < 3; ++d) {
     p.U[d] = rr.U[d];
     p.V[d] = rr.V[d];
-    p.iV[d] = fabs(rr.V[d]) > 1e-30 ? 1.0 / rr.V[d] : 0.0;
     A[d] = rr.U[d] * g.inv_H;
     B[d] = rr.V[d];
   }
+  const double aa = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+  const double bb = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+  const double ab = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+  const double det = aa * bb - ab * ab;
+  const bool full = !(det > 1e-12 * aa * bb) || !(aa > 0.0) || !(bb > 0.0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    p.As[d] = full ? 0.0 : (bb * A[d] - ab * B[d]) / det;
+    p.Bs[d] = full ? 0.0 : (aa * B[d] - ab * A[d]) / det;
+  }
   p.tie = rr.tie ? 1 : 0;
-  p.pad = 0;
+  p.full = full ? 1 : 0;
   out[r] = p;
-  double4 t;
-  t.x = A[1] * B[2] - A[2] * B[1];
-  t.y = A[2] * B[0] - A[0] * B[2];
-  t.z = A[0] * B[1] - A[1] * B[0];
-  t.w = sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
+  double n0 = A[1] * B[2] - A[2] * B[1];
+  double n1 = A[2] * B[0] - A[0] * B[2];
+  double n2 = A[0] * B[1] - A[1] * B[0];
+  const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  float4 t;
+  if (full || !(nn > 0.0)) {
+    t = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+  } else {
+    t = make_float4((float)(n0 / nn), (float)(n1 / nn), (float)(n2 / nn), 1.0f);
+  }
   test[r] = t;
 }
 
@@ -109,6 +127,19 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     if (lane >= off) v += t;
   }
   return v;
+}
+
+__device__ __forceinline__ float uniform_f(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+// A wave-uniform double moved to scalar registers (VALU results live in vector
+// registers even when every lane holds the same value).
+__device__ __forceinline__ double uniform_d(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 __device__ __forceinline__ int rank_below(unsigned long long mask) {
@@ -169,21 +200,25 @@ __device__ __forceinline__ int shell_index(int lz, int ly, int lx) {
 
 constexpr int kSegCap = 256;    // chunks listed per window
 
-struct ItemRec {
-  int rn;          // rotation (relative to the launch) | Friedel half (s = -1) << 30
-  short m0, nrow;  // first row (as m = i - H//2) and number of rows
-};
-
-// One slice row's candidate pixels in tile-local FP32 coordinates: the
-// candidate q (column jl + q) sits at st + q * sb.  The FP32 values are
-// rounded from the canonical FP64 ones (|st| < ~30, so within ~3e-6 voxel);
-// pixels within 1e-5 voxel of an integer coordinate re-evaluate the exact
-// canonical FP64 coordinate instead.
-struct RowRec {
-  float st[3], sb[3];
-  int ji;        // first candidate column | row i << 16
+// A slice's item in a brick: a rotation and Friedel half whose rows can reach
+// the brick, with its FP32 frame.  The coordinate of pixel (i, j) relative to
+// the tile centre is st + (i - i_ref) sa + (j - j_ref) sv; st is the
+// canonical FP64 coordinate of the reference pixel (the one nearest the tile
+// centre) rounded to FP32, the steps are rounded from FP64.  Over the ~30
+// rows and columns of a tile this stays within ~5e-6 voxel of the canonical
+// value (about 1e-6 in practice); pixels within kNear of an integer
+// coordinate re-evaluate the exact canonical FP64 coordinate instead.
+struct __align__(16) ItemRec {
+  float st[3];   // tile-local coordinate of the reference pixel
+  float sa[3];   // step per slice row
+  float sv[3];   // step per slice column
+  int rows;      // first row i0 | row count << 16
+  unsigned ref;  // reference row i_ref | reference column j_ref << 16
   unsigned rnt;  // rotation | Friedel half (s = -1) << 30 | tie << 31
 };
+
+constexpr int kTake = 32;              // rotations per item pass (two halves each)
+constexpr float kRowMargin = 1e-4f;    // widening of the row phase's tile box (voxels)
 
 // Fixed-point scale of the scatter: values are multiplied by
 // 2^(30 - headroom - e), with 2^e > the sampled max|v|, so a value up to
@@ -214,7 +249,7 @@ __device__ __forceinline__ void scale_of(const ScaleArgs &sa, float &scale, floa
 }
 
 constexpr float kWScale = 1073741824.0f;  // 2^30: fixed-point scale of the weights
-constexpr float kNear = 1e-5f;            // FP32 guard band around integer coordinates
+constexpr float kNear = 2e-5f;            // FP32 guard band around integer coordinates
 
 typedef unsigned long long u64;
 
@@ -343,87 +378,34 @@ __device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, cons
                                   wplane, halo);
 }
 
-// Rows m (= i - H//2) of the slice, for Friedel sign s, whose pixels can have
-// coordinates in the box [Lb, Ub]: the projection onto m of the polygon
-//   {(m, j) : Lb <= beta + s (A m + V j) <= Ub, 0 <= j <= Wr - 1},  A = U / H,
-// by Fourier-Motzkin elimination of j (each pair of a lower and an upper
-// bound on j gives a bound on m).  The bounds use the hardware reciprocal
-// (relative error far below 1e-6) and are widened by 1e-3 (relative) plus
-// 1e-2 rows, so the range always holds every row with a pixel in the box;
-// rows it adds in excess have empty column intervals.  False if empty.
-__device__ __forceinline__ bool item_rows(const PlaneRec &p, double s, const double Lb[3],
-                                          const double Ub[3], const double beta[3], double inv_H,
-                                          int Wr, double mmin, double mmax, int &m0, int &m1) {
-  // constraints alpha m + b j <= gamma: lower bounds on j (b < 0), upper (b > 0)
-  double la[4], lb[4], lg[4], ua[4], ubv[4], ug[4];
-  double lo = mmin, hi = mmax;
-  bool ok = true;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double A = s * p.U[d] * inv_H, B = s * p.V[d];
-    const double gu = Ub[d] - beta[d], gl = beta[d] - Lb[d];  // A m + B j <= gu; -A m - B j <= gl
-    const bool pos = B > 0.0;
-    ua[d] = pos ? A : -A;
-    ubv[d] = pos ? B : -B;
-    ug[d] = pos ? gu : gl;
-    la[d] = pos ? -A : A;
-    lb[d] = pos ? -B : B;
-    lg[d] = pos ? gl : gu;
-    // B == 0: the row's coordinate along d does not depend on j
-    const bool dir = B == 0.0;
-    const double rA = __builtin_amdgcn_rcp(A);
-    const double q1 = gu * rA, q2 = -gl * rA;  // A m <= gu, -A m <= gl
-    hi = (dir && A > 0.0) ? fmin(hi, q1) : hi;
-    lo = (dir && A > 0.0) ? fmax(lo, q2) : lo;
-    lo = (dir && A < 0.0) ? fmax(lo, q1) : lo;
-    hi = (dir && A < 0.0) ? fmin(hi, q2) : hi;
-    ok = ok && !(dir && A == 0.0 && (gu < 0.0 || gl < 0.0));
-  }
-  la[3] = 0.0; lb[3] = -1.0; lg[3] = 0.0;                   // j >= 0
-  ua[3] = 0.0; ubv[3] = 1.0; ug[3] = (double)(Wr - 1);      // j <= Wr - 1
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      // a feasible j exists iff m (ua lb - la ub) >= ug lb - lg ub  (lb < 0 < ub)
-      const double kap = ua[b] * lb[a] - la[a] * ubv[b];
-      const double lam = ug[b] * lb[a] - lg[a] * ubv[b];
-      const double tol = 1e-12 * (fabs(ua[b] * lb[a]) + fabs(la[a] * ubv[b]));
-      const bool valid = lb[a] < 0.0 && ubv[b] > 0.0;
-      const double q = lam * __builtin_amdgcn_rcp(kap);
-      lo = (valid && kap > tol) ? fmax(lo, q) : lo;
-      hi = (valid && kap < -tol) ? fmin(hi, q) : hi;
-    }
-  lo -= 1e-3 * fabs(lo) + 1e-2;
-  hi += 1e-3 * fabs(hi) + 1e-2;
-  lo = fmax(lo, mmin);
-  hi = fmin(hi, mmax);
-  if (!ok || !(lo <= hi)) return false;
-  m0 = (int)ceil(lo);
-  m1 = (int)floor(hi);
-  return m0 <= m1;
-}
-
 // ---------------------------------------------------------------------------
 // The brick kernel (gather, adjoint or insertion)
 // ---------------------------------------------------------------------------
 template <int BZ, int BY, int BX, int MODE, bool MASK>
 __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
-    const double4 *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
+    const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
     f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
     float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
     int accumulate) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
+  static_assert(CH == 2, "the chunk list holds pixel counts 1 and 2");
   // gather: NT complex voxels; scatter: NACC fixed-point planes of NT words
   constexpr int NWORDS = SCATTER ? C::NACC * NT : NT;
-  __shared__ u64 s_tile[NWORDS];
-  __shared__ ItemRec s_item[NW][128];
-  __shared__ int s_cross[NW][128];  // rotations whose plane meets the brick, pending
-  __shared__ RowRec s_row[NW][64];
-  __shared__ int s_seg[NW][kSegCap];  // row | pixel count << 6 | first pixel << 9
+  // one LDS block, the tile first: the trilinear corner offsets then fit the
+  // offset fields of the LDS instructions
+  struct Smem {
+    u64 tile[NWORDS];
+    ItemRec item[NW][2 * kTake];
+    int cross[NW][128];  // rotations whose plane meets the brick, pending
+    // chunks: item | row of the pass << 6 | (pixel count - 1) << 12 | first column << 13
+    unsigned seg[NW][kSegCap];
+    int rowi[NW][64];  // slice row i of each row of the pass
+  };
+  __shared__ Smem sm;
+  u64 *s_tile = sm.tile;
 
   if (SCATTER && sa.guard && *sa.flag == 0u) return;  // exact redo not needed
   const Unit un = units[blockIdx.x];
@@ -458,61 +440,155 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
   }
   __syncthreads();
 
-  const double Lb[3] = {(double)lo0 - kBoxMargin, (double)lo1 - kBoxMargin, (double)lo2 - kBoxMargin};
-  const double Ub[3] = {(double)(ch0 + 1) + kBoxMargin, (double)(ch1 + 1) + kBoxMargin,
-                        (double)(ch2 + 1) + kBoxMargin};
-  const double tor[3] = {(double)lo0, (double)lo1, (double)lo2};
+  // brick constants: uniform over the block, held in scalar registers
+  const double Lb[3] = {uniform_d((double)lo0 - kBoxMargin), uniform_d((double)lo1 - kBoxMargin),
+                        uniform_d((double)lo2 - kBoxMargin)};
+  const double Ub[3] = {uniform_d((double)(ch0 + 1) + kBoxMargin), uniform_d((double)(ch1 + 1) + kBoxMargin),
+                        uniform_d((double)(ch2 + 1) + kBoxMargin)};
+  // origin of the FP32 item frames: the tile centre, which halves the FP32
+  // rounding of the coordinates against the tile corner
+  constexpr int HZ = BZ / 2, HY = BY / 2, HX = BX / 2;
+  const double tor[3] = {uniform_d((double)(lo0 + HZ)), uniform_d((double)(lo1 + HY)),
+                         uniform_d((double)(lo2 + HX))};
   const double beta[3] = {g.bz, g.by, 0.0};
   double ctr[3], half[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    ctr[d] = 0.5 * (Lb[d] + Ub[d]);
-    half[d] = 0.5 * (Ub[d] - Lb[d]);
+    ctr[d] = uniform_d(0.5 * (Lb[d] + Ub[d]));
+    half[d] = uniform_d(0.5 * (Ub[d] - Lb[d]));
   }
-  const double mmin = (double)(-g.half_H), mmax = (double)(g.H - 1 - g.half_H);
+  const double mmin = uniform_d((double)(-g.half_H)), mmax = uniform_d((double)(g.H - 1 - g.half_H));
 
   // this wave's share of the unit's rotations
   const int64_t ub = (int64_t)n_rot * seg / nseg, ue = (int64_t)n_rot * (seg + 1) / nseg;
   const int r_beg = (int)(ub + (ue - ub) * wave / NW);
   const int r_end = (int)(ub + (ue - ub) * (wave + 1) / NW);
 
-  ItemRec *items = s_item[wave];
-  int *cross = s_cross[wave];
-  RowRec *rows = s_row[wave];
-  int *segs = s_seg[wave];
+  ItemRec *items = sm.item[wave];
+  int *cross = sm.cross[wave];
+  unsigned *segs = sm.seg[wave];
+  int *rowi = sm.rowi[wave];
   const bool have_cells = nseg_raw > 0 && ch0 >= lo0 && ch1 >= lo1 && ch2 >= lo2;
-  const double cb0 = ctr[0] - beta[0], cb1 = ctr[1] - beta[1], cb2 = ctr[2] - beta[2];
+  const double cb0 = uniform_d(ctr[0] - beta[0]), cb1 = uniform_d(ctr[1] - beta[1]),
+               cb2 = uniform_d(ctr[2] - beta[2]);
+  // candidate box of the row phase in frame coordinates, widened well beyond
+  // the FP32 error
+  const float Lt[3] = {(float)(-HZ) - kRowMargin, (float)(-HY) - kRowMargin, (float)(-HX) - kRowMargin};
+  // largest cell corner of the brick in frame coordinates
+  const float fext0 = (float)((int)ext0 - HZ), fext1 = (float)((int)ext1 - HY), fext2 = (float)((int)ext2 - HX);
+  const float Ut[3] = {(float)((int)ext0 + 1 - HZ) + kRowMargin, (float)((int)ext1 + 1 - HY) + kRowMargin,
+                       (float)((int)ext2 + 1 - HX) + kRowMargin};
+
+  // FP32 plane test constants (scalar registers); the margin covers the FP32
+  // rounding of the test with a wide safety factor (extra candidates only
+  // cost an empty item)
+  const float cf0 = uniform_f((float)cb0), cf1 = uniform_f((float)cb1), cf2 = uniform_f((float)cb2);
+  const float hf0 = uniform_f((float)half[0]), hf1 = uniform_f((float)half[1]), hf2 = uniform_f((float)half[2]);
+  const float tmarg = uniform_f(1e-3f + 4e-6f * (fabsf(cf0) + fabsf(cf1) + fabsf(cf2) + hf0 + hf1 + hf2));
 
   int rd = r_beg, ncross = 0;
   while (have_cells) {
-    // (1a) plane test, 64 rotations per pass: rotations whose plane meets the
-    // brick's coordinate box are listed (at most 127 pending)
-    while (ncross < 64 && rd < r_end) {
-      const int r = rd + lane;
-      bool hit = false;
-      if (r < r_end) {
-        const double4 pt = ptest[r];  // one 32-byte load: normal, |normal|
-        const double dist = pt.x * cb0 + pt.y * cb1 + pt.z * cb2;
-        const double rad = fabs(pt.x) * half[0] + fabs(pt.y) * half[1] + fabs(pt.z) * half[2];
-        hit = (pt.w <= 0.0) | (fabs(dist) <= rad + 1e-9 * pt.w);
+    // (1a) plane test, 64 rotations per pass, up to 8 passes whose data are
+    // loaded at once: rotations whose plane meets the brick's coordinate box
+    // are listed (at most 95 pending)
+    if (ncross < kTake && rd < r_end) {
+      // all 8 passes are tested unconditionally (so that the 8 loads are in
+      // flight together); passes are then listed while the list has room and
+      // the rest are tested again in the next round
+      unsigned long long hm[8];
+      float4 pt[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = rd + u * 64 + lane;
+        pt[u] = ptest[min(r, r_end - 1)];
       }
-      const unsigned long long bm = __ballot(hit);
-      if (hit) cross[ncross + rank_below(bm)] = r;
-      ncross += __popcll(bm);
-      rd += 64;
+      __asm__ volatile("" ::: "memory");  // keep the 8 loads ahead of their use
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = rd + u * 64 + lane;
+        const float4 q = pt[u];
+        const float dist = q.x * cf0 + q.y * cf1 + q.z * cf2;
+        const float rad = fabsf(q.x) * hf0 + fabsf(q.y) * hf1 + fabsf(q.z) * hf2;
+        hm[u] = __ballot(r < r_end && ((q.w < 0.0f) | (fabsf(dist) <= rad + tmarg)));
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (ncross < kTake && rd < r_end) {
+          if ((hm[u] >> lane) & 1ull) cross[ncross + rank_below(hm[u])] = rd + lane;
+          ncross += __popcll(hm[u]);
+          rd += 64;
+        }
+      }
     }
     if (ncross == 0) break;
     __builtin_amdgcn_wave_barrier();
-    // (1b) items: per listed rotation and Friedel half, the exact range of
-    // slice rows whose pixels can reach the box
-    const int take = min(ncross, 64);
-    bool h0 = false, h1 = false;
-    int m0a = 0, m1a = -1, m0b = 0, m1b = -1, r = 0;
-    if (lane < take) {
-      r = cross[lane];
+    // (1b) items: lane c handles listed rotation c & 31 and Friedel half
+    // c >> 5.  Its candidate rows and columns are those of the box: the ranges
+    // of m = (x - beta) . s As and j = (x - beta) . s Bs over the box (a
+    // superset of the box's section by the plane; rows without a pixel in
+    // the box come out empty).  The item's FP32 frame is referred to the pixel
+    // nearest the projection of the tile centre.
+    const int take = min(ncross, kTake);
+    const int cidx = lane & (kTake - 1);
+    const bool ineg = lane >= kTake;
+    bool h = false;
+    ItemRec it;
+    if (cidx < take) {
+      const int r = cross[cidx];
       const PlaneRec &p = pre[r];
-      h0 = item_rows(p, 1.0, Lb, Ub, beta, g.inv_H, g.Wr, mmin, mmax, m0a, m1a);
-      h1 = item_rows(p, -1.0, Lb, Ub, beta, g.inv_H, g.Wr, mmin, mmax, m0b, m1b);
+      const double s = ineg ? -1.0 : 1.0;
+      // every field is used before the branch below, so the loads are issued together
+      double A[3], B[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        A[d] = s * p.U[d] * g.inv_H;
+        B[d] = s * p.V[d];
+      }
+      const double sU0 = s * p.U[0], sU1 = s * p.U[1], U2 = p.U[2];
+      const unsigned rtie = (unsigned)p.tie;
+      double cm = 0.0, hm = 0.0, cj = 0.0, hj = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double cbd = d == 0 ? cb0 : (d == 1 ? cb1 : cb2);
+        cm += cbd * p.As[d];
+        hm += half[d] * fabs(p.As[d]);
+        cj += cbd * p.Bs[d];
+        hj += half[d] * fabs(p.Bs[d]);
+      }
+      cm *= s;
+      cj *= s;
+      double mlo = cm - hm - 1e-6, mhi = cm + hm + 1e-6;
+      double jlo = cj - hj - 1e-6, jhi = cj + hj + 1e-6;
+      if (p.full) {
+        mlo = mmin;
+        mhi = mmax;
+        jlo = 0.0;
+        jhi = (double)(g.Wr - 1);
+      }
+      const int m0 = (int)ceil(fmax(mlo, mmin)), m1 = (int)floor(fmin(mhi, mmax));
+      const double jc0 = ceil(fmax(jlo, 0.0)), jc1 = floor(fmin(jhi, (double)(g.Wr - 1)));
+      h = m0 <= m1 && jc0 <= jc1;
+      if (h) {
+        const int mref = min(max((int)fmin(fmax(rint(cm), -1e9), 1e9), m0), m1);
+        const int jref = (int)fmin(fmax(rint(cj), 0.0), (double)(g.Wr - 1));
+        const int iref = mref + g.half_H;
+        // the canonical coordinate of the reference pixel (as in every kernel)
+        const double fy = canon_fy(g, iref), jd = (double)jref;
+        const double c0 = __fma_rn(jd, B[0], __fma_rn(sU0, fy, g.bz));
+        const double c1 = __fma_rn(jd, B[1], __fma_rn(sU1, fy, g.by));
+        const double c2 = __fma_rn(jd, B[2], s * __dmul_rn(U2, fy));
+        it.st[0] = (float)(c0 - tor[0]);
+        it.st[1] = (float)(c1 - tor[1]);
+        it.st[2] = (float)(c2 - tor[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          it.sa[d] = (float)A[d];
+          it.sv[d] = (float)B[d];
+        }
+        it.rows = (m0 + g.half_H) | ((m1 - m0 + 1) << 16);
+        it.ref = ((unsigned)iref & 0xffffu) | ((unsigned)jref << 16);
+        it.rnt = (unsigned)r | ((unsigned)ineg << 30) | (rtie << 31);
+      }
     }
     {  // keep the rest of the list
       const int rest = ncross - take;
@@ -521,86 +597,61 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
       if (lane < rest) cross[lane] = v;
       ncross = rest;
     }
-    const unsigned long long b0 = __ballot(h0), b1 = __ballot(h1);
-    const int n0 = __popcll(b0);
-    const int nitems = n0 + __popcll(b1);
-    if (h0) items[rank_below(b0)] = ItemRec{r, (short)m0a, (short)(m1a - m0a + 1)};
-    if (h1) items[n0 + rank_below(b1)] = ItemRec{r | (1 << 30), (short)m0b, (short)(m1b - m0b + 1)};
+    const unsigned long long bh = __ballot(h);
+    const int nitems = __popcll(bh);
+    if (h) items[rank_below(bh)] = it;
     __builtin_amdgcn_wave_barrier();
     if (nitems == 0) continue;
 
-    // (2) inclusive prefix of the items' row counts, held in registers:
-    // item k's end in lane k (ia) or lane k - 64 (ib)
-    const int ra = lane < nitems ? (int)items[lane].nrow : 0;
-    const int rb_ = lane + 64 < nitems ? (int)items[lane + 64].nrow : 0;
-    const int ia = wave_incl_scan(ra);
-    const int ib = wave_incl_scan(rb_) + __builtin_amdgcn_readlane(ia, 63);
-    const int total_rows = __builtin_amdgcn_readlane(ib, 63);
+    // (2) inclusive prefix of the items' row counts: item k's end in lane k
+    const int ia = wave_incl_scan(lane < nitems ? (items[lane].rows >> 16) : 0);
+    const int total_rows = __builtin_amdgcn_readlane(ia, 63);
 
     for (int rb = 0; rb < total_rows; rb += 64) {
       // item of each lane's row: the few items ending inside this pass are
       // walked with wave-uniform reads of the prefix
       const int t = rb + lane;
-      const int kf = __popcll(__ballot(lane < nitems && ia <= rb)) +
-                     __popcll(__ballot(lane + 64 < nitems && ib <= rb));
+      const int kf = __popcll(__ballot(lane < nitems && ia <= rb));
       int k = kf;
-      int excl = 0;
-      if (kf > 0) excl = kf - 1 < 64 ? __builtin_amdgcn_readlane(ia, kf - 1)
-                                     : __builtin_amdgcn_readlane(ib, kf - 65);
+      int excl = kf > 0 ? __builtin_amdgcn_readlane(ia, kf - 1) : 0;
       for (int kk = kf; kk < nitems; ++kk) {
-        const int e = kk < 64 ? __builtin_amdgcn_readlane(ia, kk) : __builtin_amdgcn_readlane(ib, kk - 64);
+        const int e = __builtin_amdgcn_readlane(ia, kk);
         if (e > rb + 63) break;
         if (e <= t) {
           k = kk + 1;
           excl = e;
         }
       }
-      // one lane per row: canonical row constants and the candidate columns
-      int len = 0;
+      // one lane per row: the candidate columns, from the item's FP32 frame
+      int len = 0, jl = 0;
       if (t < total_rows) {
-        const ItemRec it = items[k];
-        const int m = (int)it.m0 + (t - excl);
-        const int i = m + g.half_H;
-        const int ir = it.rn & 0x3fffffff, ineg = (it.rn >> 30) & 1;
-        const double s = ineg ? -1.0 : 1.0;
-        const PlaneRec &p = pre[ir];
-        const double fy = canon_fy(g, i);
-        double Cr[3], sV[3];
-        Cr[0] = __fma_rn(s * p.U[0], fy, g.bz);
-        Cr[1] = __fma_rn(s * p.U[1], fy, g.by);
-        Cr[2] = s * __dmul_rn(p.U[2], fy);
-        double jlo = 0.0, jhi = (double)(g.Wr - 1);
+        const ItemRec &ir = items[k];
+        const int i = (ir.rows & 0xffff) + (t - excl);
+        rowi[lane] = i;
+        const int iref = (int)(ir.ref & 0xffffu), jref = (int)(ir.ref >> 16);
+        const float fdm = (float)(i - iref);
+        float lo = (float)(-jref), hi = (float)(g.Wr - 1 - jref);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          sV[d] = s * p.V[d];
-          if (p.iV[d] != 0.0) {
-            const double inv = s * p.iV[d];
-            const double t1 = (Lb[d] - Cr[d]) * inv, t2 = (Ub[d] - Cr[d]) * inv;
-            jlo = fmax(jlo, fmin(t1, t2));
-            jhi = fmin(jhi, fmax(t1, t2));
-          } else if (Cr[d] < Lb[d] || Cr[d] > Ub[d]) {
-            jhi = -1.0;
+          const float C = __fmaf_rn(fdm, ir.sa[d], ir.st[d]);  // at column jref
+          const float sv = ir.sv[d];
+          if (fabsf(sv) > 1e-20f) {
+            const float iv = __builtin_amdgcn_rcpf(sv);
+            const float t1 = (Lt[d] - C) * iv, t2 = (Ut[d] - C) * iv;
+            lo = fmaxf(lo, fminf(t1, t2));
+            hi = fminf(hi, fmaxf(t1, t2));
+          } else if (C < Lt[d] || C > Ut[d]) {
+            hi = -1.0e9f;
           }
         }
-        int jl = 0, jh = -1;
-        if (jlo <= jhi) {
-          jl = (int)ceil(jlo - 1e-6);
-          jh = (int)floor(jhi + 1e-6);
+        if (lo <= hi) {
+          jl = jref + (int)ceilf(lo - 1e-3f - 1e-6f * fabsf(lo));
+          int jh = jref + (int)floorf(hi + 1e-3f + 1e-6f * fabsf(hi));
           jl = max(jl, 0);
           jh = min(jh, g.Wr - 1);
+          if (MODE == kInsert && i < g.half_H) jl = max(jl, 1);  // x = 0, y < 0 is skipped
+          len = max(0, jh - jl + 1);
         }
-        if (MODE == kInsert && i < g.half_H) jl = max(jl, 1);  // x = 0, y < 0 is skipped
-        len = max(0, jh - jl + 1);
-        RowRec rec;
-        const double jld = (double)jl;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          rec.st[d] = (float)(__fma_rn(jld, sV[d], Cr[d]) - tor[d]);
-          rec.sb[d] = (float)sV[d];
-        }
-        rec.ji = jl | (i << 16);
-        rec.rnt = (unsigned)ir | ((unsigned)ineg << 30) | ((unsigned)p.tie << 31);
-        rows[lane] = rec;
       }
       const int nch = (len + CH - 1) / CH;
       const int cincl = wave_incl_scan(nch);
@@ -613,57 +664,82 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
         const int c0 = max(cexcl, w0), c1 = min(cexcl + nch, w0 + kSegCap);
         for (int c = c0; c < c1; ++c) {
           const int q0 = (c - cexcl) * CH;
-          segs[c - w0] = lane | (min(CH, len - q0) << 6) | (q0 << 9);
+          segs[c - w0] = (unsigned)k | ((unsigned)lane << 6) | ((unsigned)(min(CH, len - q0) - 1) << 12) |
+                         ((unsigned)(jl + q0) << 13);
         }
         __builtin_amdgcn_wave_barrier();
         const int wn = min(total_ch - w0, kSegCap);
         for (int pc = 0; pc < wn; pc += 64) {
           const bool live = pc + lane < wn;
-          const int sgv = segs[live ? pc + lane : 0];
-          const int k = sgv & 63, cnt = live ? (sgv >> 6) & 7 : 0, q0 = sgv >> 9;
-          const RowRec rec = rows[k];
-          const int jl = rec.ji & 0xffff, i = rec.ji >> 16;
-          const int rr = (int)(rec.rnt & 0x3fffffffu);
-          const bool neg = (rec.rnt >> 30) & 1;
-          const int64_t base = (int64_t)rr * g.P + (int64_t)i * g.Wr + jl + q0;
-          // FP32 tile coordinates of the chunk's pixels
-          int c3[CH][3];
-          float t3[CH][3];
-          bool own[CH];
-          bool anynear = false;
+          const unsigned sgv = segs[live ? pc + lane : 0];
+          const int ki = sgv & 63, cnt = live ? (int)((sgv >> 12) & 1) + 1 : 0;
+          const int j0 = (int)(sgv >> 13);
+          const ItemRec &ir = items[ki];
+          const int i = rowi[(sgv >> 6) & 63];
+          const int iref = (int)(ir.ref & 0xffffu), jref = (int)(ir.ref >> 16);
+          const int rr = (int)(ir.rnt & 0x3fffffffu);
+          const bool neg = (ir.rnt >> 30) & 1;
+          // flat index of the chunk's first pixel (i Wr + j0 < H Wr < 2^31)
+          const int64_t base = (int64_t)((u64)(unsigned)rr * (u64)(unsigned)g.P +
+                                         (u64)(unsigned)(i * g.Wr + j0));
+          // scatter: the chunk's values are loaded first, so that the load
+          // overlaps the coordinate arithmetic (pixels q < cnt are in the row)
+          f2v vv[CH];
+          if (SCATTER) {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) vv[q] = src[q < cnt ? base + q : base];
+          }
+          // FP32 frame coordinates of the chunk's pixels: cell corner fl (the
+          // cell relative to the tile centre) and fraction t
+          const float fdm = (float)(i - iref);
+          float rowc[3], sv3[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            rowc[d] = __fmaf_rn(fdm, ir.sa[d], ir.st[d]);
+            sv3[d] = ir.sv[d];
+          }
+          float fl[CH][3], t3[CH][3];
+          float tmin = 1.0f, tmax = 0.0f;
 #pragma unroll
           for (int q = 0; q < CH; ++q) {
-            own[q] = q < cnt;
-            const float qf = (float)(q0 + q);
-            float tmin = 1.0f, tmax = 0.0f;
+            const float qf = (float)(j0 + q - jref);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-              const float l = fmaf(qf, rec.sb[d], rec.st[d]);
-              const float fl = floorf(l);
-              t3[q][d] = l - fl;
-              c3[q][d] = (int)fl;
-              tmin = fminf(tmin, t3[q][d]);
-              tmax = fmaxf(tmax, t3[q][d]);
+              const float l = __fmaf_rn(qf, sv3[d], rowc[d]);
+              fl[q][d] = floorf(l);
+              t3[q][d] = l - fl[q][d];
             }
-            anynear = anynear || (own[q] && (tmin < kNear || tmax > 1.0f - kNear));
+            tmin = fminf(tmin, fminf(fminf(t3[q][0], t3[q][1]), t3[q][2]));
+            tmax = fmaxf(tmax, fmaxf(fmaxf(t3[q][0], t3[q][1]), t3[q][2]));
           }
-          if (__ballot(anynear)) {  // rare, wave-uniform: exact FP64 cells near integers
+          bool own[CH];
 #pragma unroll
-            for (int q = 0; q < CH; ++q) {
-              const float tmin = fminf(fminf(t3[q][0], t3[q][1]), t3[q][2]);
-              const float tmax = fmaxf(fmaxf(t3[q][0], t3[q][1]), t3[q][2]);
-              if (own[q] && (tmin < kNear || tmax > 1.0f - kNear))
-                own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (rec.rnt >> 31) & 1, i, jl + q0 + q,
-                                                     lo0, lo1, lo2, c3[q], t3[q]);
+          for (int q = 0; q < CH; ++q) own[q] = q < cnt;
+          const bool anynear = cnt > 0 && (tmin < kNear || tmax > 1.0f - kNear);
+          if (__ballot(anynear)) {  // rare, wave-uniform: exact FP64 cells near integers
+            if (anynear) {
+#pragma unroll
+              for (int q = 0; q < CH; ++q) {
+                if (!own[q]) continue;
+                int cell[3];
+                own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (ir.rnt >> 31) & 1, i, j0 + q, lo0,
+                                                     lo1, lo2, cell, t3[q]);
+                fl[q][0] = (float)(cell[0] - HZ);
+                fl[q][1] = (float)(cell[1] - HY);
+                fl[q][2] = (float)(cell[2] - HX);
+              }
             }
           }
+          // owned: the cell lies in the brick; e = its tile index
           int e[CH];
 #pragma unroll
           for (int q = 0; q < CH; ++q) {
-            own[q] = own[q] && (unsigned)c3[q][0] <= ext0 && (unsigned)c3[q][1] <= ext1 &&
-                     (unsigned)c3[q][2] <= ext2;
-            if (MASK) own[q] = own[q] && in_disc(g, i, jl + q0 + q);
-            e[q] = own[q] ? (c3[q][0] * SY + c3[q][1]) * SX + c3[q][2] : 0;
+            own[q] = own[q] & (__builtin_amdgcn_fmed3f(fl[q][0], (float)-HZ, fext0) == fl[q][0]) &
+                     (__builtin_amdgcn_fmed3f(fl[q][1], (float)-HY, fext1) == fl[q][1]) &
+                     (__builtin_amdgcn_fmed3f(fl[q][2], (float)-HX, fext2) == fl[q][2]);
+            if (MASK) own[q] = own[q] && in_disc(g, i, j0 + q);
+            const float ef = __fmaf_rn(fl[q][0], (float)(SY * SX), __fmaf_rn(fl[q][1], (float)SX, fl[q][2]));
+            e[q] = own[q] ? (int)ef + (HZ * SY * SX + HY * SX + HX) : 0;
           }
           if (MODE == kGather) {
             const f2v *tv = reinterpret_cast<const f2v *>(s_tile);
@@ -688,9 +764,6 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
               if (own[q]) out[base + q] = val;
             }
           } else {
-            f2v vv[CH];
-#pragma unroll
-            for (int q = 0; q < CH; ++q) vv[q] = src[own[q] ? base + q : base];
 #pragma unroll
             for (int q = 0; q < CH; ++q) {
               if (!own[q]) continue;
@@ -1287,11 +1360,11 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
     if (pe.n_units > 0) {
       if (g.use_mask)
         hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, true>), dim3((unsigned)pe.n_units), dim3(C::T),
-                           0, s, src, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units, (int)n_rot,
+                           0, s, src, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot,
                            out, wout, plane, wplane, segbuf, halo, sa, accumulate);
       else
         hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, false>), dim3((unsigned)pe.n_units), dim3(C::T),
-                           0, s, src, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units, (int)n_rot,
+                           0, s, src, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot,
                            out, wout, plane, wplane, segbuf, halo, sa, accumulate);
       if (int st = check_launch("brick_kernel")) return st;
     }
@@ -1341,12 +1414,12 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
   constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
   if (g.use_mask)
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)pe.n_units), dim3(T),
-                       0, s, vol, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units, (int)n_rot, out,
+                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot, out,
                        (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
                        (u64 *)nullptr, sa, 0);
   else
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)pe.n_units),
-                       dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const double4 *)ptest, (const Unit *)pe.units,
+                       dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
                        (u64 *)nullptr, (u64 *)nullptr, sa, 0);
   return check_launch("brick_kernel");
