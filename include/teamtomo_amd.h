@@ -42,7 +42,7 @@ extern "C" {
 #define TT_FLAG_PIXEL_ATOMIC 4  /* force the pixel-driven global-atomic scatter         */
 
 /* Extraction gather kernels (process-wide choice, see tt_set_gather_path) */
-#define TT_GATHER_AUTO 0  /* the library's choice (currently the pixel gather)  */
+#define TT_GATHER_AUTO 0  /* brick gather, pixel gather beyond the brick limits */
 #define TT_GATHER_PIXEL 1 /* one lane per pixel, corners read through L2/L1      */
 #define TT_GATHER_BRICK 2 /* volume bricks staged in LDS, serving all slices     */
 

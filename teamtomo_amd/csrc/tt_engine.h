@@ -5,14 +5,19 @@
 
 namespace tt {
 
+// Geometries the engine handles: fewer than 2^30 rotations, slice height
+// below 32768 and rfft width below 65536 (the packed chunk list and item
+// records hold the row and column in 16 bits).
+bool engine_supports(const SliceGeom &g, int64_t n_rot);
+
 // slices = extraction of `vol` at the rotations; writes every pixel of the
 // n_rot slices (a zero pass for pixels outside the volume or the disc, then
 // the brick gather for the rest).
 int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
                    hipStream_t s);
 
-// gV (+)= A^T gS, A the extraction of geometry g.  LDS float accumulation per
-// brick, bricks flushed with global float atomics (order not fixed).
+// gV (+)= A^T gS, A the extraction of geometry g.  64-bit fixed-point LDS
+// accumulation per brick: bitwise reproducible.
 int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot, f2v *gV,
                    int accumulate, hipStream_t s);
 

@@ -1285,7 +1285,7 @@ BrickGrid grid_of(const SliceGeom &g, int BZ, int BY, int BX) {
 int check_engine(const SliceGeom &g, int64_t n_rot) {
   if (n_rot >= ((int64_t)1 << 30)) return fail_invalid("too many rotations for one call (< 2^30)");
   if (g.H >= 32768) return fail_invalid("slice height must be < 32768");
-  if (g.Wr >= 65536) return fail_invalid("slice width must be < 131070");
+  if (g.Wr >= 65536) return fail_invalid("slice rfft width must be < 65536");
   return TT_OK;
 }
 
@@ -1388,6 +1388,10 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
 }
 
 }  // namespace
+
+bool engine_supports(const SliceGeom &g, int64_t n_rot) {
+  return n_rot < ((int64_t)1 << 30) && g.H < 32768 && g.Wr < 65536;
+}
 
 int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
                    hipStream_t s) {

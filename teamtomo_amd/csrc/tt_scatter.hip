@@ -134,8 +134,10 @@ int scatter_extract_adjoint(const SliceGeom &g, const f2v *gS, const float *rot,
                             f2v *gV, int flags, hipStream_t s) {
   const bool det = flags & TT_FLAG_DETERMINISTIC, overwrite = flags & TT_FLAG_OVERWRITE;
   // The brick engine accumulates in fixed point: deterministic as it stands.
-  if (!(flags & TT_FLAG_PIXEL_ATOMIC)) return engine_adjoint(g, gS, rot, n_rot, gV, !overwrite, s);
-  if (det) return fail_invalid("the pixel-atomic scatter is not deterministic");
+  if (!(flags & TT_FLAG_PIXEL_ATOMIC) && engine_supports(g, n_rot))
+    return engine_adjoint(g, gS, rot, n_rot, gV, !overwrite, s);
+  if (det) return fail_invalid("the pixel-atomic scatter is not deterministic (and the geometry is beyond "
+                               "the brick engine's limits, or TT_FLAG_PIXEL_ATOMIC was given)");
   int st;
   if (overwrite && (st = zero_fill<float>((float *)gV, 2 * (int64_t)g.vD * g.vH * g.vW, s))) return st;
   Plan pl = make_plan(g);
@@ -153,9 +155,10 @@ template <typename T>
 int scatter_insert(const SliceGeom &g, const typename Cx<T>::type *slices, const float *rot,
                    int64_t n_rot, typename Cx<T>::type *vol, T *w, int flags, hipStream_t s) {
   const bool det = flags & TT_FLAG_DETERMINISTIC, overwrite = flags & TT_FLAG_OVERWRITE;
-  if (sizeof(T) == sizeof(float) && !(flags & TT_FLAG_PIXEL_ATOMIC))
+  if (sizeof(T) == sizeof(float) && !(flags & TT_FLAG_PIXEL_ATOMIC) && engine_supports(g, n_rot))
     return engine_insert(g, (const f2v *)slices, rot, n_rot, (f2v *)vol, (float *)w, !overwrite, s);
-  if (det) return fail_invalid("deterministic insertion is available for complex64 only");
+  if (det) return fail_invalid("deterministic insertion is available for complex64 within the brick "
+                               "engine's limits only");
   int st;
   const int64_t nv = (int64_t)g.vD * g.vH * g.vW;
   if (overwrite && ((st = zero_fill<T>((T *)vol, 2 * nv, s)) || (st = zero_fill<T>(w, nv, s))))

@@ -365,7 +365,9 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img
   if (!volume || !slices) return fail_invalid("volume / slices pointer is NULL");
   SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, x_tolerance);
   hipStream_t s = (hipStream_t)stream;
-  if (g_gather_path != TT_GATHER_PIXEL)
+  // the brick engine, unless the pixel gather is asked for or (automatic
+  // choice) the geometry is beyond the engine's limits
+  if (g_gather_path == TT_GATHER_BRICK || (g_gather_path == TT_GATHER_AUTO && engine_supports(g, n_rot)))
     return engine_extract(g, (const f2v *)volume, rot_zyx, n_rot, (f2v *)slices, s);
   int64_t bpr = (g.P + kBlock - 1) / kBlock;
   int64_t rots_per_launch = std::max<int64_t>(1, kMaxBlocksPerLaunch / bpr);
