@@ -485,6 +485,13 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
   const float cf0 = uniform_f((float)cb0), cf1 = uniform_f((float)cb1), cf2 = uniform_f((float)cb2);
   const float hf0 = uniform_f((float)half[0]), hf1 = uniform_f((float)half[1]), hf2 = uniform_f((float)half[2]);
   const float tmarg = uniform_f(1e-3f + 4e-6f * (fabsf(cf0) + fabsf(cf1) + fabsf(cf2) + hf0 + hf1 + hf2));
+  // the box relative to beta (FP32), for the ranges of the box's section
+  const float flo[3] = {uniform_f((float)(Lb[0] - beta[0])), uniform_f((float)(Lb[1] - beta[1])),
+                        uniform_f((float)(Lb[2] - beta[2]))};
+  const float fhi[3] = {uniform_f((float)(Ub[0] - beta[0])), uniform_f((float)(Ub[1] - beta[1])),
+                        uniform_f((float)(Ub[2] - beta[2]))};
+  const float fbig = uniform_f(fmaxf(fabsf(flo[0]), fabsf(fhi[0])) + fmaxf(fabsf(flo[1]), fabsf(fhi[1])) +
+                               fmaxf(fabsf(flo[2]), fabsf(fhi[2])) + 1.0f);
 
   int rd = r_beg, ncross = 0;
   while (have_cells) {
@@ -565,9 +572,49 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
         jlo = 0.0;
         jhi = (double)(g.Wr - 1);
       }
-      const int m0 = (int)ceil(fmax(mlo, mmin)), m1 = (int)floor(fmin(mhi, mmax));
+      int m0 = (int)ceil(fmax(mlo, mmin)), m1 = (int)floor(fmin(mhi, mmax));
       const double jc0 = ceil(fmax(jlo, 0.0)), jc1 = floor(fmin(jhi, (double)(g.Wr - 1)));
       h = m0 <= m1 && jc0 <= jc1;
+      if (h && !p.full) {
+        // tighter: the ranges of m and j over the box's section by the plane,
+        // min a.y subject to y in the box (relative to beta) and n.y = 0, by
+        // its Lagrange dual max_lam sum_d min((a_d - lam n_d) l_d, (a_d - lam n_d) u_d),
+        // whose maximum lies at a breakpoint lam = a_k / n_k (FP32, widened
+        // well beyond its rounding)
+        const float nz = (float)(A[1] * B[2] - A[2] * B[1]), ny = (float)(A[2] * B[0] - A[0] * B[2]),
+                    nx = (float)(A[0] * B[1] - A[1] * B[0]);
+        const float nmax = fmaxf(fmaxf(fabsf(nz), fabsf(ny)), fabsf(nx));
+        const float fa[3] = {(float)(s * p.As[0]), (float)(s * p.As[1]), (float)(s * p.As[2])};
+        const float fb[3] = {(float)(s * p.Bs[0]), (float)(s * p.Bs[1]), (float)(s * p.Bs[2])};
+        const float nn[3] = {nz, ny, nx};
+        float lo_m = -3.0e38f, lo_j = -3.0e38f, hi_m = -3.0e38f, hi_j = -3.0e38f;  // hi: of -a
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (!(fabsf(nn[k]) > 1e-6f * nmax)) continue;
+          const float rk = 1.0f / nn[k];
+          const float lm = fa[k] * rk, lj = fb[k] * rk;
+          float gm = 0.0f, gj = 0.0f, gm2 = 0.0f, gj2 = 0.0f;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const float cm = fa[d] - lm * nn[d], cj = fb[d] - lj * nn[d];
+            gm += fminf(cm * flo[d], cm * fhi[d]);
+            gm2 += fminf(-cm * flo[d], -cm * fhi[d]);
+            gj += fminf(cj * flo[d], cj * fhi[d]);
+            gj2 += fminf(-cj * flo[d], -cj * fhi[d]);
+          }
+          lo_m = fmaxf(lo_m, gm);
+          hi_m = fmaxf(hi_m, gm2);
+          lo_j = fmaxf(lo_j, gj);
+          hi_j = fmaxf(hi_j, gj2);
+        }
+        if (lo_m > -1.0e38f) {
+          const float mg = 1e-2f + 1e-5f * fbig * (fabsf(fa[0]) + fabsf(fa[1]) + fabsf(fa[2]));
+          const float jg = 1e-2f + 1e-5f * fbig * (fabsf(fb[0]) + fabsf(fb[1]) + fabsf(fb[2]));
+          m0 = max(m0, (int)ceilf(fmaxf(lo_m - mg, -1.0e9f)));
+          m1 = min(m1, (int)floorf(fminf(-hi_m + mg, 1.0e9f)));
+          h = m0 <= m1 && ceilf(fmaxf(lo_j - jg, 0.0f)) <= floorf(fminf(-hi_j + jg, (float)(g.Wr - 1)));
+        }
+      }
       if (h) {
         const int mref = min(max((int)fmin(fmax(rint(cm), -1e9), 1e9), m0), m1);
         const int jref = (int)fmin(fmax(rint(cj), 0.0), (double)(g.Wr - 1));
