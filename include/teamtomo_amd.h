@@ -157,6 +157,48 @@ int tt_splat3d_f64(const double *values, const float *coords, int64_t n, int cha
                    int is_complex, int d, int h, int w, int mode,
                    double *image, void *weights, int weights_f64, void *stream);
 
+/*
+ * Batched extraction or insertion over many volumes of their own box size and
+ * rotation count.  No single reference function corresponds: it runs the
+ * work of a loop of extract_central_slices_rfft_3d
+ * (.../slice_extraction/_extract_central_slices_rfft_3d.py:84-222) or
+ * insert_central_slices_rfft_3d (.../slice_insertion/_insert_central_slices_rfft_3d.py:9-106)
+ * calls, one per volume, as one persistent work-queue launch of the brick
+ * kernel per tile configuration (plus batched helper kernels).
+ *
+ * tt_batch_create plans the batch and allocates everything a run needs
+ * (hipMalloc), optionally timing the candidate tile configurations per box
+ * size (autotune != 0: runs the batch, synchronises the stream).
+ * tt_batch_run only launches kernels on `stream`: no allocation, no host
+ * planning and no memset, so it can be captured in a graph and replayed.  The
+ * item buffers are bound at creation; a run reads the current contents.
+ * Outputs are written, not accumulated.  Insertion: deterministic fixed-point
+ * sums, as the one-volume entry point.
+ */
+#define TT_BATCH_EXTRACT 0
+#define TT_BATCH_INSERT 1
+
+typedef struct tt_batch_item {
+  const void *input;          /* extract: volume (vol_d, vol_h, vol_wr) c64;
+                                 insert: slices (n_rot, img_h, img_w/2+1) c64   */
+  int vol_d, vol_h, vol_wr;   /* rfft volume shape                              */
+  int img_d, img_h, img_w;    /* image shape (insertion: the full volume shape) */
+  const float *rot_zyx;       /* (n_rot, 3, 3) prepared as for the one-call API */
+  int64_t n_rot;
+  double fftfreq_max;         /* < 0 means None                                 */
+  double x_tolerance;         /* extraction only                                */
+  void *output;               /* extract: slices (n_rot, img_h, img_w/2+1) c64;
+                                 insert: volume (vol_d, vol_h, vol_wr) c64       */
+  float *weights;             /* insert: (vol_d, vol_h, vol_wr) float32          */
+} tt_batch_item;
+
+int tt_batch_create(int kind, const tt_batch_item *items, int n_items, int autotune, void *stream,
+                    void **handle);
+int tt_batch_run(void *handle, void *stream);
+/* Tile (brick of cells) the batch uses for one item, as picked at creation. */
+int tt_batch_tile(void *handle, int item, int *bz, int *by, int *bx);
+int tt_batch_destroy(void *handle);
+
 #ifdef __cplusplus
 }
 #endif

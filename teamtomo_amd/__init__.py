@@ -23,6 +23,7 @@ from .fourier_slice import (
     project_3d_to_2d,
     project_3d_to_2d_multichannel,
 )
+from .batch import SliceBatch
 from .image_interpolation import insert_into_image_3d, sample_image_3d
 
 __all__ = [
@@ -41,6 +42,7 @@ __all__ = [
     "set_pixel_atomic",
     "set_gather_path",
     "install_dropin",
+    "SliceBatch",
 ]
 
 

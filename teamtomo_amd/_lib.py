@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -29,6 +29,9 @@ TT_FLAG_PIXEL_ATOMIC = 4
 TT_GATHER_AUTO = 0
 TT_GATHER_PIXEL = 1
 TT_GATHER_BRICK = 2
+
+TT_BATCH_EXTRACT = 0
+TT_BATCH_INSERT = 1
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -48,7 +51,27 @@ _SIGNATURES = {
     "tt_sample3d_f64": [_p, _i, _i, _i, _i, _i, _p, _l, _i, _p, _p],
     "tt_splat3d_f32": [_p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p],
     "tt_splat3d_f64": [_p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p],
+    "tt_batch_create": [_i, _p, _i, _i, _p, ctypes.POINTER(ctypes.c_void_p)],
+    "tt_batch_run": [_p, _p],
+    "tt_batch_tile": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "tt_batch_destroy": [_p],
 }
+
+
+class BatchItem(ctypes.Structure):
+    """``tt_batch_item`` of include/teamtomo_amd.h."""
+
+    _fields_ = [
+        ("input", _p),
+        ("vol_d", _i), ("vol_h", _i), ("vol_wr", _i),
+        ("img_d", _i), ("img_h", _i), ("img_w", _i),
+        ("rot_zyx", _p),
+        ("n_rot", _l),
+        ("fftfreq_max", _d),
+        ("x_tolerance", _d),
+        ("output", _p),
+        ("weights", _p),
+    ]
 
 
 def library_path() -> str:

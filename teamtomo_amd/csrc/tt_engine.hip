@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -73,11 +74,8 @@ struct PlaneRec {
 // degenerate plane (every brick is a candidate).
 typedef float4 PlaneTest;
 
-__global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float *__restrict__ rot,
-                                                        int n_rot, PlaneRec *__restrict__ out,
-                                                        PlaneTest *__restrict__ test) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= n_rot) return;
+__device__ __forceinline__ void plane_rec_one(const SliceGeom &g, const float *__restrict__ rot, int r,
+                                              PlaneRec *__restrict__ out, PlaneTest *__restrict__ test) {
   const Rot rr = load_rot(rot + (int64_t)r * 9, g);
   PlaneRec p;
   double A[3], B[3];
@@ -112,6 +110,13 @@ __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float
     t = make_float4((float)(n0 / nn), (float)(n1 / nn), (float)(n2 / nn), 1.0f);
   }
   test[r] = t;
+}
+
+__global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float *__restrict__ rot,
+                                                        int n_rot, PlaneRec *__restrict__ out,
+                                                        PlaneTest *__restrict__ test) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r < n_rot) plane_rec_one(g, rot, r, out, test);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,6 +241,7 @@ struct ScaleArgs {
 // words apart (separate cache lines, so the atomics do not queue on one
 // address); the maximum is the largest slot.
 constexpr int kMaxSlots = 32, kSlotStride = 32;
+constexpr int kSetWords = kMaxSlots * kSlotStride;  // words of one set of slots
 
 __device__ __forceinline__ void scale_of(const ScaleArgs &sa, float &scale, float &limit) {
   unsigned mb = 0u;
@@ -381,13 +387,14 @@ __device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, cons
 // ---------------------------------------------------------------------------
 // The brick kernel (gather, adjoint or insertion)
 // ---------------------------------------------------------------------------
+// One unit (a brick and a range of rotations) of the gather, the adjoint or
+// the insertion: shared by the one-call kernel and the batched persistent one.
 template <int BZ, int BY, int BX, int MODE, bool MASK>
-__global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
-    const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
-    const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
-    f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
-    float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
-    int accumulate) {
+__device__ __forceinline__ void brick_body(
+    const f2v *__restrict__ src, const SliceGeom &g, const BrickGrid &bg, const PlaneRec *__restrict__ pre,
+    const PlaneTest *__restrict__ ptest, const Unit &un, int n_rot, f2v *__restrict__ out,
+    float *__restrict__ wout, f2v *__restrict__ plane, float *__restrict__ wplane,
+    u64 *__restrict__ segbuf, u64 *__restrict__ halo, const ScaleArgs &sa, int accumulate) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
@@ -408,7 +415,6 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
   u64 *s_tile = sm.tile;
 
   if (SCATTER && sa.guard && *sa.flag == 0u) return;  // exact redo not needed
-  const Unit un = units[blockIdx.x];
   const int brick = un.brick;
   const int seg = un.seg >> 16, nseg_raw = un.seg & 0xffff;
   const int nseg = max(nseg_raw, 1);  // an empty brick (no slice reaches it) only writes zeros
@@ -857,19 +863,29 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
                                halo, tid, C::T);
 }
 
+template <int BZ, int BY, int BX, int MODE, bool MASK>
+__global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
+    const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
+    const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
+    f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
+    float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
+    int accumulate) {
+  const Unit un = units[blockIdx.x];
+  brick_body<BZ, BY, BX, MODE, MASK>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
+                                     halo, sa, accumulate);
+}
+
 // Sums the rotation segments of a split brick (integer sums: the result does
 // not depend on the order) and flushes the tile like an unsplit brick; one
 // thread per tile voxel, blockIdx.x over the split bricks.
 template <int BZ, int BY, int BX, int MODE>
-__global__ __launch_bounds__(256) void split_combine_kernel(
-    SliceGeom g, BrickGrid bg, const SplitRec *__restrict__ splits, const u64 *__restrict__ segbuf,
-    ScaleArgs sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo) {
+__device__ __forceinline__ void split_body(
+    const SliceGeom &g, const BrickGrid &bg, const SplitRec &sp, const u64 *__restrict__ segbuf,
+    const ScaleArgs &sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
+    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo, int v) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NT = C::NT, NA = C::NACC;
   if (sa.guard && *sa.flag == 0u) return;
-  const SplitRec sp = splits[blockIdx.x];
-  const int v = blockIdx.y * 256 + threadIdx.x;
   if (v >= NT) return;
   u64 acc[NA];
 #pragma unroll
@@ -886,21 +902,28 @@ __global__ __launch_bounds__(256) void split_combine_kernel(
                                 wplane, halo);
 }
 
+template <int BZ, int BY, int BX, int MODE>
+__global__ __launch_bounds__(256) void split_combine_kernel(
+    SliceGeom g, BrickGrid bg, const SplitRec *__restrict__ splits, const u64 *__restrict__ segbuf,
+    ScaleArgs sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
+    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo) {
+  const SplitRec sp = splits[blockIdx.x];
+  split_body<BZ, BY, BX, MODE>(g, bg, sp, segbuf, sa, accumulate, out, wout, plane, wplane, halo,
+                               (int)(blockIdx.y * 256 + threadIdx.x));
+}
+
 // Adds the halo partial sums of the lower neighbours to the voxels of a
 // brick's lower shell (l = 0 on an axis where the brick has a neighbour
 // below): a voxel on a face, edge or corner receives 1, 3 or 7 halos.
 template <int BZ, int BY, int BX, int MODE>
-__global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGrid bg,
-                                                           const u64 *__restrict__ halo, ScaleArgs sa,
-                                                           f2v *__restrict__ out,
-                                                           float *__restrict__ wout,
-                                                           f2v *__restrict__ plane,
-                                                           float *__restrict__ wplane) {
+__device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &bg,
+                                          const u64 *__restrict__ halo, const ScaleArgs &sa,
+                                          f2v *__restrict__ out, float *__restrict__ wout,
+                                          f2v *__restrict__ plane, float *__restrict__ wplane, int brick,
+                                          int u) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int SX = C::SX, SY = C::SY, NA = C::NACC, NSH = C::NSH;
   if (sa.guard && *sa.flag == 0u) return;
-  const int brick = blockIdx.x;
-  const int u = blockIdx.y * 256 + threadIdx.x;
   if (u >= NSH) return;
   const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
   const int B[3] = {BZ, BY, BX};
@@ -972,19 +995,29 @@ __global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGri
   }
 }
 
+template <int BZ, int BY, int BX, int MODE>
+__global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGrid bg,
+                                                           const u64 *__restrict__ halo, ScaleArgs sa,
+                                                           f2v *__restrict__ out,
+                                                           float *__restrict__ wout,
+                                                           f2v *__restrict__ plane,
+                                                           float *__restrict__ wplane) {
+  halo_body<BZ, BY, BX, MODE>(g, bg, halo, sa, out, wout, plane, wplane, (int)blockIdx.x,
+                              (int)(blockIdx.y * 256 + threadIdx.x));
+}
+
 // max(|re|, |im|) as float bits into *res, over chunks of 8192 consecutive
 // values (64 KiB) taken every `stride_chunks` chunks: 1 reads everything, 64
 // a 1/64 sample of contiguous blocks.  Four loads in flight per lane; one
 // atomic per block, into slot blockIdx.x % kMaxSlots.
-__global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restrict__ x, int64_t n,
-                                                            int64_t stride_chunks,
-                                                            const unsigned *guard,
-                                                            unsigned *__restrict__ res) {
+__device__ __forceinline__ void maxabs_body(const f2v *__restrict__ x, int64_t n, int64_t stride_chunks,
+                                            const unsigned *guard, unsigned *__restrict__ res, int block,
+                                            int nblocks) {
   __shared__ float s_m[4];
   if (guard && *guard == 0u) return;
-  const int64_t lanes = (int64_t)gridDim.x * 256;
+  const int64_t lanes = (int64_t)nblocks * 256;
   float m = 0.0f;
-  for (int64_t t0 = (int64_t)blockIdx.x * 256 + threadIdx.x;; t0 += 4 * lanes) {
+  for (int64_t t0 = (int64_t)block * 256 + threadIdx.x;; t0 += 4 * lanes) {
     f2v v[4];
     bool any = false;
 #pragma unroll
@@ -1002,8 +1035,15 @@ __global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restric
   if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0)
-    atomicMax(res + (blockIdx.x % kMaxSlots) * kSlotStride,
+    atomicMax(res + (block % kMaxSlots) * kSlotStride,
               __float_as_uint(fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]))));
+}
+
+__global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restrict__ x, int64_t n,
+                                                            int64_t stride_chunks,
+                                                            const unsigned *guard,
+                                                            unsigned *__restrict__ res) {
+  maxabs_body(x, n, stride_chunks, guard, res, (int)blockIdx.x, (int)gridDim.x);
 }
 
 __global__ void set_word_kernel(unsigned *p, unsigned v) {
@@ -1036,12 +1076,12 @@ __device__ __forceinline__ void lin_interval(double A, double B, double lo, doub
   }
 }
 
-__global__ __launch_bounds__(256) void gather_zero_kernel(SliceGeom g, const float *__restrict__ rot,
-                                                          int64_t n_rows, f2v *__restrict__ out) {
+__device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
+                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block) {
   constexpr int CH = 4;
   __shared__ ZeroRow s_zr[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t t = ((int64_t)blockIdx.x * 4 + wave) * 64 + lane;
+  const int64_t t = (block * 4 + wave) * 64 + lane;
   int total = 0;
   if (t < n_rows) {
     const int64_t r = t / g.H;
@@ -1118,14 +1158,17 @@ __global__ __launch_bounds__(256) void gather_zero_kernel(SliceGeom g, const flo
   }
 }
 
+__global__ __launch_bounds__(256) void gather_zero_kernel(SliceGeom g, const float *__restrict__ rot,
+                                                          int64_t n_rows, f2v *__restrict__ out) {
+  gather_zero_body(g, rot, n_rows, out, (int64_t)blockIdx.x);
+}
+
 // x=0-plane Friedel mirror of the insertion, from the plane of direct splats
 // (slice_insertion/_insert_central_slices_rfft_3d.py:350-404):
 //   V[z,y,0] = P[z,y] + conj(P[mz,my])  unless (mz,my) is DC.
-__global__ __launch_bounds__(256) void mirror_kernel(const f2v *__restrict__ plane,
-                                                     const float *__restrict__ wplane, int D, int H,
-                                                     int Wr, f2v *__restrict__ out,
-                                                     float *__restrict__ wout, int accumulate) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void mirror_one(const f2v *__restrict__ plane, const float *__restrict__ wplane,
+                                           int D, int H, int Wr, f2v *__restrict__ out,
+                                           float *__restrict__ wout, int accumulate, int idx) {
   if (idx >= D * H) return;
   const int z = idx / H, y = idx - z * H;
   const int mz = (D - z) % D, my = (H - y) % H;
@@ -1144,6 +1187,13 @@ __global__ __launch_bounds__(256) void mirror_kernel(const f2v *__restrict__ pla
   }
   out[e] = val;
   wout[e] = w;
+}
+
+__global__ __launch_bounds__(256) void mirror_kernel(const f2v *__restrict__ plane,
+                                                     const float *__restrict__ wplane, int D, int H,
+                                                     int Wr, f2v *__restrict__ out,
+                                                     float *__restrict__ wout, int accumulate) {
+  mirror_one(plane, wplane, D, H, Wr, out, wout, accumulate, (int)(blockIdx.x * 256 + threadIdx.x));
 }
 
 template <typename T>
@@ -1177,6 +1227,10 @@ struct PlanEntry {
   SplitRec *splits = nullptr;
   int n_splits = 0;
   int n_slots = 0;  // rotation-segment partial slots of the split bricks
+  // host copies (the batched plans merge the units of many volumes)
+  std::shared_ptr<std::vector<Unit>> h_units;
+  std::shared_ptr<std::vector<double>> h_est;
+  std::shared_ptr<std::vector<SplitRec>> h_splits;
 };
 
 std::mutex g_plan_mutex;
@@ -1191,7 +1245,10 @@ constexpr int kTargetSlots = 256 * 2 * 4;  // CUs x resident blocks x units per 
 // dropped (gather) or kept as empty units that write zeros (scatter).
 int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n_rot, PlanEntry &pe) {
   int dev = 0;
-  (void)hipGetDevice(&dev);
+  if (hipError_t e0 = hipGetDevice(&dev)) {
+    set_error(std::string("brick plan: hipGetDevice: ") + hipGetErrorString(e0));
+    return TT_ERR_HIP;
+  }
   int64_t nq = 64;
   while (nq < n_rot) nq <<= 1;
   float fm = g.use_mask ? g.fmax : -1.0f;
@@ -1203,6 +1260,10 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
   auto it = g_plans.find(key);
   if (it != g_plans.end()) {
     pe = it->second;
+    if (hipError_t e1 = hipGetLastError()) {
+      set_error(std::string("brick plan (cached): ") + hipGetErrorString(e1));
+      return TT_ERR_HIP;
+    }
     return TT_OK;
   }
   // bricks tile the cells 0..N-2 of each axis
@@ -1279,6 +1340,10 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
   e.n_units = (int)host.size();
   e.n_splits = (int)splits.size();
   e.n_slots = slots;
+  e.h_units = std::make_shared<std::vector<Unit>>(host);
+  e.h_est = std::make_shared<std::vector<double>>(us.size());
+  for (size_t k = 0; k < us.size(); ++k) (*e.h_est)[k] = us[k].est;
+  e.h_splits = std::make_shared<std::vector<SplitRec>>(splits);
   hipError_t err = hipSuccess;
   if (e.n_units > 0) {
     err = hipMalloc((void **)&e.units, host.size() * sizeof(Unit));
@@ -1290,8 +1355,10 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
     if (err == hipSuccess)
       err = hipMemcpy(e.splits, splits.data(), splits.size() * sizeof(SplitRec), hipMemcpyHostToDevice);
   }
+  if (err == hipSuccess) err = hipGetLastError();
   if (err != hipSuccess) {
-    set_error(std::string("brick plan upload: ") + hipGetErrorString(err));
+    set_error(std::string("brick plan upload (") + std::to_string(host.size()) + " units, " +
+              std::to_string(splits.size()) + " splits): " + hipGetErrorString(err));
     return TT_ERR_HIP;
   }
   g_plans[key] = e;
@@ -1348,7 +1415,6 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   const size_t np = (size_t)g.vD * g.vH;
   // scratch: words (max slots sampled, max slots exact, flag) | plane | wplane |
   // PlaneRec | PlaneTest | rotation-segment slots | halo shells
-  constexpr int kSetWords = kMaxSlots * kSlotStride;
   size_t off = ((2 * kSetWords + 1) * sizeof(unsigned) + 255) & ~(size_t)255;
   const size_t off_plane = off;
   if (MODE == kInsert) off += np * sizeof(f2v);
@@ -1434,6 +1500,457 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   return TT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched launches: many volumes (box sizes and rotation counts of their own)
+// in one persistent work-queue launch of the brick kernel per tile
+// configuration, plus batched helper kernels.  Everything a run needs is
+// planned and allocated when the batch is created, so a run only launches
+// kernels (and can be captured in a graph).
+// ---------------------------------------------------------------------------
+struct BatchItemDev {
+  SliceGeom g;
+  BrickGrid bg;
+  const f2v *src;  // volume (extraction) or slices (insertion)
+  const float *rot;
+  f2v *out;        // slices (extraction) or volume (insertion)
+  float *wout;     // weights (insertion)
+  PlaneRec *pre;
+  PlaneTest *ptest;
+  unsigned *words;  // scale slots (sampled, exact) and the violation flag
+  f2v *plane;
+  float *wplane;
+  u64 *segbuf;
+  u64 *halo;
+  int n_rot;
+  int n_bricks;
+  int64_t n_src;  // values in src (insertion)
+};
+
+struct BatchUnit {
+  int item, brick, seg, slot;
+};
+
+__device__ __forceinline__ ScaleArgs batch_scale(const BatchItemDev &it, int pass) {
+  ScaleArgs sa;
+  sa.maxbits = it.words + (pass ? kSetWords : 0);
+  sa.flag = it.words + 2 * kSetWords;
+  sa.headroom = pass ? 0 : 8;
+  sa.guard = pass;
+  return sa;
+}
+
+// Persistent: every block takes units from the queue until it is empty.
+template <int BZ, int BY, int BX, int MODE>
+__global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_batch_kernel(
+    const BatchItemDev *__restrict__ items, const BatchUnit *__restrict__ units, int n_units,
+    unsigned *__restrict__ queue, int pass) {
+  __shared__ int s_next;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();  // s_next is rewritten by the next iteration
+    if (u >= n_units) break;
+    const BatchUnit bu = units[u];
+    const int ix = __builtin_amdgcn_readfirstlane(bu.item);
+    const BatchItemDev &it = items[ix];
+    const Unit un{__builtin_amdgcn_readfirstlane(bu.brick), __builtin_amdgcn_readfirstlane(bu.seg),
+                  __builtin_amdgcn_readfirstlane(bu.slot)};
+    const ScaleArgs sa = batch_scale(it, pass);
+    brick_body<BZ, BY, BX, MODE, true>(it.src, it.g, it.bg, it.pre, it.ptest, un, it.n_rot, it.out, it.wout,
+                                       it.plane, it.wplane, it.segbuf, it.halo, sa, 0);
+    __syncthreads();  // the tile is reused by the next unit
+  }
+}
+
+// blocks: (item, block within the item[, blocks of the item])
+__global__ __launch_bounds__(256) void gather_zero_batch_kernel(const BatchItemDev *__restrict__ items,
+                                                                const int2 *__restrict__ blocks) {
+  const int2 b = blocks[blockIdx.x];
+  const BatchItemDev &it = items[b.x];
+  gather_zero_body(it.g, it.rot, (int64_t)it.n_rot * it.g.H, it.out, (int64_t)b.y);
+}
+
+__global__ __launch_bounds__(256) void plane_rec_batch_kernel(const BatchItemDev *__restrict__ items,
+                                                              const int2 *__restrict__ blocks) {
+  const int2 b = blocks[blockIdx.x];
+  const BatchItemDev &it = items[b.x];
+  const int r = b.y * 256 + threadIdx.x;
+  if (r < it.n_rot) plane_rec_one(it.g, it.rot, r, it.pre, it.ptest);
+}
+
+__global__ __launch_bounds__(256) void maxabs_batch_kernel(const BatchItemDev *__restrict__ items,
+                                                           const int3 *__restrict__ blocks, int pass) {
+  const int3 b = blocks[blockIdx.x];
+  const BatchItemDev &it = items[b.x];
+  maxabs_body(it.src, it.n_src, pass ? 1 : 64, pass ? (const unsigned *)(it.words + 2 * kSetWords) : nullptr,
+              it.words + (pass ? kSetWords : 0), b.y, b.z);
+}
+
+template <int BZ, int BY, int BX, int MODE>
+__global__ __launch_bounds__(256) void split_batch_kernel(const BatchItemDev *__restrict__ items,
+                                                          const BatchUnit *__restrict__ splits, int pass) {
+  const BatchUnit sp = splits[blockIdx.x];  // item, brick, number of segments, first slot
+  const BatchItemDev &it = items[sp.item];
+  const SplitRec rec{sp.brick, sp.seg, sp.slot};
+  split_body<BZ, BY, BX, MODE>(it.g, it.bg, rec, it.segbuf, batch_scale(it, pass), 0, it.out, it.wout,
+                               it.plane, it.wplane, it.halo, (int)(blockIdx.y * 256 + threadIdx.x));
+}
+
+template <int BZ, int BY, int BX, int MODE>
+__global__ __launch_bounds__(256) void halo_batch_kernel(const BatchItemDev *__restrict__ items,
+                                                         const int2 *__restrict__ bricks, int pass) {
+  const int2 b = bricks[blockIdx.x];  // item, brick
+  const BatchItemDev &it = items[b.x];
+  halo_body<BZ, BY, BX, MODE>(it.g, it.bg, it.halo, batch_scale(it, pass), it.out, it.wout, it.plane,
+                              it.wplane, b.y, (int)(blockIdx.y * 256 + threadIdx.x));
+}
+
+__global__ __launch_bounds__(256) void mirror_batch_kernel(const BatchItemDev *__restrict__ items,
+                                                           const int2 *__restrict__ blocks) {
+  const int2 b = blocks[blockIdx.x];
+  const BatchItemDev &it = items[b.x];
+  mirror_one(it.plane, it.wplane, it.g.vD, it.g.vH, it.g.vW, it.out, it.wout, 0,
+             (int)(b.y * 256 + threadIdx.x));
+}
+
+// Tile configurations the batch (and its autotuner) can use.
+struct TileCfg {
+  int bz, by, bx;
+};
+constexpr TileCfg kGatherCfg[2] = {{kGBZ, kGBY, kGBX}, {8, 8, 8}};
+constexpr TileCfg kInsertCfg[2] = {{kIBZ, kIBY, kIBX}, {8, 8, 8}};
+
+struct BatchGroup {
+  int cfg = 0;
+  int n_units = 0, n_splits = 0, n_bricks = 0;
+  BatchUnit *units = nullptr;   // device
+  BatchUnit *splits = nullptr;  // device
+  int2 *bricks = nullptr;       // device (item, brick) for the shell pass
+  unsigned *queue = nullptr;    // two work counters (pass 0, pass 1), in the zeroed region
+};
+
+struct BatchPlan {
+  int kind = 0;
+  int device = 0;
+  int grid = 0;  // resident blocks of a persistent launch
+  std::vector<tt_batch_item> items;
+  std::vector<int> cfg;  // tile configuration of each item
+  std::vector<BatchGroup> groups;
+  BatchItemDev *d_items = nullptr;
+  int2 *zero_blocks = nullptr, *rec_blocks = nullptr, *mirror_blocks = nullptr;
+  int3 *max0_blocks = nullptr, *max1_blocks = nullptr;
+  int n_zero = 0, n_rec = 0, n_mirror = 0, n_max0 = 0, n_max1 = 0;
+  void *zeroed = nullptr;  // counters, scale words, x = 0 planes: cleared by every run
+  size_t zeroed_bytes = 0;
+  void *scratch = nullptr;  // plane records, partial tiles, halo shells
+  std::vector<void *> tables;  // every uploaded table, the item table included
+  ~BatchPlan() {
+    for (void *p : tables) (void)hipFree(p);
+    if (zeroed) (void)hipFree(zeroed);
+    if (scratch) (void)hipFree(scratch);
+  }
+};
+
+int pending(const char *stage) {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return TT_OK;
+  set_error(std::string("batch plan (") + stage + "): " + hipGetErrorString(e));
+  return TT_ERR_HIP;
+}
+
+template <typename T>
+int upload(BatchPlan &bp, const std::vector<T> &h, T **d) {
+  *d = nullptr;
+  if (h.empty()) return TT_OK;
+  hipError_t e = hipMalloc((void **)d, h.size() * sizeof(T));
+  if (e == hipSuccess) e = hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error(std::string("batch table upload: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  bp.tables.push_back(*d);
+  return TT_OK;
+}
+
+template <int BZ, int BY, int BX, int MODE>
+void batch_sizes(const SliceGeom &g, int n_slots, size_t &seg_bytes, size_t &halo_bytes, int &n_bricks) {
+  typedef Cfg<BZ, BY, BX, MODE> C;
+  const BrickGrid bg = grid_of(g, BZ, BY, BX);
+  n_bricks = bg.nbz * bg.nby * bg.nbx;
+  seg_bytes = MODE == kGather ? 0 : (size_t)n_slots * C::NACC * C::NT * sizeof(u64);
+  halo_bytes = MODE == kGather ? 0 : (size_t)n_bricks * C::NACC * C::NSH * sizeof(u64);
+}
+
+int batch_item_sizes(int kind, int cfg, const SliceGeom &g, int n_slots, size_t &seg, size_t &halo,
+                     int &nb) {
+  if (kind == TT_BATCH_EXTRACT) {
+    if (cfg == 0) batch_sizes<kGatherCfg[0].bz, kGatherCfg[0].by, kGatherCfg[0].bx, kGather>(g, n_slots, seg, halo, nb);
+    else batch_sizes<kGatherCfg[1].bz, kGatherCfg[1].by, kGatherCfg[1].bx, kGather>(g, n_slots, seg, halo, nb);
+  } else {
+    if (cfg == 0) batch_sizes<kInsertCfg[0].bz, kInsertCfg[0].by, kInsertCfg[0].bx, kInsert>(g, n_slots, seg, halo, nb);
+    else batch_sizes<kInsertCfg[1].bz, kInsertCfg[1].by, kInsertCfg[1].bx, kInsert>(g, n_slots, seg, halo, nb);
+  }
+  return TT_OK;
+}
+
+SliceGeom batch_geom(const tt_batch_item &it) {
+  return make_geom(it.img_d, it.img_h, it.img_w, it.vol_d, it.vol_h, it.vol_wr, it.fftfreq_max, it.x_tolerance);
+}
+
+// Builds the plan for the items with the given tile configurations.
+int batch_build(BatchPlan &bp) {
+  const int n = (int)bp.items.size();
+  const bool ext = bp.kind == TT_BATCH_EXTRACT;
+  std::vector<SliceGeom> geo(n);
+  std::vector<PlanEntry> pes(n);
+  std::vector<size_t> seg_b(n), halo_b(n);
+  std::vector<int> nbr(n);
+  for (int i = 0; i < n; ++i) {
+    geo[i] = batch_geom(bp.items[i]);
+    const TileCfg tc = ext ? kGatherCfg[bp.cfg[i]] : kInsertCfg[bp.cfg[i]];
+    if (int st = get_plan(geo[i], tc.bz, tc.by, tc.bx, !ext, bp.items[i].n_rot, pes[i])) return st;
+    if (int st = pending("unit plan")) return st;
+    batch_item_sizes(bp.kind, bp.cfg[i], geo[i], pes[i].n_slots, seg_b[i], halo_b[i], nbr[i]);
+  }
+  // zeroed region: 2 queue words per group, then per item the scale words and
+  // (insertion) the x = 0 planes
+  const int n_groups = 2;
+  size_t zoff = 256;
+  std::vector<size_t> words_off(n), plane_off(n), wplane_off(n);
+  for (int i = 0; i < n; ++i) {
+    words_off[i] = zoff;
+    zoff += ((2 * kSetWords + 1) * sizeof(unsigned) + 255) & ~(size_t)255;
+    const size_t np = (size_t)geo[i].vD * geo[i].vH;
+    plane_off[i] = zoff;
+    if (!ext) zoff += (np * sizeof(f2v) + 255) & ~(size_t)255;
+    wplane_off[i] = zoff;
+    if (!ext) zoff += (np * sizeof(float) + 255) & ~(size_t)255;
+  }
+  // scratch: per item plane records and tests, partial tiles, halo shells
+  size_t soff = 0;
+  std::vector<size_t> pre_off(n), test_off(n), seg_off(n), halo_off(n);
+  for (int i = 0; i < n; ++i) {
+    const size_t nr = (size_t)std::max<int64_t>(bp.items[i].n_rot, 1);
+    pre_off[i] = soff;
+    soff += (nr * sizeof(PlaneRec) + 255) & ~(size_t)255;
+    test_off[i] = soff;
+    soff += (nr * sizeof(PlaneTest) + 255) & ~(size_t)255;
+    seg_off[i] = soff;
+    soff += (seg_b[i] + 255) & ~(size_t)255;
+    halo_off[i] = soff;
+    soff += (halo_b[i] + 255) & ~(size_t)255;
+  }
+  hipError_t e = hipMalloc(&bp.zeroed, zoff);
+  if (e == hipSuccess) e = hipMalloc(&bp.scratch, std::max<size_t>(soff, 256));
+  if (e != hipSuccess) {
+    set_error(std::string("batch scratch: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  bp.zeroed_bytes = zoff;
+  if (int st = pending("scratch")) return st;
+  char *z = (char *)bp.zeroed, *sc = (char *)bp.scratch;
+  std::vector<BatchItemDev> di(n);
+  for (int i = 0; i < n; ++i) {
+    const tt_batch_item &it = bp.items[i];
+    const TileCfg tc = ext ? kGatherCfg[bp.cfg[i]] : kInsertCfg[bp.cfg[i]];
+    BatchItemDev &d = di[i];
+    d.g = geo[i];
+    d.bg = grid_of(geo[i], tc.bz, tc.by, tc.bx);
+    d.src = (const f2v *)it.input;
+    d.rot = it.rot_zyx;
+    d.out = (f2v *)it.output;
+    d.wout = it.weights;
+    d.pre = (PlaneRec *)(sc + pre_off[i]);
+    d.ptest = (PlaneTest *)(sc + test_off[i]);
+    d.words = (unsigned *)(z + words_off[i]);
+    d.plane = ext ? nullptr : (f2v *)(z + plane_off[i]);
+    d.wplane = ext ? nullptr : (float *)(z + wplane_off[i]);
+    d.segbuf = (u64 *)(sc + seg_off[i]);
+    d.halo = (u64 *)(sc + halo_off[i]);
+    d.n_rot = (int)it.n_rot;
+    d.n_bricks = nbr[i];
+    d.n_src = it.n_rot * (int64_t)geo[i].P;
+  }
+  if (int st = upload(bp, di, &bp.d_items)) return st;
+  if (int st = pending("item table")) return st;
+  // groups: the units of all items of a configuration, heaviest first
+  bp.groups.clear();
+  for (int c = 0; c < n_groups; ++c) {
+    struct W {
+      BatchUnit u;
+      double est;
+    };
+    std::vector<W> w;
+    std::vector<BatchUnit> splits;
+    std::vector<int2> bricks;
+    for (int i = 0; i < n; ++i) {
+      if (bp.cfg[i] != c || (ext && bp.items[i].n_rot == 0)) continue;
+      const PlanEntry &pe = pes[i];
+      for (size_t k = 0; k < pe.h_units->size(); ++k) {
+        const Unit &u = (*pe.h_units)[k];
+        w.push_back({BatchUnit{i, u.brick, u.seg, u.slot}, (*pe.h_est)[k]});
+      }
+      for (const SplitRec &sp : *pe.h_splits) splits.push_back(BatchUnit{i, sp.brick, sp.nseg, sp.slot});
+      if (!ext)
+        for (int b = 0; b < nbr[i]; ++b) bricks.push_back(make_int2(i, b));
+    }
+    if (w.empty()) continue;
+    std::stable_sort(w.begin(), w.end(), [](const W &a, const W &b) { return a.est > b.est; });
+    std::vector<BatchUnit> units(w.size());
+    for (size_t k = 0; k < w.size(); ++k) units[k] = w[k].u;
+    BatchGroup grp;
+    grp.cfg = c;
+    grp.n_units = (int)units.size();
+    grp.n_splits = (int)splits.size();
+    grp.n_bricks = (int)bricks.size();
+    grp.queue = (unsigned *)z + 2 * c;
+    if (int st = upload(bp, units, &grp.units)) return st;
+    if (int st = upload(bp, splits, &grp.splits)) return st;
+    if (int st = upload(bp, bricks, &grp.bricks)) return st;
+    bp.groups.push_back(grp);
+    if (int st = pending("unit tables")) return st;
+  }
+  // helper block tables
+  std::vector<int2> zb, rb, mb;
+  std::vector<int3> m0, m1;
+  for (int i = 0; i < n; ++i) {
+    const int64_t nr = bp.items[i].n_rot;
+    if (nr == 0) continue;
+    for (int64_t b = 0; b < (nr + 255) / 256; ++b) rb.push_back(make_int2(i, (int)b));
+    if (ext) {
+      const int64_t nb = (nr * geo[i].H + 255) / 256;
+      for (int64_t b = 0; b < nb; ++b) zb.push_back(make_int2(i, (int)b));
+    } else {
+      const int64_t nv = nr * (int64_t)geo[i].P;
+      const int64_t nsamp = (nv / (64 * 8192) + 1) * 8192;
+      const int s0 = (int)std::min<int64_t>((nsamp + 1023) / 1024, 2048);
+      const int s1 = (int)std::min<int64_t>(std::max<int64_t>(nv / 4096, 1), 2048);
+      for (int b = 0; b < s0; ++b) m0.push_back(make_int3(i, b, s0));
+      for (int b = 0; b < s1; ++b) m1.push_back(make_int3(i, b, s1));
+    }
+  }
+  // the insertion's x = 0 mirror pass writes that plane of every volume
+  if (!ext)
+    for (int i = 0; i < n; ++i) {
+      const int64_t np = (int64_t)geo[i].vD * geo[i].vH;
+      for (int64_t b = 0; b < (np + 255) / 256; ++b) mb.push_back(make_int2(i, (int)b));
+    }
+  bp.n_zero = (int)zb.size();
+  bp.n_rec = (int)rb.size();
+  bp.n_mirror = (int)mb.size();
+  bp.n_max0 = (int)m0.size();
+  bp.n_max1 = (int)m1.size();
+  if (int st = upload(bp, zb, &bp.zero_blocks)) return st;
+  if (int st = upload(bp, rb, &bp.rec_blocks)) return st;
+  if (int st = upload(bp, mb, &bp.mirror_blocks)) return st;
+  if (int st = upload(bp, m0, &bp.max0_blocks)) return st;
+  if (int st = upload(bp, m1, &bp.max1_blocks)) return st;
+  return pending("block tables");
+}
+
+template <int BZ, int BY, int BX>
+int batch_run_gather(const BatchPlan &bp, const BatchGroup &grp, hipStream_t s) {
+  typedef Cfg<BZ, BY, BX, kGather> C;
+  hipLaunchKernelGGL((brick_batch_kernel<BZ, BY, BX, kGather>), dim3((unsigned)std::min(grp.n_units, bp.grid)),
+                     dim3(C::T), 0, s, (const BatchItemDev *)bp.d_items, (const BatchUnit *)grp.units, grp.n_units,
+                     grp.queue, 0);
+  return check_launch("brick_batch_kernel");
+}
+
+template <int BZ, int BY, int BX>
+int batch_run_insert_pass(const BatchPlan &bp, const BatchGroup &grp, int pass, hipStream_t s) {
+  typedef Cfg<BZ, BY, BX, kInsert> C;
+  hipLaunchKernelGGL((brick_batch_kernel<BZ, BY, BX, kInsert>), dim3((unsigned)std::min(grp.n_units, bp.grid)),
+                     dim3(C::T), 0, s, (const BatchItemDev *)bp.d_items, (const BatchUnit *)grp.units, grp.n_units,
+                     grp.queue + pass, pass);
+  if (int st = check_launch("brick_batch_kernel")) return st;
+  if (grp.n_splits > 0) {
+    hipLaunchKernelGGL((split_batch_kernel<BZ, BY, BX, kInsert>),
+                       dim3((unsigned)grp.n_splits, (unsigned)((C::NT + 255) / 256)), dim3(256), 0, s,
+                       (const BatchItemDev *)bp.d_items, (const BatchUnit *)grp.splits, pass);
+    if (int st = check_launch("split_batch_kernel")) return st;
+  }
+  if (grp.n_bricks > 0) {
+    hipLaunchKernelGGL((halo_batch_kernel<BZ, BY, BX, kInsert>),
+                       dim3((unsigned)grp.n_bricks, (unsigned)((C::NSH + 255) / 256)), dim3(256), 0, s,
+                       (const BatchItemDev *)bp.d_items, (const int2 *)grp.bricks, pass);
+    if (int st = check_launch("halo_batch_kernel")) return st;
+  }
+  return TT_OK;
+}
+
+int batch_run(const BatchPlan &bp, hipStream_t s) {
+  if (int st = fill_zero<unsigned>((unsigned *)bp.zeroed, (int64_t)(bp.zeroed_bytes / sizeof(unsigned)), s))
+    return st;
+  if (bp.kind == TT_BATCH_EXTRACT) {
+    if (bp.n_zero > 0) {
+      hipLaunchKernelGGL(gather_zero_batch_kernel, dim3((unsigned)bp.n_zero), dim3(256), 0, s,
+                         (const BatchItemDev *)bp.d_items, (const int2 *)bp.zero_blocks);
+      if (int st = check_launch("gather_zero_batch_kernel")) return st;
+    }
+    if (bp.n_rec > 0) {
+      hipLaunchKernelGGL(plane_rec_batch_kernel, dim3((unsigned)bp.n_rec), dim3(256), 0, s,
+                         (const BatchItemDev *)bp.d_items, (const int2 *)bp.rec_blocks);
+      if (int st = check_launch("plane_rec_batch_kernel")) return st;
+    }
+    for (const BatchGroup &grp : bp.groups) {
+      int st = grp.cfg == 0 ? batch_run_gather<kGatherCfg[0].bz, kGatherCfg[0].by, kGatherCfg[0].bx>(bp, grp, s)
+                            : batch_run_gather<kGatherCfg[1].bz, kGatherCfg[1].by, kGatherCfg[1].bx>(bp, grp, s);
+      if (st) return st;
+    }
+    return TT_OK;
+  }
+  if (bp.n_max0 > 0) {
+    hipLaunchKernelGGL(maxabs_batch_kernel, dim3((unsigned)bp.n_max0), dim3(256), 0, s,
+                       (const BatchItemDev *)bp.d_items, (const int3 *)bp.max0_blocks, 0);
+    if (int st = check_launch("maxabs_batch_kernel")) return st;
+  }
+  if (bp.n_rec > 0) {
+    hipLaunchKernelGGL(plane_rec_batch_kernel, dim3((unsigned)bp.n_rec), dim3(256), 0, s,
+                       (const BatchItemDev *)bp.d_items, (const int2 *)bp.rec_blocks);
+    if (int st = check_launch("plane_rec_batch_kernel")) return st;
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1 && bp.n_max1 > 0) {
+      hipLaunchKernelGGL(maxabs_batch_kernel, dim3((unsigned)bp.n_max1), dim3(256), 0, s,
+                         (const BatchItemDev *)bp.d_items, (const int3 *)bp.max1_blocks, 1);
+      if (int st = check_launch("maxabs_batch_kernel")) return st;
+    }
+    for (const BatchGroup &grp : bp.groups) {
+      int st = grp.cfg == 0
+                   ? batch_run_insert_pass<kInsertCfg[0].bz, kInsertCfg[0].by, kInsertCfg[0].bx>(bp, grp, pass, s)
+                   : batch_run_insert_pass<kInsertCfg[1].bz, kInsertCfg[1].by, kInsertCfg[1].bx>(bp, grp, pass, s);
+      if (st) return st;
+    }
+  }
+  if (bp.n_mirror > 0) {
+    hipLaunchKernelGGL(mirror_batch_kernel, dim3((unsigned)bp.n_mirror), dim3(256), 0, s,
+                       (const BatchItemDev *)bp.d_items, (const int2 *)bp.mirror_blocks);
+    if (int st = check_launch("mirror_batch_kernel")) return st;
+  }
+  return TT_OK;
+}
+
+int batch_time(BatchPlan &bp, hipStream_t s, float &ms) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_error("batch autotune: hipEventCreate failed");
+    return TT_ERR_HIP;
+  }
+  int st = batch_run(bp, s);  // warm-up
+  if (!st) {
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < 3 && !st; ++k) st = batch_run(bp, s);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
 }  // namespace
 
 bool engine_supports(const SliceGeom &g, int64_t n_rot) {
@@ -1491,3 +2008,125 @@ int engine_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64
 }
 
 }  // namespace tt
+
+// ---------------------------------------------------------------------------
+// C ABI of the batched launches
+// ---------------------------------------------------------------------------
+using namespace tt;
+
+namespace {
+
+int batch_check(int kind, const tt_batch_item &it) {
+  if (it.vol_d < 2 || it.vol_h < 2 || it.vol_wr < 2) return fail_invalid("batch item: volume dimensions must be >= 2");
+  if (it.img_d < 2 || it.img_h < 2 || it.img_w < 2) return fail_invalid("batch item: image dimensions must be >= 2");
+  if (it.n_rot < 0) return fail_invalid("batch item: n_rot must be >= 0");
+  if (kind == TT_BATCH_INSERT &&
+      (it.img_d != it.vol_d || it.img_h != it.vol_h || it.img_w / 2 + 1 != it.vol_wr))
+    return fail_invalid("batch item: insertion needs image shape = the full volume shape");
+  if ((int64_t)it.img_h * (it.img_w / 2 + 1) >= (1 << 22)) return fail_invalid("batch item: slice too large");
+  if ((int64_t)it.vol_d * it.vol_h * it.vol_wr >= ((int64_t)1 << 31)) return fail_invalid("batch item: volume too large");
+  if (it.n_rot > 0 && (!it.rot_zyx || !it.input)) return fail_invalid("batch item: NULL rotations or input");
+  if (!it.output || (kind == TT_BATCH_INSERT && !it.weights)) return fail_invalid("batch item: NULL output");
+  const SliceGeom g = batch_geom(it);
+  if (!engine_supports(g, it.n_rot)) return fail_invalid("batch item: beyond the brick engine's limits");
+  return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tt_batch_create(int kind, const tt_batch_item *items, int n_items, int autotune, void *stream,
+                    void **handle) {
+  if (!handle) return fail_invalid("handle pointer is NULL");
+  *handle = nullptr;
+  if (kind != TT_BATCH_EXTRACT && kind != TT_BATCH_INSERT) return fail_invalid("kind must be TT_BATCH_EXTRACT or TT_BATCH_INSERT");
+  if (n_items <= 0 || !items) return fail_invalid("a batch needs at least one item");
+  for (int i = 0; i < n_items; ++i)
+    if (int st = batch_check(kind, items[i])) return st;
+  hipStream_t s = (hipStream_t)stream;
+  int dev = 0, ncu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess || ncu <= 0) {
+    set_error(std::string("tt_batch_create: device query: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  if ((e = hipGetLastError()) != hipSuccess) {  // an earlier, unrelated failure: report it, do not run on it
+    set_error(std::string("tt_batch_create: pending HIP error: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  auto make = [&](const std::vector<int> &cfg, const std::vector<int> &which, BatchPlan *&out) -> int {
+    BatchPlan *bp = new BatchPlan();
+    bp->kind = kind;
+    bp->device = dev;
+    bp->grid = 2 * ncu;
+    for (int i : which) bp->items.push_back(items[i]);
+    bp->cfg = cfg;
+    int st = batch_build(*bp);
+    if (st) {
+      delete bp;
+      return st;
+    }
+    out = bp;
+    return TT_OK;
+  };
+  std::vector<int> all(n_items), cfg(n_items, 0);
+  for (int i = 0; i < n_items; ++i) all[i] = i;
+  if (autotune) {
+    // per box size (the image shape), time each tile configuration on the
+    // items of that size alone and keep the faster
+    std::map<std::tuple<int, int, int>, std::vector<int>> sizes;
+    for (int i = 0; i < n_items; ++i) sizes[std::make_tuple(items[i].img_d, items[i].img_h, items[i].img_w)].push_back(i);
+    for (auto &kv : sizes) {
+      float best = 3.0e38f;
+      int best_c = 0;
+      for (int c = 0; c < 2; ++c) {
+        BatchPlan *bp = nullptr;
+        if (int st = make(std::vector<int>(kv.second.size(), c), kv.second, bp)) return st;
+        float ms = 0.0f;
+        if ((e = hipGetLastError()) != hipSuccess) {
+          delete bp;
+          set_error(std::string("tt_batch_create: after planning: ") + hipGetErrorString(e));
+          return TT_ERR_HIP;
+        }
+        int st = batch_time(*bp, s, ms);
+        if (!st && hipStreamSynchronize(s) != hipSuccess) st = check_launch("batch autotune");
+        delete bp;
+        if (st) return st;
+        if (ms < best) {
+          best = ms;
+          best_c = c;
+        }
+      }
+      for (int i : kv.second) cfg[i] = best_c;
+    }
+  }
+  BatchPlan *bp = nullptr;
+  if (int st = make(cfg, all, bp)) return st;
+  *handle = bp;
+  return TT_OK;
+}
+
+int tt_batch_run(void *handle, void *stream) {
+  if (!handle) return fail_invalid("batch handle is NULL");
+  return batch_run(*(BatchPlan *)handle, (hipStream_t)stream);
+}
+
+int tt_batch_tile(void *handle, int item, int *bz, int *by, int *bx) {
+  if (!handle || !bz || !by || !bx) return fail_invalid("NULL argument");
+  const BatchPlan &bp = *(const BatchPlan *)handle;
+  if (item < 0 || item >= (int)bp.items.size()) return fail_invalid("item index out of range");
+  const TileCfg tc = bp.kind == TT_BATCH_EXTRACT ? kGatherCfg[bp.cfg[item]] : kInsertCfg[bp.cfg[item]];
+  *bz = tc.bz;
+  *by = tc.by;
+  *bx = tc.bx;
+  return TT_OK;
+}
+
+int tt_batch_destroy(void *handle) {
+  delete (BatchPlan *)handle;
+  return TT_OK;
+}
+
+}  // extern "C"

@@ -64,3 +64,24 @@ def test_invalid_arguments_rejected_without_gpu(lib):
     assert st == 1 and b"volume dimensions" in lib.tt_last_error()
     st = lib.tt_sample3d_f32(None, 1, 4, 4, 4, 0, None, 0, 7, None, None)
     assert st == 1 and b"interpolation" in lib.tt_last_error()
+
+
+def test_batch_create_rejects_bad_items_without_gpu(lib):
+    """The batch entry point validates its items on the host before any HIP call."""
+    import ctypes
+
+    from teamtomo_amd import _lib
+
+    lib = _lib.load()
+    items = (_lib.BatchItem * 1)()
+    items[0].vol_d, items[0].vol_h, items[0].vol_wr = 1, 4, 3
+    items[0].img_d, items[0].img_h, items[0].img_w = 4, 4, 4
+    h = ctypes.c_void_p()
+    st = lib.tt_batch_create(0, ctypes.cast(items, ctypes.c_void_p), 1, 0, None, ctypes.byref(h))
+    assert st == 1 and b"volume dimensions" in lib.tt_last_error() and not h.value
+    st = lib.tt_batch_create(7, ctypes.cast(items, ctypes.c_void_p), 1, 0, None, ctypes.byref(h))
+    assert st == 1 and b"kind" in lib.tt_last_error()
+    items[0].vol_d = 4
+    items[0].img_w = 8  # insertion needs the image shape to be the volume shape
+    st = lib.tt_batch_create(1, ctypes.cast(items, ctypes.c_void_p), 1, 0, None, ctypes.byref(h))
+    assert st == 1 and b"insertion" in lib.tt_last_error()
