@@ -185,7 +185,9 @@ template <int BZ, int BY, int BX, int MODE>
 struct Cfg {
   static constexpr int NW = 8;                          // waves per block
   static constexpr int T = NW * 64;
-  static constexpr int CH = 2;  // pixels per chunk
+  // pixels per chunk (measured at 256^3 x 4096: one pixel per lane makes the
+  // scatter 20 % slower, four spill registers)
+  static constexpr int CH = 2;
   // tile: the brick's cells lo..lo+B-1 and their corners, B + 1 voxels per axis
   static constexpr int SX = BX + 1, SY = BY + 1, SZ = BZ + 1;
   static constexpr int NT = SX * SY * SZ;
@@ -398,7 +400,7 @@ __device__ __forceinline__ void brick_body(
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
-  static_assert(CH == 2, "the chunk list holds pixel counts 1 and 2");
+  static_assert(CH >= 1 && CH <= 4, "the chunk list holds pixel counts 1 to 4");
   // gather: NT complex voxels; scatter: NACC fixed-point planes of NT words
   constexpr int NWORDS = SCATTER ? C::NACC * NT : NT;
   // one LDS block, the tile first: the trilinear corner offsets then fit the
@@ -407,7 +409,7 @@ __device__ __forceinline__ void brick_body(
     u64 tile[NWORDS];
     ItemRec item[NW][2 * kTake];
     int cross[NW][128];  // rotations whose plane meets the brick, pending
-    // chunks: item | row of the pass << 6 | (pixel count - 1) << 12 | first column << 13
+    // chunks: item | row of the pass << 6 | (pixel count - 1) << 12 | first column << 14
     unsigned seg[NW][kSegCap];
     int rowi[NW][64];  // slice row i of each row of the pass
   };
@@ -718,15 +720,15 @@ __device__ __forceinline__ void brick_body(
         for (int c = c0; c < c1; ++c) {
           const int q0 = (c - cexcl) * CH;
           segs[c - w0] = (unsigned)k | ((unsigned)lane << 6) | ((unsigned)(min(CH, len - q0) - 1) << 12) |
-                         ((unsigned)(jl + q0) << 13);
+                         ((unsigned)(jl + q0) << 14);
         }
         __builtin_amdgcn_wave_barrier();
         const int wn = min(total_ch - w0, kSegCap);
         for (int pc = 0; pc < wn; pc += 64) {
           const bool live = pc + lane < wn;
           const unsigned sgv = segs[live ? pc + lane : 0];
-          const int ki = sgv & 63, cnt = live ? (int)((sgv >> 12) & 1) + 1 : 0;
-          const int j0 = (int)(sgv >> 13);
+          const int ki = sgv & 63, cnt = live ? (int)((sgv >> 12) & 3) + 1 : 0;
+          const int j0 = (int)(sgv >> 14);
           const ItemRec &ir = items[ki];
           const int i = rowi[(sgv >> 6) & 63];
           const int iref = (int)(ir.ref & 0xffffu), jref = (int)(ir.ref >> 16);
