@@ -1386,7 +1386,7 @@ struct Scratch {
   }
 };
 
-constexpr int kGBZ = 16, kGBY = 16, kGBX = 8;  // gather brick (cells)
+constexpr int kGBZ = 8, kGBY = 16, kGBX = 16;  // gather brick (cells)
 constexpr int kABZ = 16, kABY = 16, kABX = 8;  // adjoint brick (cells)
 constexpr int kIBZ = 16, kIBY = 8, kIBX = 8;   // insertion brick (cells)
 
