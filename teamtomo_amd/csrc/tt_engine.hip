@@ -1066,11 +1066,12 @@ struct ZeroRow {
   int a1, n1, a2, n2, a3, n3;  // three runs of columns [a, a + n)
 };
 
-__device__ __forceinline__ void lin_interval(double A, double B, double lo, double hi, double &jlo,
-                                             double &jhi) {
-  // lo <= A + j B <= hi
+__device__ __forceinline__ void lin_interval(double A, double B, double iB, double lo, double hi,
+                                             double &jlo, double &jhi) {
+  // lo <= A + j B <= hi, iB = 1 / B (the bounds carry a 1e-7 margin, far
+  // above the rounding of the reciprocal)
   if (fabs(B) > 1e-30) {
-    const double t1 = (lo - A) / B, t2 = (hi - A) / B;
+    const double t1 = (lo - A) * iB, t2 = (hi - A) * iB;
     jlo = fmax(jlo, fmin(t1, t2));
     jhi = fmin(jhi, fmax(t1, t2));
   } else if (A < lo || A > hi) {
@@ -1091,6 +1092,9 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
     const Rot rr = load_rot(rot + r * 9, g);
     const double fy = canon_fy(g, i);
     const double m = 1e-7;
+    double iV[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) iV[d] = fabs(rr.V[d]) > 1e-30 ? 1.0 / rr.V[d] : 0.0;
     const double top[3] = {(double)(g.vD - 1), (double)(g.vH - 1), (double)(g.vW - 1)};
     int a[2], b[2];
 #pragma unroll
@@ -1099,11 +1103,11 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
       double jlo = 0.0, jhi = (double)(g.Wr - 1);
       // certain Friedel half: the extraction decision flips within tolx of x' = 0
       const double ux = __dmul_rn(rr.U[2], fy);
-      lin_interval(s * ux, s * rr.V[2], g.tolx + m, 1e300, jlo, jhi);
+      lin_interval(s * ux, s * rr.V[2], s * iV[2], g.tolx + m, 1e300, jlo, jhi);
       const double C0 = __fma_rn(s * rr.U[0], fy, g.bz), C1 = __fma_rn(s * rr.U[1], fy, g.by);
-      lin_interval(C0, s * rr.V[0], m, top[0] - m, jlo, jhi);
-      lin_interval(C1, s * rr.V[1], m, top[1] - m, jlo, jhi);
-      lin_interval(s * ux, s * rr.V[2], m, top[2] - m, jlo, jhi);
+      lin_interval(C0, s * rr.V[0], s * iV[0], m, top[0] - m, jlo, jhi);
+      lin_interval(C1, s * rr.V[1], s * iV[1], m, top[1] - m, jlo, jhi);
+      lin_interval(s * ux, s * rr.V[2], s * iV[2], m, top[2] - m, jlo, jhi);
       if (g.use_mask) {
         // in_disc (float32) holds wherever the exact radius is 1e-6 below fmax
         const double fm = (double)g.fmax * (1.0 - 1e-6);
