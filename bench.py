@@ -189,22 +189,27 @@ def time_torch_comparator(V, R, gS, steps=1, chunk=256):
     return N_ROT / dt, dt * 1e3, chunk
 
 
-def cpu_baseline(V, R, gS, n_rot=1024):
+def cpu_baseline(V, R, gS, min_s=10.0, chunk=256):
     """FP64 C oracle (OpenMP over rotations) on the host CPU: extract + adjoint of
-    a bounded sample of n_rot of the rotations at 256^3, reported per slice."""
+    the rotations in chunks of `chunk` until `min_s` seconds have passed or all
+    N_ROT are done (a bounded sample of the workload), reported per slice."""
     from oracle import c_oracle as C
     from oracle import fourier_slice as O
 
     Vn = V.cpu().numpy().astype(np.complex128)
-    Rn = O.prepare_rotations(R[:n_rot].cpu().numpy(), False, 1e-8)
-    g = gS[:n_rot].cpu().numpy()
+    Rn = O.prepare_rotations(R.cpu().numpy(), False, 1e-8)
+    g = gS.cpu().numpy()
     C.extract(Vn, (N, N, N), Rn[:2])  # load + first-touch outside the timing
-    t0 = time.perf_counter()
-    C.extract(Vn, (N, N, N), Rn)
-    C.extract_adjoint(g, Vn.shape, (N, N, N), Rn)
-    dt = time.perf_counter() - t0
-    return {"value": n_rot / dt, "unit": "slices/s", "cores": C.threads(), "kind": "port",
-            "sample": f"FP64 C oracle (oracle/c/tt_oracle.c, OpenMP), extract+adjoint of {n_rot} of the "
+    done, dt = 0, 0.0
+    while done < N_ROT and dt < min_s:
+        sl = slice(done, min(done + chunk, N_ROT))
+        t0 = time.perf_counter()
+        C.extract(Vn, (N, N, N), Rn[sl])
+        C.extract_adjoint(g[sl], Vn.shape, (N, N, N), Rn[sl])
+        dt += time.perf_counter() - t0
+        done = sl.stop
+    return {"value": done / dt, "unit": "slices/s", "cores": C.threads(), "kind": "port",
+            "sample": f"FP64 C oracle (oracle/c/tt_oracle.c, OpenMP), extract+adjoint of {done} of the "
                       f"{N_ROT} rotations at {N}^3 in {dt:.1f} s"}
 
 
