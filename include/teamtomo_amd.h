@@ -19,6 +19,8 @@
  *   torch-fourier-slice/src/torch_fourier_slice/slice_extraction/_extract_central_slices_rfft_3d.py:84
  *   torch-fourier-slice/src/torch_fourier_slice/slice_insertion/_insert_central_slices_rfft_3d.py:9,289
  *   torch-image-interpolation/src/torch_image_interpolation/image_interpolation_3d.py:11,104
+ *   torch-fourier-slice/src/torch_fourier_slice/slice_extraction/_extract_central_slices_rfft_2d.py:9
+ *   torch-image-interpolation/src/torch_image_interpolation/image_interpolation_2d.py:11,104
  */
 #ifndef TEAMTOMO_AMD_H
 #define TEAMTOMO_AMD_H
@@ -35,6 +37,8 @@ extern "C" {
 
 #define TT_INTERP_NEAREST 0
 #define TT_INTERP_TRILINEAR 1
+#define TT_INTERP_BILINEAR 1 /* the 2-D name of the linear mode  */
+#define TT_INTERP_BICUBIC 2  /* 2-D samplers only                */
 
 /* `flags` bits of the scatter entry points */
 #define TT_FLAG_DETERMINISTIC 1 /* fixed-point accumulation: bitwise reproducible      */
@@ -155,6 +159,66 @@ int tt_splat3d_f32(const float *values, const float *coords, int64_t n, int chan
                    float *image, void *weights, int weights_f64, void *stream);
 int tt_splat3d_f64(const double *values, const float *coords, int64_t n, int channels,
                    int is_complex, int d, int h, int w, int mode,
+                   double *image, void *weights, int weights_f64, void *stream);
+
+/*
+ * Central lines of a 2-D rfft (the 2-D analogue of tt_extract_c64).
+ * Replaces extract_central_slices_rfft_2d
+ * (torch-fourier-slice/.../slice_extraction/_extract_central_slices_rfft_2d.py:9-92)
+ * and its sample_image_2d / grid_sample call (.../image_interpolation_2d.py:11-101).
+ *   image       (h, wr) complex64 fftshifted rfft, DC at (h/2, 0)
+ *   img_h/img_w image_shape (h, w) that drives the frequency line
+ *   rot_yx      (n_rot, 2, 2) float32, already cast and flipped to yx
+ *   lines       (n_rot, img_w/2+1) complex64, written (every pixel)
+ *   fftfreq_max < 0 means None; otherwise pixels with f_x > fftfreq_max are 0
+ *               (the reference raises for every fftfreq_max: its elementwise
+ *               (w, 2) mask flattens the grid, :34-36).
+ */
+int tt_extract2d_c64(const void *image, int h, int wr, int img_h, int img_w,
+                     const float *rot_yx, int64_t n_rot, double fftfreq_max,
+                     void *lines, void *stream);
+
+/*
+ * Backward of tt_extract2d_c64: the image gradient (adjoint bilinear splat,
+ * accumulated into grad_image, complex64 (h, wr)) and/or the rotation
+ * gradient (accumulated into grad_rot, float64 (n_rot, 2, 2) yx).  Either
+ * output may be NULL; `image` is read only when grad_rot != NULL.
+ * Replaces torch autograd through grid_sample (grid_sampler_2d_backward
+ * reached from .../image_interpolation_2d.py:69-75).
+ */
+int tt_extract2d_bwd_c64(const void *grad_lines, const void *image, int h, int wr,
+                         int img_h, int img_w, const float *rot_yx, int64_t n_rot,
+                         double fftfreq_max, void *grad_image, double *grad_rot,
+                         void *stream);
+
+/*
+ * Generic 2-D sampler: replaces sample_image_2d
+ * (torch-image-interpolation/.../image_interpolation_2d.py:11-101).
+ *   image   (channels, h, w) real, or complex interleaved if is_complex
+ *   coords  (n, 2) yx array coordinates
+ *   out     (n, channels) values (complex interleaved if is_complex)
+ *   mode    TT_INTERP_BILINEAR, TT_INTERP_NEAREST or TT_INTERP_BICUBIC
+ * _f32: float32 image/coords/out; _f64: float64.
+ */
+int tt_sample2d_f32(const float *image, int channels, int h, int w, int is_complex,
+                    const float *coords, int64_t n, int mode, float *out, void *stream);
+int tt_sample2d_f64(const double *image, int channels, int h, int w, int is_complex,
+                    const double *coords, int64_t n, int mode, double *out, void *stream);
+
+/*
+ * Generic 2-D splat, accumulating in place: replaces insert_into_image_2d
+ * (torch-image-interpolation/.../image_interpolation_2d.py:104-275).
+ *   values  (n, channels) (complex interleaved if is_complex)
+ *   coords  (n, 2) float32 yx
+ *   image   (channels, h, w) accumulated
+ *   weights (h, w) accumulated; float32 if weights_f64 == 0 else float64
+ *   mode    TT_INTERP_BILINEAR or TT_INTERP_NEAREST
+ */
+int tt_splat2d_f32(const float *values, const float *coords, int64_t n, int channels,
+                   int is_complex, int h, int w, int mode,
+                   float *image, void *weights, int weights_f64, void *stream);
+int tt_splat2d_f64(const double *values, const float *coords, int64_t n, int channels,
+                   int is_complex, int h, int w, int mode,
                    double *image, void *weights, int weights_f64, void *stream);
 
 /*

@@ -1,9 +1,9 @@
 """teamtomo_amd: MI355X (gfx950) Fourier-slice gather/scatter for TeamTomo.
 
-Drop-in replacements for the 3-D central-slice operators of
-``torch_fourier_slice`` and the 3-D sampling/splatting of
-``torch_image_interpolation``, running as hand-written HIP kernels behind a
-C ABI (``include/teamtomo_amd.h``).  ``install_dropin()`` registers modules
+Drop-in replacements for the central-slice operators of
+``torch_fourier_slice`` (3-D slices and 2-D lines) and the 3-D / 2-D
+sampling and splatting of ``torch_image_interpolation``, running as
+hand-written HIP kernels behind a C ABI (``include/teamtomo_amd.h``).  ``install_dropin()`` registers modules
 under the reference package names so unchanged callers pick them up.
 """
 
@@ -16,15 +16,17 @@ from ._config import deterministic_enabled, set_deterministic, set_gather_path, 
 from .fourier_slice import (
     backproject_2d_to_3d,
     backproject_2d_to_3d_multichannel,
+    extract_central_slices_rfft_2d,
     extract_central_slices_rfft_3d,
     extract_central_slices_rfft_3d_multichannel,
     insert_central_slices_rfft_3d,
     insert_central_slices_rfft_3d_multichannel,
+    project_2d_to_1d,
     project_3d_to_2d,
     project_3d_to_2d_multichannel,
 )
 from .batch import SliceBatch
-from .image_interpolation import insert_into_image_3d, sample_image_3d
+from .image_interpolation import insert_into_image_2d, insert_into_image_3d, sample_image_2d, sample_image_3d
 
 __all__ = [
     "extract_central_slices_rfft_3d",
@@ -35,8 +37,12 @@ __all__ = [
     "project_3d_to_2d_multichannel",
     "backproject_2d_to_3d",
     "backproject_2d_to_3d_multichannel",
+    "extract_central_slices_rfft_2d",
+    "project_2d_to_1d",
     "sample_image_3d",
     "insert_into_image_3d",
+    "sample_image_2d",
+    "insert_into_image_2d",
     "set_deterministic",
     "deterministic_enabled",
     "set_pixel_atomic",
@@ -44,6 +50,20 @@ __all__ = [
     "install_dropin",
     "SliceBatch",
 ]
+
+# the names torch_fourier_slice exports (TFS/__init__.py:16-42)
+_TFS_NAMES = (
+    "extract_central_slices_rfft_3d",
+    "extract_central_slices_rfft_3d_multichannel",
+    "extract_central_slices_rfft_2d",
+    "insert_central_slices_rfft_3d",
+    "insert_central_slices_rfft_3d_multichannel",
+    "project_3d_to_2d",
+    "project_3d_to_2d_multichannel",
+    "project_2d_to_1d",
+    "backproject_2d_to_3d",
+    "backproject_2d_to_3d_multichannel",
+)
 
 
 def install_dropin() -> None:
@@ -54,15 +74,17 @@ def install_dropin() -> None:
     tfs = types.ModuleType("torch_fourier_slice")
     ext = types.ModuleType("torch_fourier_slice.slice_extraction")
     ins = types.ModuleType("torch_fourier_slice.slice_insertion")
-    for name in ("extract_central_slices_rfft_3d", "extract_central_slices_rfft_3d_multichannel"):
+    for name in ("extract_central_slices_rfft_3d", "extract_central_slices_rfft_3d_multichannel",
+                 "extract_central_slices_rfft_2d"):
         setattr(ext, name, globals()[name])
     for name in ("insert_central_slices_rfft_3d", "insert_central_slices_rfft_3d_multichannel"):
         setattr(ins, name, globals()[name])
-    for name in __all__[:8]:
+    for name in _TFS_NAMES:
         setattr(tfs, name, globals()[name])
     tfs.slice_extraction, tfs.slice_insertion = ext, ins
     tii = types.ModuleType("torch_image_interpolation")
-    tii.sample_image_3d, tii.insert_into_image_3d = sample_image_3d, insert_into_image_3d
+    for name in ("sample_image_3d", "insert_into_image_3d", "sample_image_2d", "insert_into_image_2d"):
+        setattr(tii, name, globals()[name])
     sys.modules.update({
         "torch_fourier_slice": tfs,
         "torch_fourier_slice.slice_extraction": ext,

@@ -17,10 +17,12 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
+TT_INTERP_BILINEAR = 1
+TT_INTERP_BICUBIC = 2
 
 TT_FLAG_DETERMINISTIC = 1
 TT_FLAG_OVERWRITE = 2
@@ -51,6 +53,12 @@ _SIGNATURES = {
     "tt_sample3d_f64": [_p, _i, _i, _i, _i, _i, _p, _l, _i, _p, _p],
     "tt_splat3d_f32": [_p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p],
     "tt_splat3d_f64": [_p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p],
+    "tt_extract2d_c64": [_p, _i, _i, _i, _i, _p, _l, _d, _p, _p],
+    "tt_extract2d_bwd_c64": [_p, _p, _i, _i, _i, _i, _p, _l, _d, _p, _p, _p],
+    "tt_sample2d_f32": [_p, _i, _i, _i, _i, _p, _l, _i, _p, _p],
+    "tt_sample2d_f64": [_p, _i, _i, _i, _i, _p, _l, _i, _p, _p],
+    "tt_splat2d_f32": [_p, _p, _l, _i, _i, _i, _i, _i, _p, _p, _i, _p],
+    "tt_splat2d_f64": [_p, _p, _l, _i, _i, _i, _i, _i, _p, _p, _i, _p],
     "tt_batch_create": [_i, _p, _i, _i, _p, ctypes.POINTER(ctypes.c_void_p)],
     "tt_batch_run": [_p, _p],
     "tt_batch_tile": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],

@@ -1,11 +1,14 @@
-// Generic 3-D sampling and splatting at arbitrary array coordinates
-// (the drop-in for torch_image_interpolation's 3-D entry points).
+// Generic 3-D and 2-D sampling and splatting at arbitrary array coordinates
+// (the drop-in for torch_image_interpolation's 3-D and 2-D entry points).
 //
 // sample: packages/primitives/torch-image-interpolation/src/torch_image_interpolation/
-//         image_interpolation_3d.py:11-101 (grid_sample, border, align_corners=True,
-//         zero outside [0, N-1]); trilinear corners floor / floor+1.
-// splat:  image_interpolation_3d.py:104-275 (index_put_ accumulate); trilinear
-//         corners floor / ceil with float32 weights, nearest = round half to even.
+//         image_interpolation_3d.py:11-101, image_interpolation_2d.py:11-101
+//         (grid_sample, border, align_corners=True, zero outside [0, N-1]);
+//         (tri/bi)linear corners floor / floor+1; 2-D bicubic = Keys cubic
+//         convolution (A = -0.75) over 4x4 taps clipped to the image.
+// splat:  image_interpolation_3d.py:104-275, image_interpolation_2d.py:104-275
+//         (index_put_ accumulate); linear corners floor / ceil with float32
+//         weights, nearest = round half to even.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -123,6 +126,110 @@ __global__ __launch_bounds__(kBlock) void splat3d_kernel(const T *__restrict__ v
   }
 }
 
+// Keys cubic-convolution weights (A = -0.75, grid_sample's bicubic) of the
+// taps at -1, 0, 1, 2 for the fraction t.
+template <typename T>
+__device__ __forceinline__ void cubic_weights(T t, T (&wt)[4]) {
+  const T A = (T)-0.75;
+  const T x0 = t + (T)1, x1 = t, x2 = (T)1 - t, x3 = (T)2 - t;
+  wt[0] = ((A * x0 - (T)5 * A) * x0 + (T)8 * A) * x0 - (T)4 * A;
+  wt[1] = ((A + (T)2) * x1 - (A + (T)3)) * x1 * x1 + (T)1;
+  wt[2] = ((A + (T)2) * x2 - (A + (T)3)) * x2 * x2 + (T)1;
+  wt[3] = ((A * x3 - (T)5 * A) * x3 + (T)8 * A) * x3 - (T)4 * A;
+}
+
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(kBlock) void sample2d_kernel(const T *__restrict__ img, int C, int h, int w,
+                                                          const T *__restrict__ coords, int64_t n,
+                                                          int mode, T *__restrict__ out) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t plane = (int64_t)h * w;
+  for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kBlock) {
+    T cy = coords[2 * p], cx = coords[2 * p + 1];
+    bool ok = cy >= 0 && cy <= (T)(h - 1) && cx >= 0 && cx <= (T)(w - 1);
+    T *o = out + p * C * E;
+    if (!ok) {
+      for (int q = 0; q < C * E; ++q) o[q] = 0;
+      continue;
+    }
+    if (mode == TT_INTERP_NEAREST) {
+      const int64_t e = (int64_t)nearest_index_f(cy, h) * w + (int64_t)nearest_index_f(cx, w);
+      for (int ch = 0; ch < C; ++ch)
+        for (int q = 0; q < E; ++q) o[ch * E + q] = img[(ch * plane + e) * E + q];
+      continue;
+    }
+    const int y0 = (int)floor(cy), x0 = (int)floor(cx);
+    const T ty = cy - (T)y0, tx = cx - (T)x0;
+    if (mode == TT_INTERP_BICUBIC) {
+      T wy[4], wx[4];
+      cubic_weights(ty, wy);
+      cubic_weights(tx, wx);
+      for (int ch = 0; ch < C; ++ch) {
+        T acc[2] = {0, 0};
+        for (int a = 0; a < 4; ++a) {
+          const int yy = min(max(y0 - 1 + a, 0), h - 1);
+          for (int b = 0; b < 4; ++b) {
+            const int xx = min(max(x0 - 1 + b, 0), w - 1);
+            const T wt = wy[a] * wx[b];
+            const int64_t e = ch * plane + (int64_t)yy * w + xx;
+            for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
+          }
+        }
+        for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
+      }
+      continue;
+    }
+    for (int ch = 0; ch < C; ++ch) {
+      T acc[2] = {0, 0};
+      for (int q = 0; q < 4; ++q) {
+        const int dy = q >> 1, dx = q & 1;
+        const int y = y0 + dy, x = x0 + dx;
+        if (y >= h || x >= w) continue;  // weight is 0 there
+        const T wt = (dy ? ty : (T)1 - ty) * (dx ? tx : (T)1 - tx);
+        const int64_t e = ch * plane + (int64_t)y * w + x;
+        for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
+      }
+      for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
+    }
+  }
+}
+
+template <typename T, typename TW, bool CPLX>
+__global__ __launch_bounds__(kBlock) void splat2d_kernel(const T *__restrict__ vals,
+                                                         const float *__restrict__ coords,
+                                                         int64_t n, int C, int h, int w, int mode,
+                                                         T *__restrict__ img, TW *__restrict__ wts) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t plane = (int64_t)h * w;
+  for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kBlock) {
+    const float cy = coords[2 * p], cx = coords[2 * p + 1];
+    if (!(cy >= 0.0f && cy <= (float)(h - 1) && cx >= 0.0f && cx <= (float)(w - 1))) continue;
+    const T *v = vals + p * C * E;
+    if (mode == TT_INTERP_NEAREST) {
+      const int64_t e = (int64_t)rintf(cy) * w + (int64_t)rintf(cx);
+      for (int ch = 0; ch < C; ++ch)
+        for (int k = 0; k < E; ++k) unsafeAtomicAdd(img + (ch * plane + e) * E + k, v[ch * E + k]);
+      unsafeAtomicAdd(wts + e, (TW)1);
+      continue;
+    }
+    const int y0 = (int)floorf(cy), x0 = (int)floorf(cx);
+    const int y1 = (int)ceilf(cy), x1 = (int)ceilf(cx);
+    const float ty = cy - (float)y0, tx = cx - (float)x0;
+    for (int q = 0; q < 4; ++q) {
+      const int dy = q >> 1, dx = q & 1;
+      const int y = dy ? y1 : y0, x = dx ? x1 : x0;
+      const float wt = __fmul_rn(dy ? ty : 1.0f - ty, dx ? tx : 1.0f - tx);
+      const int64_t e = (int64_t)y * w + x;
+      for (int ch = 0; ch < C; ++ch)
+        for (int k = 0; k < E; ++k)
+          unsafeAtomicAdd(img + (ch * plane + e) * E + k, v[ch * E + k] * (T)wt);
+      unsafeAtomicAdd(wts + e, (TW)wt);
+    }
+  }
+}
+
 int grid_for(int64_t n) {
   int64_t b = (n + kBlock - 1) / kBlock;
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 20));
@@ -182,6 +289,54 @@ int splat_impl(const T *values, const float *coords, int64_t n, int channels, in
   return check_launch("splat3d_kernel");
 }
 
+template <typename T>
+int sample2d_impl(const T *image, int channels, int h, int w, int is_complex, const T *coords, int64_t n,
+                  int mode, T *out, void *stream) {
+  int st = check_image(channels, 1, h, w);
+  if (st) return st;
+  if (mode != TT_INTERP_NEAREST && mode != TT_INTERP_BILINEAR && mode != TT_INTERP_BICUBIC)
+    return fail_invalid("unknown interpolation mode");
+  if (n <= 0) return TT_OK;
+  if (!image || !coords || !out) return fail_invalid("NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (is_complex)
+    hipLaunchKernelGGL((sample2d_kernel<T, true>), dim3(grid_for(n)), dim3(kBlock), 0, s, image, channels,
+                       h, w, coords, n, mode, out);
+  else
+    hipLaunchKernelGGL((sample2d_kernel<T, false>), dim3(grid_for(n)), dim3(kBlock), 0, s, image, channels,
+                       h, w, coords, n, mode, out);
+  return check_launch("sample2d_kernel");
+}
+
+template <typename T>
+int splat2d_impl(const T *values, const float *coords, int64_t n, int channels, int is_complex, int h,
+                 int w, int mode, T *image, void *weights, int weights_f64, void *stream) {
+  int st = check_image(channels, 1, h, w);
+  if (st) return st;
+  if (mode != TT_INTERP_NEAREST && mode != TT_INTERP_BILINEAR)
+    return fail_invalid("unknown interpolation mode");
+  if (n <= 0) return TT_OK;
+  if (!values || !coords || !image || !weights) return fail_invalid("NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid(grid_for(n)), block(kBlock);
+  if (weights_f64) {
+    if (is_complex)
+      hipLaunchKernelGGL((splat2d_kernel<T, double, true>), grid, block, 0, s, values, coords, n, channels,
+                         h, w, mode, image, (double *)weights);
+    else
+      hipLaunchKernelGGL((splat2d_kernel<T, double, false>), grid, block, 0, s, values, coords, n, channels,
+                         h, w, mode, image, (double *)weights);
+  } else {
+    if (is_complex)
+      hipLaunchKernelGGL((splat2d_kernel<T, float, true>), grid, block, 0, s, values, coords, n, channels,
+                         h, w, mode, image, (float *)weights);
+    else
+      hipLaunchKernelGGL((splat2d_kernel<T, float, false>), grid, block, 0, s, values, coords, n, channels,
+                         h, w, mode, image, (float *)weights);
+  }
+  return check_launch("splat2d_kernel");
+}
+
 }  // namespace
 
 }  // namespace tt
@@ -212,6 +367,28 @@ int tt_splat3d_f64(const double *values, const float *coords, int64_t n, int cha
                    int weights_f64, void *stream) {
   return splat_impl<double>(values, coords, n, channels, is_complex, d, h, w, mode, image, weights,
                             weights_f64, stream);
+}
+
+int tt_sample2d_f32(const float *image, int channels, int h, int w, int is_complex, const float *coords,
+                    int64_t n, int mode, float *out, void *stream) {
+  return sample2d_impl<float>(image, channels, h, w, is_complex, coords, n, mode, out, stream);
+}
+
+int tt_sample2d_f64(const double *image, int channels, int h, int w, int is_complex, const double *coords,
+                    int64_t n, int mode, double *out, void *stream) {
+  return sample2d_impl<double>(image, channels, h, w, is_complex, coords, n, mode, out, stream);
+}
+
+int tt_splat2d_f32(const float *values, const float *coords, int64_t n, int channels, int is_complex,
+                   int h, int w, int mode, float *image, void *weights, int weights_f64, void *stream) {
+  return splat2d_impl<float>(values, coords, n, channels, is_complex, h, w, mode, image, weights,
+                             weights_f64, stream);
+}
+
+int tt_splat2d_f64(const double *values, const float *coords, int64_t n, int channels, int is_complex,
+                   int h, int w, int mode, double *image, void *weights, int weights_f64, void *stream) {
+  return splat2d_impl<double>(values, coords, n, channels, is_complex, h, w, mode, image, weights,
+                              weights_f64, stream);
 }
 
 }  // extern "C"

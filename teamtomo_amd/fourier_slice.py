@@ -36,6 +36,8 @@ __all__ = [
     "project_3d_to_2d_multichannel",
     "backproject_2d_to_3d",
     "backproject_2d_to_3d_multichannel",
+    "extract_central_slices_rfft_2d",
+    "project_2d_to_1d",
 ]
 
 
@@ -373,3 +375,114 @@ def backproject_2d_to_3d_multichannel(
         for c in range(images.shape[-3])
     ]
     return torch.stack(outs, dim=0)
+
+
+# ---------------------------------------------------------------- 2-D analogues
+
+
+class _ExtractLineFn(torch.autograd.Function):
+    """lines = gather(image, R_yx); backward = adjoint splat + rotation grad."""
+
+    @staticmethod
+    def forward(ctx, image, R, image_shape, fftfreq_max):
+        n = R.shape[0]
+        h, wr = image.shape
+        img_h, img_w = (int(v) for v in image_shape)
+        out = torch.empty((n, _rfft_width(img_w)), dtype=image.dtype, device=image.device)
+        _lib.call("tt_extract2d_c64", _lib.ptr(image), h, wr, img_h, img_w, _lib.ptr(R), n,
+                  _fmax_arg(fftfreq_max), _lib.ptr(out), _lib.stream_of(image))
+        ctx.save_for_backward(image, R)
+        ctx.geom = (img_h, img_w, fftfreq_max)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_lines):
+        image, R = ctx.saved_tensors
+        img_h, img_w, fftfreq_max = ctx.geom
+        need_i, need_r = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_i or need_r):
+            return None, None, None, None
+        g = grad_lines.to(torch.complex64).contiguous()
+        h, wr = image.shape
+        gI = torch.zeros_like(image) if need_i else None
+        gR = torch.zeros((R.shape[0], 2, 2), dtype=torch.float64, device=R.device) if need_r else None
+        _lib.call("tt_extract2d_bwd_c64", _lib.ptr(g), _lib.ptr(image), h, wr, img_h, img_w, _lib.ptr(R),
+                  R.shape[0], _fmax_arg(fftfreq_max), _lib.ptr(gI), _lib.ptr(gR), _lib.stream_of(image))
+        return gI, (gR.to(R.dtype) if gR is not None else None), None, None
+
+
+def extract_central_slices_rfft_2d(
+    image_rfft: torch.Tensor,
+    image_shape: tuple[int, int],
+    rotation_matrices: torch.Tensor,
+    fftfreq_max: float | None = None,
+    yx_matrices: bool = False,
+) -> torch.Tensor:
+    """Extract central lines from an fftshifted 2-D rfft.
+
+    Reference: slice_extraction/_extract_central_slices_rfft_2d.py:9-92.
+    ``image_rfft`` is (h, w//2+1) complex64 on a ROCm device; returns
+    (..., w//2+1) for rotation matrices of shape (..., 2, 2).  The reference
+    raises einops.EinopsError for every ``fftfreq_max`` (its elementwise mask,
+    :34-36, flattens the grid); here the line is cut at |f| <= fftfreq_max.
+    """
+    _lib.require_gpu(image_rfft)
+    if image_rfft.ndim != 2:
+        raise ValueError(f"image_rfft should have shape (h, w//2+1), got {tuple(image_rfft.shape)}")
+    if image_rfft.dtype != torch.complex64:
+        raise RuntimeError(
+            "extract_central_slices_rfft_2d: image_rfft must be complex64 "
+            f"(grid_sample needs input and grid of the same dtype), got {image_rfft.dtype}"
+        )
+    R = rotation_matrices.to(device=image_rfft.device, dtype=torch.float32)
+    if not yx_matrices:
+        R = torch.flip(R, dims=(-2, -1))
+    stack_shape = tuple(R.shape[:-2])
+    R = R.reshape(-1, 2, 2).contiguous()
+    out = _ExtractLineFn.apply(image_rfft.contiguous(), R, tuple(image_shape), fftfreq_max)
+    return out.reshape(*stack_shape, out.shape[-1])
+
+
+def _sinc2_grid_2d(shape, device):
+    """sinc(|f|)^2 on the fftshifted full 2-D frequency grid (TFS/project.py:218-225)."""
+    axes = [torch.fft.fftshift(torch.fft.fftfreq(n, device=device)) for n in shape]
+    gy, gx = torch.meshgrid(*axes, indexing="ij")
+    return torch.sinc((gy * gy + gx * gx) ** 0.5) ** 2
+
+
+def project_2d_to_1d(
+    image: torch.Tensor,
+    rotation_matrices: torch.Tensor,
+    pad_factor: float = 2.0,
+    fftfreq_max: float | None = None,
+    yx_matrices: bool = False,
+) -> torch.Tensor:
+    """Project a square image by sampling central lines of its DFT.
+
+    Reference: torch_fourier_slice/project.py:180-260 (padding, sinc^2
+    gridding correction and FFTs on torch; the central-line gather on gfx950).
+    Returns (..., d) for rotation matrices of shape (..., 2, 2).
+    """
+    h, w = image.shape[-2:]
+    if h != w:
+        raise ValueError("all dimensions of image must be equal.")
+    if pad_factor < 1.0:
+        raise ValueError("pad_factor must be >= 1.0")
+    p = 0
+    if pad_factor > 1.0:
+        p = int((image.shape[-1] * (pad_factor - 1.0)) // 2)
+        image = F.pad(image, pad=[p] * 4)
+    image_shape = tuple(image.shape[-2:])
+    image = image * _sinc2_grid_2d(image_shape, image.device)
+    dft = torch.fft.fftshift(image, dim=(-2, -1))
+    dft = torch.fft.rfftn(dft, dim=(-2, -1))
+    dft = torch.fft.fftshift(dft, dim=(-2,))
+    lines = extract_central_slices_rfft_2d(
+        image_rfft=dft, image_shape=image_shape, rotation_matrices=rotation_matrices,
+        fftfreq_max=fftfreq_max, yx_matrices=yx_matrices,
+    )
+    projections = torch.fft.irfftn(lines, dim=(-1,))
+    projections = torch.fft.ifftshift(projections, dim=(-1,))
+    if pad_factor > 1.0:
+        projections = F.pad(projections, pad=[-p] * 2)
+    return projections

@@ -1,8 +1,11 @@
-"""Drop-in replacements for torch_image_interpolation's 3-D entry points.
+"""Drop-in replacements for torch_image_interpolation's 3-D and 2-D entry points.
 
 Reference: packages/primitives/torch-image-interpolation/src/
-torch_image_interpolation/image_interpolation_3d.py.  Sampling and splatting
-run as gfx950 HIP kernels through the C ABI (include/teamtomo_amd.h).
+torch_image_interpolation/image_interpolation_3d.py and image_interpolation_2d.py.
+Sampling and splatting run as gfx950 HIP kernels through the C ABI
+(include/teamtomo_amd.h).  These generic samplers are forward-only: a call
+whose image or values require grad raises, rather than returning a result
+without a gradient.
 """
 
 from __future__ import annotations
@@ -13,7 +16,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["sample_image_3d", "insert_into_image_3d"]
+__all__ = ["sample_image_3d", "insert_into_image_3d", "sample_image_2d", "insert_into_image_2d"]
 
 _REAL_OF = {
     torch.float32: torch.float32,
@@ -23,16 +26,26 @@ _REAL_OF = {
 }
 
 
-def _mode(interpolation: str) -> int:
-    if interpolation == "trilinear":
+def _mode(interpolation: str, linear: str = "trilinear", bicubic: bool = False) -> int:
+    if interpolation == linear:
         return _lib.TT_INTERP_TRILINEAR
     if interpolation == "nearest":
         return _lib.TT_INTERP_NEAREST
+    if bicubic and interpolation == "bicubic":
+        return _lib.TT_INTERP_BICUBIC
     # grid_sample's own error for an unknown mode
     raise ValueError(
         "nn.functional.grid_sample(): expected mode to be 'bilinear', 'nearest' or "
         f"'bicubic', but got: '{interpolation}'"
     )
+
+
+def _no_grad(*tensors) -> None:
+    if torch.is_grad_enabled() and any(t.requires_grad for t in tensors):
+        raise RuntimeError(
+            "teamtomo_amd: the generic samplers have no backward; call them on tensors "
+            "that do not require grad (or under torch.no_grad())"
+        )
 
 
 def sample_image_3d(
@@ -48,6 +61,7 @@ def sample_image_3d(
     if image.ndim not in (3, 4):
         raise ValueError(f"image should have shape (d, h, w) or (c, d, h, w), got {image.shape}")
     _lib.require_gpu(image)
+    _no_grad(image, coordinates)
     if image.dtype not in _REAL_OF:
         raise TypeError(f"sample_image_3d: unsupported image dtype {image.dtype}")
     real_t = _REAL_OF[image.dtype]
@@ -96,6 +110,7 @@ def insert_into_image_3d(
     if image.dtype != values.dtype:
         raise ValueError("Image and values must have the same dtype.")
     _lib.require_gpu(image, values)
+    _no_grad(image, values)
     if weights is None:
         weights = torch.zeros(size=(d, h, w), dtype=torch.float32, device=image.device)
     if interpolation not in ("nearest", "trilinear"):
@@ -114,6 +129,90 @@ def insert_into_image_3d(
     fn = "tt_splat3d_f32" if real_t == torch.float32 else "tt_splat3d_f64"
     _lib.call(fn, _lib.ptr(vals), _lib.ptr(crd), n, C, int(image.is_complex()), d, h, w,
               _lib.TT_INTERP_TRILINEAR if interpolation == "trilinear" else _lib.TT_INTERP_NEAREST,
+              _lib.ptr(image), _lib.ptr(weights), int(weights.dtype == torch.float64),
+              _lib.stream_of(image))
+    return image, weights
+
+
+def sample_image_2d(
+    image: torch.Tensor,
+    coordinates: torch.Tensor,
+    interpolation: Literal["nearest", "bilinear", "bicubic"] = "bilinear",
+) -> torch.Tensor:
+    """Sample a (h, w) or (c, h, w) image at (..., 2) yx array coordinates.
+
+    Reference: image_interpolation_2d.py:11-101.  Returns (...) or (..., c);
+    points outside [0, N-1] in any dimension give 0.  bicubic is grid_sample's
+    Keys cubic convolution with border-clipped taps.
+    """
+    if image.ndim not in (2, 3):
+        raise ValueError(f"image should have shape (h, w) or (c, h, w), got {image.shape}")
+    _lib.require_gpu(image)
+    _no_grad(image, coordinates)
+    if image.dtype not in _REAL_OF:
+        raise TypeError(f"sample_image_2d: unsupported image dtype {image.dtype}")
+    real_t = _REAL_OF[image.dtype]
+    if coordinates.is_floating_point() and coordinates.dtype != real_t:
+        raise RuntimeError(
+            "grid_sampler(): expected input and grid to have same dtype, but input has "
+            f"{real_t} and grid has {coordinates.dtype}"
+        )
+    mode = _mode(interpolation, linear="bilinear", bicubic=True)
+    multichannel = image.ndim == 3
+    img = (image if multichannel else image.unsqueeze(0)).contiguous()
+    C, h, w = img.shape
+    lead = tuple(coordinates.shape[:-1])
+    crd = coordinates.to(device=image.device, dtype=real_t).reshape(-1, 2).contiguous()
+    n = crd.shape[0]
+    out = torch.empty((n, C), dtype=image.dtype, device=image.device)
+    fn = "tt_sample2d_f32" if real_t == torch.float32 else "tt_sample2d_f64"
+    _lib.call(fn, _lib.ptr(img), C, h, w, int(image.is_complex()), _lib.ptr(crd), n, mode, _lib.ptr(out),
+              _lib.stream_of(image))
+    out = out.reshape(*lead, C)
+    return out if multichannel else out[..., 0]
+
+
+def insert_into_image_2d(
+    values: torch.Tensor,
+    coordinates: torch.Tensor,
+    image: torch.Tensor,
+    weights: torch.Tensor | None = None,
+    interpolation: Literal["nearest", "bilinear"] = "bilinear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """Splat values into ``image`` (and ``weights``) in place; returns both.
+
+    Reference: image_interpolation_2d.py:104-275.  Bilinear splats use
+    floor/ceil corners, nearest rounds half to even with weight 1; points
+    outside [0, N-1] are dropped and an unknown mode changes nothing.
+    """
+    multichannel = image.ndim == 3
+    h, w = image.shape[-2:]
+    values_shape = values.shape[:-1] if multichannel else values.shape
+    if values_shape != coordinates.shape[:-1]:
+        raise ValueError("One coordinate pair is required for each value in data.")
+    if coordinates.shape[-1] != 2:
+        raise ValueError("Coordinates must be 2D with shape (..., 2).")
+    if image.dtype != values.dtype:
+        raise ValueError("Image and values must have the same dtype.")
+    _lib.require_gpu(image, values)
+    _no_grad(image, values)
+    if weights is None:
+        weights = torch.zeros(size=(h, w), dtype=torch.float32, device=image.device)
+    if interpolation not in ("nearest", "bilinear"):
+        return image, weights
+    if image.dtype not in _REAL_OF:
+        raise TypeError(f"insert_into_image_2d: unsupported image dtype {image.dtype}")
+    if weights.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"insert_into_image_2d: weights must be float32 or float64, got {weights.dtype}")
+    if not image.is_contiguous() or not weights.is_contiguous():
+        raise ValueError("insert_into_image_2d: image and weights must be contiguous (updated in place)")
+    C = image.shape[0] if multichannel else 1
+    vals = values.reshape(-1, C).contiguous()
+    crd = coordinates.to(device=image.device, dtype=torch.float32).reshape(-1, 2).contiguous()
+    n = crd.shape[0]
+    fn = "tt_splat2d_f32" if _REAL_OF[image.dtype] == torch.float32 else "tt_splat2d_f64"
+    _lib.call(fn, _lib.ptr(vals), _lib.ptr(crd), n, C, int(image.is_complex()), h, w,
+              _lib.TT_INTERP_BILINEAR if interpolation == "bilinear" else _lib.TT_INTERP_NEAREST,
               _lib.ptr(image), _lib.ptr(weights), int(weights.dtype == torch.float64),
               _lib.stream_of(image))
     return image, weights

@@ -1,8 +1,8 @@
 """Write the golden input/output vectors of the reference Fourier-slice path.
 
 Run once, on a machine that has the reference sources, with
-``python tests/golden/make_golden.py [REFERENCE_ROOT]`` (default
-``/root/reference``).  It imports the reference's pure-Python packages
+``python tests/golden/make_golden.py [REFERENCE_ROOT] [--2d]`` (default
+``/root/reference``; ``--2d`` writes only the fixtures of the 2-D analogues).  It imports the reference's pure-Python packages
 (torch-fourier-slice, torch-image-interpolation, torch-grid-utils) on the CPU
 and stores inputs and outputs as small ``.npz`` fixtures next to this script.
 Nothing at test time reads the reference: the tests load the fixtures.
@@ -231,5 +231,115 @@ def main(root):
          friedel_volume=Vf.numpy(), friedel_weights=Wf.numpy())
 
 
+def rotations_2d(n, seed):
+    """(n, 2, 2) xy rotation matrices at uniform angles, plus the exact
+    0/90/180/270-degree ones first (float32)."""
+    import torch
+
+    g = torch.Generator().manual_seed(seed)
+    theta = torch.rand(n, generator=g, dtype=torch.float64) * 2 * np.pi
+    c, s = torch.cos(theta), torch.sin(theta)
+    R = torch.stack([c, -s, s, c], dim=-1).reshape(n, 2, 2)
+    exact = torch.tensor([[[1.0, 0.0], [0.0, 1.0]], [[0.0, -1.0], [1.0, 0.0]],
+                          [[-1.0, 0.0], [0.0, -1.0]], [[0.0, 1.0], [-1.0, 0.0]]], dtype=torch.float64)
+    return torch.cat([exact, R]).to(torch.float32)
+
+
+def image_rfft_2d(img):
+    import torch
+
+    f = torch.fft.fftshift(img, dim=(-2, -1))
+    f = torch.fft.rfftn(f, dim=(-2, -1))
+    return torch.fft.fftshift(f, dim=(-2,))
+
+
+def main_2d(root):
+    """The 2-D analogues: central lines of 2-D rffts, project_2d_to_1d, and
+    the 2-D sampler / splat (TFS/slice_extraction/_extract_central_slices_rfft_2d.py,
+    TFS/project.py:180-260, TII/image_interpolation_2d.py)."""
+    import torch
+
+    torch.set_num_threads(8)
+    tfs, tii = _import_reference(root)
+
+    def save(name, **arrays):
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
+        print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+    # G11: central lines of a 32x32 and an odd 33x33 N(0,1) image, and of a
+    # 16x16 image with image_shape (32, 32).
+    out = {}
+    for key, n, shape, seed in (("even", 32, (32, 32), 21), ("odd", 33, (33, 33), 22),
+                                ("shape2x", 16, (32, 32), 23)):
+        g = torch.Generator().manual_seed(seed)
+        F = image_rfft_2d(torch.randn(n, n, generator=g))
+        R = rotations_2d(20, seed + 100)
+        out[f"{key}_image_rfft"] = F.numpy()
+        out[f"{key}_rotations"] = R.numpy()
+        out[f"{key}_lines"] = tfs.extract_central_slices_rfft_2d(F, shape, R.clone()).numpy()
+        out[f"{key}_lines_yx"] = tfs.extract_central_slices_rfft_2d(F, shape, R.clone(),
+                                                                    yx_matrices=True).numpy()
+    # G12: project_2d_to_1d of a 24x24 blob image, pad 2 and pad 1.
+    g = torch.Generator().manual_seed(24)
+    yy, xx = torch.meshgrid(torch.arange(24.0), torch.arange(24.0), indexing="ij")
+    img = torch.zeros(24, 24)
+    for _ in range(4):
+        cy, cx = (torch.rand(2, generator=g) * 12 + 6).tolist()
+        s = float(torch.rand(1, generator=g) * 2 + 1.5)
+        img += torch.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * s * s))
+    R = rotations_2d(8, 124).reshape(2, 6, 2, 2)
+    out["proj_image"] = img.numpy()
+    out["proj_rotations"] = R.numpy()
+    out["proj_pad2"] = tfs.project_2d_to_1d(img, R.clone()).numpy()
+    out["proj_pad1"] = tfs.project_2d_to_1d(img, R.clone(), pad_factor=1.0).numpy()
+    save("extract_2d", **out)
+
+    # G13: generic 2-D sampler, real / complex / multichannel, bilinear /
+    # nearest / bicubic, coordinates partly out of bounds, .5 ties, integers.
+    g = torch.Generator().manual_seed(25)
+    img_r = torch.randn(12, 10, generator=g)
+    img_c = torch.complex(torch.randn(12, 10, generator=g), torch.randn(12, 10, generator=g))
+    img_m = torch.randn(3, 12, 10, generator=g)
+    hi = torch.tensor([12.0, 10.0])
+    crd = torch.rand(400, 2, generator=g) * (hi + 1.0) - 1.0
+    ties = torch.randint(0, 9, (100, 2), generator=g).float() + 0.5
+    ints = torch.randint(0, 10, (50, 2), generator=g).float()
+    crd = torch.cat([crd, ties, ints]).reshape(11, 50, 2)
+    out = {}
+    for mode in ("bilinear", "nearest", "bicubic"):
+        out[f"real_{mode}"] = tii.sample_image_2d(img_r, crd, interpolation=mode).numpy()
+        out[f"complex_{mode}"] = tii.sample_image_2d(img_c, crd, interpolation=mode).numpy()
+        out[f"multi_{mode}"] = tii.sample_image_2d(img_m, crd, interpolation=mode).numpy()
+    save("sample_image_2d", image_real=img_r.numpy(), image_complex=img_c.numpy(),
+         image_multi=img_m.numpy(), coordinates=crd.numpy(), **out)
+
+    # G14: generic 2-D splat, bilinear / nearest, two accumulating calls.
+    g = torch.Generator().manual_seed(26)
+    crd1 = torch.cat([torch.rand(300, 2, generator=g) * (hi + 1.0) - 1.0, ties[:50]])
+    crd2 = torch.rand(200, 2, generator=g) * (hi + 1.0) - 1.0
+    val_r1, val_r2 = torch.randn(350, generator=g), torch.randn(200, generator=g)
+    val_c1 = torch.complex(torch.randn(350, generator=g), torch.randn(350, generator=g))
+    val_c2 = torch.complex(torch.randn(200, generator=g), torch.randn(200, generator=g))
+    val_m1, val_m2 = torch.randn(350, 2, generator=g), torch.randn(200, 2, generator=g)
+    out = {}
+    for mode in ("bilinear", "nearest"):
+        for key, v1, v2, img0 in (("real", val_r1, val_r2, torch.zeros(12, 10)),
+                                  ("complex", val_c1, val_c2, torch.zeros(12, 10, dtype=torch.complex64)),
+                                  ("multi", val_m1, val_m2, torch.zeros(2, 12, 10))):
+            im, w = tii.insert_into_image_2d(v1, crd1, img0, interpolation=mode)
+            im, w = tii.insert_into_image_2d(v2, crd2, im, w, interpolation=mode)
+            out[f"{key}_{mode}_image"], out[f"{key}_{mode}_weights"] = im.numpy(), w.numpy()
+    save("insert_into_image_2d", coords1=crd1.numpy(), coords2=crd2.numpy(),
+         val_r1=val_r1.numpy(), val_r2=val_r2.numpy(), val_c1=val_c1.numpy(),
+         val_c2=val_c2.numpy(), val_m1=val_m1.numpy(), val_m2=val_m2.numpy(), **out)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    root = args[0] if args else "/root/reference"
+    if "--2d" in sys.argv:  # only the fixtures of the 2-D analogues
+        main_2d(root)
+    else:
+        main(root)
+        main_2d(root)
