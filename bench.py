@@ -13,6 +13,11 @@ roofline and the FP64 oracle timed on the host CPU on a bounded sample.  The
 plain-torch comparator on the same GPU is timed with --torch.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu] [--copy-probe MIB]
+                      [--workload slices3d|lines2d]
+
+``--workload lines2d`` measures the 2-D analogue instead (central lines of a
+(1024, 513) complex64 2-D rfft for 65536 rotations, forward + image
+gradient) and prints its own JSON line; the default is the headline above.
 """
 
 from __future__ import annotations
@@ -46,6 +51,8 @@ def parse():
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--gather", choices=["auto", "pixel", "brick"], default="auto",
                     help="extraction gather kernel (default: auto)")
+    ap.add_argument("--workload", choices=["slices3d", "lines2d"], default="slices3d",
+                    help="slices3d: the headline (default); lines2d: the 2-D central-line analogue")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
@@ -213,6 +220,96 @@ def cpu_baseline(V, R, gS, min_s=10.0, chunk=256):
                       f"{N_ROT} rotations at {N}^3 in {dt:.1f} s"}
 
 
+L2D_N = 1024      # 2-D image side: rfft (1024, 513)
+L2D_ROT = 65536   # rotations (lines) per step
+
+
+def run_lines2d(args, device):
+    """Central lines of a 2-D rfft, forward + image gradient, as
+    ``extract_central_slices_rfft_2d(...).backward(grad)`` runs them: the line
+    gather, then a zero fill of the image gradient and the adjoint splat."""
+    from teamtomo_amd import _lib
+
+    n, wr = L2D_N, L2D_N // 2 + 1
+    g = torch.Generator(device=device).manual_seed(21)
+    img = torch.randn((n, n), generator=g, device=device)
+    F = torch.fft.fftshift(torch.fft.rfftn(torch.fft.fftshift(img, dim=(-2, -1)), dim=(-2, -1)), dim=(-2,))
+    F = F.to(torch.complex64).contiguous()
+    th = torch.rand(L2D_ROT, generator=g, device=device, dtype=torch.float64) * 2 * np.pi
+    R = torch.stack([torch.cos(th), -torch.sin(th), torch.sin(th), torch.cos(th)], -1).reshape(-1, 2, 2)
+    Ryx = torch.flip(R.to(torch.float32), dims=(-2, -1)).contiguous()
+    gL = torch.complex(torch.randn((L2D_ROT, wr), generator=g, device=device),
+                       torch.randn((L2D_ROT, wr), generator=g, device=device)).contiguous()
+    out = torch.empty((L2D_ROT, wr), dtype=torch.complex64, device=device)
+    gI = torch.empty_like(F)
+    st = _lib.stream_of(F)
+    events = []
+
+    def step(timed=False):
+        s = torch.cuda.current_stream()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)] if timed else None
+        if timed:
+            e[0].record(s)
+        _lib.call("tt_extract2d_c64", _lib.ptr(F), n, wr, n, n, _lib.ptr(Ryx), L2D_ROT, -1.0, _lib.ptr(out), st)
+        if timed:
+            e[1].record(s)
+        gI.zero_()
+        _lib.call("tt_extract2d_bwd_c64", _lib.ptr(gL), _lib.ptr(F), n, wr, n, n, _lib.ptr(Ryx), L2D_ROT, -1.0,
+                  _lib.ptr(gI), None, st)
+        if timed:
+            e[2].record(s)
+            events.append(e)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        step(timed=True)
+    stop.record()
+    torch.cuda.synchronize()
+    ms_step = start.elapsed_time(stop) / args.steps
+    fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    adj_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    # algorithmic bytes: 8 B per line pixel + 16 B per rotation, plus the image read
+    # (forward) or written (adjoint) once
+    fwd_b = L2D_ROT * (8 * wr + 16) + 8 * n * wr
+    adj_b = L2D_ROT * (8 * wr + 16) + 8 * n * wr
+    kernels = {"forward": {"ms": fwd_ms, "bytes": fwd_b, "GBs": fwd_b / fwd_ms / 1e6,
+                           "launch": "tt_extract2d_c64: line_fwd_kernel"},
+               "adjoint": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6,
+                           "launch": "image-gradient zero fill + tt_extract2d_bwd_c64: line_bwd_kernel"}}
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    line = {
+        "metric": "lines/s fwd+adjoint of 2-D central lines at 1024^2 c64; % HBM roofline",
+        "value": L2D_ROT / (ms_step / 1e3), "unit": "lines/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64",
+        "data": "synthetic (seeded N(0,1) 1024^2 image, uniform in-plane angles, N(0,1) c64 gradients)",
+        "config": {"workload": "extract_central_slices_rfft_2d fwd+adjoint, rfft (1024, 513) c64, 65536 rotations",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kernels[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        from oracle import slice_2d as O2
+
+        Fn = F.cpu().numpy().astype(np.complex128)
+        Rn = Ryx[:512].cpu().numpy()
+        gn = gL[:512].cpu().numpy()
+        O2.extract_central_lines(Fn, (n, n), Rn[:2])
+        t0 = time.perf_counter()
+        O2.extract_central_lines(Fn, (n, n), Rn)
+        O2.extract_central_lines_adjoint(gn, Fn.shape, (n, n), Rn)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 512 / dt, "unit": "lines/s", "cores": 1, "kind": "port",
+                                "sample": f"FP64 NumPy oracle (oracle/slice_2d.py), forward + image adjoint of "
+                                          f"512 of the {L2D_ROT} lines in {dt:.2f} s"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -229,6 +326,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_each_way": 4 * n, "copies": 5}), flush=True)
+        return
+    if args.workload == "lines2d":
+        run_lines2d(args, device)
         return
     from teamtomo_amd import _lib, set_gather_path
 

@@ -10,13 +10,18 @@
 //   and sampled bilinearly with zero outside the image array
 //   (torch-image-interpolation .../image_interpolation_2d.py:11-101).
 //
-// Lines are short (W//2+1 pixels) and read a small image that stays in L2, so
-// one workgroup takes one rotation; no LDS staging is needed.  Coordinates are
+// Lines are short (W//2+1 pixels) and read a small image that stays in L2:
+// one lane per line pixel over the flat (rotation, pixel) range, no LDS
+// staging.  Coordinates are
 // evaluated in FP64 and snapped within 1e-9 of an integer, as in the 3-D path.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "../../include/teamtomo_amd.h"
 #include "tt_common.h"
@@ -32,6 +37,7 @@ struct LineGeom {
   int wr;             // line length W//2+1 of the image shape
   double ay, by, ax;  // array units per cycle/pixel (y full, x half), y of f = 0
   double inv_W;       // 1/W of the image shape
+  double inv_wr;      // 1/wr, for the flat index split
   int use_mask;       // fftfreq_max given
   float fmax, inv_W32;
 };
@@ -45,6 +51,7 @@ LineGeom make_line_geom(int h, int wr_arr, int img_h, int img_w, double fftfreq_
   g.by = 0.5 * g.ay;
   g.ax = 2.0 * (double)(g.wr - 1);
   g.inv_W = 1.0 / (double)img_w;
+  g.inv_wr = 1.0 / (double)g.wr;
   g.use_mask = fftfreq_max >= 0.0;
   g.fmax = (float)fftfreq_max;
   g.inv_W32 = (float)(1.0 / (double)img_w);
@@ -90,13 +97,45 @@ __device__ __forceinline__ void line_rot(const LineGeom &g, const float *__restr
   Vx = __dmul_rn(__dmul_rn(g.ax, (double)R[3]), g.inv_W);
 }
 
+// Flat (rotation, pixel) index -> rotation, with an FP64 reciprocal and a
+// one-step correction (n_rot * wr < 2^53).
+__device__ __forceinline__ void split_line(const LineGeom &g, int64_t p, int64_t &r, int &j) {
+  r = (int64_t)((double)p * g.inv_wr);
+  int64_t jj = p - r * g.wr;
+  if (jj < 0) {
+    r -= 1;
+    jj += g.wr;
+  } else if (jj >= g.wr) {
+    r += 1;
+    jj -= g.wr;
+  }
+  j = (int)jj;
+}
+
+// Bilinear read of the four corners (weight-0 corners past N-1 read as 0).
+__device__ __forceinline__ void line_corners(const f2v *__restrict__ img, const LineGeom &g, int y0, int x0,
+                                             bool yu, bool xu, f2v (&v)[4]) {
+  const f2v *b = img + (int64_t)y0 * g.WR + x0;
+  const f2v zero = {0.0f, 0.0f};
+  v[0] = b[0];
+  v[1] = xu ? b[1] : zero;
+  v[2] = yu ? b[g.WR] : zero;
+  v[3] = (yu && xu) ? b[g.WR + 1] : zero;
+}
+
+// One lane per line pixel over the flat (rotation, pixel) range; consecutive
+// lanes take consecutive pixels of a line, so the corner reads of a wave
+// fall on a few image rows that stay in L2.
 __global__ __launch_bounds__(kLineBlock) void line_fwd_kernel(const f2v *__restrict__ img, LineGeom g,
-                                                             const float *__restrict__ rot,
+                                                             const float *__restrict__ rot, int64_t total,
                                                              f2v *__restrict__ out) {
-  const int64_t r = blockIdx.x;
-  double Vy, Vx;
-  line_rot(g, rot + r * 4, Vy, Vx);
-  for (int j = threadIdx.x; j < g.wr; j += kLineBlock) {
+  for (int64_t p = (int64_t)blockIdx.x * kLineBlock + threadIdx.x; p < total;
+       p += (int64_t)gridDim.x * kLineBlock) {
+    int64_t r;
+    int j;
+    split_line(g, p, r, j);
+    double Vy, Vx;
+    line_rot(g, rot + r * 4, Vy, Vx);
     const LineCoord c = line_coord(g, Vy, Vx, j);
     f2v val = {0.0f, 0.0f};
     if (line_ok(g, c, j)) {
@@ -105,103 +144,400 @@ __global__ __launch_bounds__(kLineBlock) void line_fwd_kernel(const f2v *__restr
       bool yu, xu;
       line_cell(c.y, g.H, y0, ty, yu);
       line_cell(c.x, g.WR, x0, tx, xu);
-      const f2v *b = img + (int64_t)y0 * g.WR + x0;
-      const f2v zero = {0.0f, 0.0f};
-      const f2v v00 = b[0], v01 = xu ? b[1] : zero;
-      const f2v v10 = yu ? b[g.WR] : zero, v11 = (yu && xu) ? b[g.WR + 1] : zero;
+      f2v v[4];
+      line_corners(img, g, y0, x0, yu, xu, v);
       const float ux = 1.0f - tx, uy = 1.0f - ty;
-      const float r0 = lerp1(v00.x, v01.x, ux, tx), i0 = lerp1(v00.y, v01.y, ux, tx);
-      const float r1 = lerp1(v10.x, v11.x, ux, tx), i1 = lerp1(v10.y, v11.y, ux, tx);
+      const float r0 = lerp1(v[0].x, v[1].x, ux, tx), i0 = lerp1(v[0].y, v[1].y, ux, tx);
+      const float r1 = lerp1(v[2].x, v[3].x, ux, tx), i1 = lerp1(v[2].y, v[3].y, ux, tx);
       val.x = lerp1(r0, r1, uy, ty);
       val.y = lerp1(i0, i1, uy, ty);
       if (c.conj) val.y = -val.y;
     }
-    out[r * g.wr + j] = val;
+    out[p] = val;
   }
 }
 
 // Backward.  GI: grad_image += the adjoint splat of kappa(grad_line) (kappa =
-// conj where the pixel was read conjugated).  GR: grad_rot[r] (yx, row-major)
-// += dL/dR through the coordinates: dL/dc = Re(conj(kappa(g)) dS/dc), zero
-// where c <= 0 or c >= N-1 (grid_sample's border-clip gradient);
-// dc_y/dR[0][1] = a_y s f and dc_x/dR[1][1] = a_x s f.
+// conj where the pixel was read conjugated); corners of weight 0 are skipped,
+// which keeps the atomics off the DC pixel that every line starts on.
+// GR: grad_rot[r] (yx, row-major) += dL/dR through the coordinates:
+// dL/dc = Re(conj(kappa(g)) dS/dc), zero where c <= 0 or c >= N-1
+// (grid_sample's border-clip gradient); dc_y/dR[0][1] = a_y s f and
+// dc_x/dR[1][1] = a_x s f.  Lanes of a wave that share a rotation are summed
+// first, then one FP64 atomic per wave and entry.
 template <bool GI, bool GR>
 __global__ __launch_bounds__(kLineBlock) void line_bwd_kernel(const f2v *__restrict__ gl,
                                                              const f2v *__restrict__ img, LineGeom g,
-                                                             const float *__restrict__ rot,
+                                                             const float *__restrict__ rot, int64_t total,
                                                              f2v *__restrict__ gimg,
                                                              double *__restrict__ grot) {
-  const int64_t r = blockIdx.x;
-  double Vy, Vx;
-  line_rot(g, rot + r * 4, Vy, Vx);
-  double acc_y = 0.0, acc_x = 0.0;
-  for (int j = threadIdx.x; j < g.wr; j += kLineBlock) {
-    const LineCoord c = line_coord(g, Vy, Vx, j);
-    if (!line_ok(g, c, j)) continue;
-    int y0, x0;
-    float ty, tx;
-    bool yu, xu;
-    line_cell(c.y, g.H, y0, ty, yu);
-    line_cell(c.x, g.WR, x0, tx, xu);
-    f2v gs = gl[r * g.wr + j];
-    if (c.conj) gs.y = -gs.y;
-    const float ux = 1.0f - tx, uy = 1.0f - ty;
-    const int64_t e = (int64_t)y0 * g.WR + x0;
-    if (GI) {
-      float *gf = reinterpret_cast<float *>(gimg);
-      const float w[4] = {uy * ux, uy * tx, ty * ux, ty * tx};
-      const bool ok[4] = {true, xu, yu, yu && xu};
-      const int64_t off[4] = {0, 1, g.WR, g.WR + 1};
+  for (int64_t p0 = (int64_t)blockIdx.x * kLineBlock; p0 < total; p0 += (int64_t)gridDim.x * kLineBlock) {
+    const int64_t p = p0 + threadIdx.x;
+    double acc_y = 0.0, acc_x = 0.0;
+    int64_t r = 0;
+    if (p < total) {
+      int j;
+      split_line(g, p, r, j);
+      double Vy, Vx;
+      line_rot(g, rot + r * 4, Vy, Vx);
+      const LineCoord c = line_coord(g, Vy, Vx, j);
+      if (line_ok(g, c, j)) {
+        int y0, x0;
+        float ty, tx;
+        bool yu, xu;
+        line_cell(c.y, g.H, y0, ty, yu);
+        line_cell(c.x, g.WR, x0, tx, xu);
+        f2v gs = gl[p];
+        if (c.conj) gs.y = -gs.y;
+        const float ux = 1.0f - tx, uy = 1.0f - ty;
+        const int64_t e = (int64_t)y0 * g.WR + x0;
+        if (GI) {
+          float *gf = reinterpret_cast<float *>(gimg);
+          const float w[4] = {uy * ux, uy * tx, ty * ux, ty * tx};
+          const bool ok[4] = {true, xu, yu, yu && xu};
+          const int64_t off[4] = {0, 1, g.WR, g.WR + 1};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (!ok[q]) continue;
-        unsafeAtomicAdd(gf + 2 * (e + off[q]), w[q] * gs.x);
-        unsafeAtomicAdd(gf + 2 * (e + off[q]) + 1, w[q] * gs.y);
+          for (int q = 0; q < 4; ++q) {
+            if (!ok[q] || w[q] == 0.0f) continue;
+            unsafeAtomicAdd(gf + 2 * (e + off[q]), w[q] * gs.x);
+            unsafeAtomicAdd(gf + 2 * (e + off[q]) + 1, w[q] * gs.y);
+          }
+        }
+        if (GR) {
+          f2v v[4];
+          line_corners(img, g, y0, x0, yu, xu, v);
+          const float dyr = ux * (v[2].x - v[0].x) + tx * (v[3].x - v[1].x);
+          const float dyi = ux * (v[2].y - v[0].y) + tx * (v[3].y - v[1].y);
+          const float dxr = uy * (v[1].x - v[0].x) + ty * (v[3].x - v[2].x);
+          const float dxi = uy * (v[1].y - v[0].y) + ty * (v[3].y - v[2].y);
+          float gy = gs.x * dyr + gs.y * dyi;
+          float gx = gs.x * dxr + gs.y * dxi;
+          if (!(c.y > 0.0 && c.y < (double)(g.H - 1))) gy = 0.0f;
+          if (!(c.x > 0.0 && c.x < (double)(g.WR - 1))) gx = 0.0f;
+          const double sf = (c.conj ? -1.0 : 1.0) * (double)j * g.inv_W;
+          acc_y = (double)gy * g.ay * sf;
+          acc_x = (double)gx * g.ax * sf;
+        }
       }
     }
     if (GR) {
-      const f2v *b = img + e;
-      const f2v zero = {0.0f, 0.0f};
-      const f2v v00 = b[0], v01 = xu ? b[1] : zero;
-      const f2v v10 = yu ? b[g.WR] : zero, v11 = (yu && xu) ? b[g.WR + 1] : zero;
-      const float dyr = ux * (v10.x - v00.x) + tx * (v11.x - v01.x);
-      const float dyi = ux * (v10.y - v00.y) + tx * (v11.y - v01.y);
-      const float dxr = uy * (v01.x - v00.x) + ty * (v11.x - v10.x);
-      const float dxi = uy * (v01.y - v00.y) + ty * (v11.y - v10.y);
-      float gy = gs.x * dyr + gs.y * dyi;
-      float gx = gs.x * dxr + gs.y * dxi;
-      if (!(c.y > 0.0 && c.y < (double)(g.H - 1))) gy = 0.0f;
-      if (!(c.x > 0.0 && c.x < (double)(g.WR - 1))) gx = 0.0f;
-      const double sf = (c.conj ? -1.0 : 1.0) * (double)j * g.inv_W;
-      acc_y += (double)gy * g.ay * sf;
-      acc_x += (double)gx * g.ax * sf;
+      // the wave's lanes cover at most a few rotations: sum each run of
+      // lanes with the same rotation (a segmented scan by shuffles), then the
+      // last lane of every run adds its sum
+      const int lane = threadIdx.x & 63;
+      const int64_t rn = __shfl_down(r, 1, 64);
+      for (int off = 1; off < 64; off <<= 1) {
+        const double uy = __shfl_up(acc_y, off, 64), ux = __shfl_up(acc_x, off, 64);
+        const int64_t ru = __shfl_up(r, off, 64);
+        if (lane >= off && ru == r) {
+          acc_y += uy;
+          acc_x += ux;
+        }
+      }
+      const bool last = lane == 63 || rn != r || p + 1 >= total;
+      if (p < total && last && (acc_y != 0.0 || acc_x != 0.0)) {
+        unsafeAtomicAdd(grot + r * 4 + 1, acc_y);
+        unsafeAtomicAdd(grot + r * 4 + 3, acc_x);
+      }
     }
   }
-  if (GR) {
-    __shared__ double red[kLineBlock / 64][2];
-    for (int off = 32; off > 0; off >>= 1) {
-      acc_y += __shfl_down(acc_y, off, 64);
-      acc_x += __shfl_down(acc_x, off, 64);
+}
+
+// ---------------------------------------------------------------------------
+// Image gradient by tiles: the adjoint splat of the lines, accumulated per
+// image tile in LDS.  Global float atomics from every XCD onto the pixels
+// around DC (which every line crosses) serialise outside L2, so each unit (an
+// image tile and a range of rotations) keeps its tile in LDS: for every
+// rotation it intersects the ray with the tile's coordinate box, walks the
+// pixels whose cell lies in the tile (the same line_coord / line_cell
+// decision as the forward kernel) and adds their bilinear contributions with
+// 64-bit LDS atomics in fixed point (scale 2^(30-e) from the exact max|g|;
+// 64-bit integer LDS atomics run far faster than float ones); the flush then
+// adds the tile to the image, with global atomics only on the rows and
+// columns a neighbouring tile also touches (and for tiles split over several
+// units).
+// ---------------------------------------------------------------------------
+constexpr int kTY = 32, kTX = 32;  // cells per tile
+constexpr int kTP = (kTY + 1) * (kTX + 1);
+
+struct LineUnit {
+  int tile;    // ty * ntx + tx
+  int r0, r1;  // rotation range
+  int split;   // 1 if the tile is shared by several units
+};
+
+typedef unsigned long long u64;
+
+// max(|re|, |im|) over n complex values, as float bits, into *res (zeroed
+// beforehand); one atomic per block.
+__global__ __launch_bounds__(kLineBlock) void line_maxabs_kernel(const f2v *__restrict__ x, int64_t n,
+                                                                unsigned *__restrict__ res) {
+  __shared__ float s_m[kLineBlock / 64];
+  float m = 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * kLineBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLineBlock) {
+    const f2v v = x[i];
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = s_m[0];
+    for (int w = 1; w < kLineBlock / 64; ++w) b = fmaxf(b, s_m[w]);
+    atomicMax(res, __float_as_uint(b));
+  }
+}
+
+__global__ void line_zero_word_kernel(unsigned *w) {
+  if (threadIdx.x == 0) *w = 0u;
+}
+
+// Fixed-point scale 2^(30 - e), 2^e > max|g|: every contribution w g (|w| <= 1)
+// converts to a 32-bit integer, and 2^33 of them fit the 64-bit sums.
+__device__ __forceinline__ float line_scale(const unsigned *maxbits) {
+  const float m = __uint_as_float(*maxbits);
+  int e = 0;
+  if (m > 0.0f && m < 3.0e38f) frexpf(m, &e);
+  return ldexpf(1.0f, 30 - e);
+}
+
+__device__ __forceinline__ int line_cvt_floor(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));  // floor and convert in one VALU op
+  return r;
+}
+
+__global__ __launch_bounds__(kLineBlock) void line_adj_tile_kernel(const f2v *__restrict__ gl, LineGeom g,
+                                                                  const float *__restrict__ rot,
+                                                                  const LineUnit *__restrict__ units, int ntx,
+                                                                  const unsigned *__restrict__ maxbits,
+                                                                  f2v *__restrict__ gimg) {
+  constexpr int NW = kLineBlock / 64;
+  __shared__ u64 acc[2][kTP];
+  // per wave: inclusive prefix of the candidate counts, first column, rotation
+  __shared__ int s_end[NW][64], s_j0[NW][64], s_r[NW][64];
+  const LineUnit u = units[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int ty = u.tile / ntx, tx = u.tile - ty * ntx;
+  const int y_lo = ty * kTY, x_lo = tx * kTX;
+  const int nyc = min(kTY, g.H - 1 - y_lo), nxc = min(kTX, g.WR - 1 - x_lo);  // cells in the tile
+  for (int v = tid; v < kTP; v += kLineBlock) acc[0][v] = acc[1][v] = 0ull;
+  const float vscale = line_scale(maxbits);
+  __syncthreads();
+  // the tile's coordinate box relative to DC in FP32, widened by 1e-3 (far
+  // beyond the FP32 rounding of the box and of the ray, |error| < 2e-4); the
+  // candidate columns are widened by 0.01 more, and the exact cell test below
+  // decides ownership
+  const float yb0 = (float)((double)y_lo - g.by) - 1e-3f, yb1 = (float)((double)(y_lo + nyc) - g.by) + 1e-3f;
+  const float xb0 = (float)x_lo - 1e-3f, xb1 = (float)(x_lo + nxc) + 1e-3f;
+  const float jmax = (float)(g.wr - 1);
+  int *end = s_end[wid], *j0s = s_j0[wid], *rs = s_r[wid];
+  // each wave walks its own share of the rotations, skipping batches of 64
+  // rotations whose rays all miss the tile without any block barrier
+  for (int rb = u.r0 + wid * 64; rb < u.r1; rb += kLineBlock) {
+    const int r = rb + lane;
+    int cnt = 0, j0 = 0;
+    if (r < u.r1) {
+      const float R01 = rot[(int64_t)r * 4 + 1], R11 = rot[(int64_t)r * 4 + 3];
+      const float vy = (float)(g.ay * g.inv_W) * R01, vx = (float)(g.ax * g.inv_W) * R11;
+      // the ray c(j) = (by + s j vy, j |vx|), s = -1 iff vx < 0
+      const float sy = vx < 0.0f ? -vy : vy, ax = fabsf(vx);
+      float lo = 0.0f, hi = jmax;
+      if (fabsf(sy) > 1e-20f) {
+        const float rs_ = 1.0f / sy;
+        const float t0 = yb0 * rs_, t1 = yb1 * rs_;
+        lo = fmaxf(lo, fminf(t0, t1));
+        hi = fminf(hi, fmaxf(t0, t1));
+      } else if (yb0 > 0.0f || yb1 < 0.0f) {  // c_y = by for every j
+        hi = -1.0f;
+      }
+      if (ax > 1e-20f) {
+        const float ra = 1.0f / ax;
+        lo = fmaxf(lo, xb0 * ra);
+        hi = fminf(hi, xb1 * ra);
+      } else if (xb0 > 0.0f) {  // c_x = 0 for every j
+        hi = -1.0f;
+      }
+      if (lo <= hi + 0.02f) {
+        j0 = max(0, (int)ceilf(lo - 0.01f));
+        const int j1 = min(g.wr - 1, (int)floorf(hi + 0.01f));
+        cnt = max(0, j1 - j0 + 1);
+      }
     }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-      red[wid][0] = acc_y;
-      red[wid][1] = acc_x;
+    int v = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(v, off, 64);
+      if (lane >= off) v += t;
     }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-      double s = 0.0;
-      for (int w = 0; w < kLineBlock / 64; ++w) s += red[w][threadIdx.x];
-      // one workgroup per rotation: a plain read-modify-write accumulates
-      grot[r * 4 + (threadIdx.x ? 3 : 1)] += s;
+    const int total = __shfl(v, 63, 64);
+    if (total == 0) continue;  // wave-uniform
+    end[lane] = v;
+    j0s[lane] = j0;
+    rs[lane] = r;
+    __builtin_amdgcn_wave_barrier();
+    for (int q = lane; q < total; q += 64) {
+      int k = 0;  // the candidate's rotation: first k with end[k] > q
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (end[k + step - 1] <= q) k += step;
+      const int j = j0s[k] + (q - (k ? end[k - 1] : 0));
+      const int rr = rs[k];
+      double Vy, Vx;
+      line_rot(g, rot + (int64_t)rr * 4, Vy, Vx);
+      const LineCoord c = line_coord(g, Vy, Vx, j);
+      if (!line_ok(g, c, j)) continue;
+      int y0, x0;
+      float fy, fx;
+      bool yu, xu;
+      line_cell(c.y, g.H, y0, fy, yu);
+      line_cell(c.x, g.WR, x0, fx, xu);
+      const int ly = y0 - y_lo, lx = x0 - x_lo;
+      if (ly < 0 || ly >= nyc || lx < 0 || lx >= nxc) continue;  // another tile's cell
+      f2v gs = gl[(int64_t)rr * g.wr + j];
+      if (c.conj) gs.y = -gs.y;
+      gs.x *= vscale;
+      gs.y *= vscale;
+      const float ux = 1.0f - fx, uy = 1.0f - fy;
+      const float w[4] = {uy * ux, uy * fx, fy * ux, fy * fx};
+      const int o[4] = {0, 1, kTX + 1, kTX + 2};
+      const int e = ly * (kTX + 1) + lx;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (w[t] == 0.0f) continue;
+        // round(w g) (half up) as a 64-bit two's-complement fixed-point value
+        atomicAdd(&acc[0][e + o[t]], (u64)(long long)line_cvt_floor(__fmaf_rn(w[t], gs.x, 0.5f)));
+        atomicAdd(&acc[1][e + o[t]], (u64)(long long)line_cvt_floor(__fmaf_rn(w[t], gs.y, 0.5f)));
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // the next batch rewrites this wave's arrays
+  }
+  __syncthreads();
+  // flush: exclusive pixels by a plain read-modify-write, shared ones
+  // (the tile's first row / column, its far halo row / column, or all of a
+  // split tile) by atomics
+  float *gf = reinterpret_cast<float *>(gimg);
+  const float inv = 1.0f / vscale;
+  for (int v = tid; v < kTP; v += kLineBlock) {
+    const int ly = v / (kTX + 1), lx = v - ly * (kTX + 1);
+    if (ly > nyc || lx > nxc) continue;
+    const u64 qr = acc[0][v], qi = acc[1][v];
+    if (qr == 0ull && qi == 0ull) continue;
+    const float ar = (float)(long long)qr * inv, ai = (float)(long long)qi * inv;
+    const int64_t e = (int64_t)(y_lo + ly) * g.WR + (x_lo + lx);
+    const bool shared = u.split || ly == 0 || lx == 0 || ly == nyc || lx == nxc;
+    if (shared) {
+      unsafeAtomicAdd(gf + 2 * e, ar);
+      unsafeAtomicAdd(gf + 2 * e + 1, ai);
+    } else {
+      f2v cur = gimg[e];
+      cur.x += ar;
+      cur.y += ai;
+      gimg[e] = cur;
     }
   }
+}
+
+// Units of the tiles: a tile's work is estimated for uniformly spread angles
+// (the fraction of rays through DC that cross the tile is the angle it
+// subtends over pi) and heavy tiles are split into rotation ranges.
+struct LinePlanKey {
+  int dev, H, WR, wr, nrot;
+  bool operator<(const LinePlanKey &o) const {
+    return std::tie(dev, H, WR, wr, nrot) < std::tie(o.dev, o.H, o.WR, o.wr, o.nrot);
+  }
+};
+std::mutex g_line_mutex;
+std::map<LinePlanKey, std::pair<LineUnit *, int>> g_line_plans;
+
+int line_plan(const LineGeom &g, int64_t n_rot, LineUnit **units, int *n_units, int *ntx_out) {
+  int dev = 0;
+  if (hipError_t e = hipGetDevice(&dev)) {
+    set_error(std::string("line plan: hipGetDevice: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  const int nty = (g.H - 1 + kTY - 1) / kTY, ntx = (g.WR - 1 + kTX - 1) / kTX;
+  *ntx_out = ntx;
+  const LinePlanKey key{dev, g.H, g.WR, g.wr, (int)n_rot};
+  std::lock_guard<std::mutex> lock(g_line_mutex);
+  auto it = g_line_plans.find(key);
+  if (it != g_line_plans.end()) {
+    *units = it->second.first;
+    *n_units = it->second.second;
+    return TT_OK;
+  }
+  const double kPi = 3.14159265358979323846;
+  std::vector<double> est(nty * ntx);
+  double total = 0.0;
+  for (int ty = 0; ty < nty; ++ty)
+    for (int tx = 0; tx < ntx; ++tx) {
+      // the tile's box relative to DC (by, 0); x >= 0 throughout
+      const double y0 = ty * kTY - g.by, y1 = std::min((ty + 1) * kTY, g.H - 1) - g.by;
+      const double x0 = tx * kTX, x1 = std::min((tx + 1) * kTX, g.WR - 1);
+      double frac = 1.0;
+      const double dist = std::hypot(x0, (y0 <= 0.0 && y1 >= 0.0) ? 0.0 : std::min(std::fabs(y0), std::fabs(y1)));
+      if (dist > 2.0) {
+        double amin = 1e9, amax = -1e9;
+        const double ys[2] = {y0, y1}, xs[2] = {x0, x1};
+        for (double yy : ys)
+          for (double xx : xs) {
+            const double a = std::atan2(xx, yy);  // in [0, pi]
+            amin = std::min(amin, a);
+            amax = std::max(amax, a);
+          }
+        // subtended angle plus the rays' one-pixel bilinear reach
+        frac = std::min(1.0, (amax - amin + 2.0 / dist) / kPi);
+      }
+      // per rotation a range test; per crossing ray about a tile width of pixels
+      est[ty * ntx + tx] = (double)n_rot * (0.05 + frac * kTX);
+      total += est[ty * ntx + tx];
+    }
+  // about 8 units per CU of 256, none below a few range tests per rotation
+  const double target = std::max(total / (256.0 * 8.0), (double)n_rot * 0.2);
+  struct U {
+    LineUnit u;
+    double w;
+  };
+  std::vector<U> hu;
+  for (int t = 0; t < nty * ntx; ++t) {
+    const int nseg = (int)std::max<int64_t>(
+        1, std::min<int64_t>(n_rot / 256 + 1, (int64_t)std::ceil(est[t] / target)));
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+      LineUnit u;
+      u.tile = t;
+      u.r0 = (int)(n_rot * sgi / nseg);
+      u.r1 = (int)(n_rot * (sgi + 1) / nseg);
+      u.split = nseg > 1;
+      if (u.r1 > u.r0) hu.push_back({u, est[t] / nseg});
+    }
+  }
+  // heaviest units first
+  std::stable_sort(hu.begin(), hu.end(), [](const U &a, const U &b) { return a.w > b.w; });
+  std::vector<LineUnit> h(hu.size());
+  for (size_t i = 0; i < hu.size(); ++i) h[i] = hu[i].u;
+  LineUnit *d = nullptr;
+  if (hipError_t e = hipMalloc(&d, h.size() * sizeof(LineUnit))) {
+    set_error(std::string("line plan: hipMalloc: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  if (hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(LineUnit), hipMemcpyHostToDevice)) {
+    set_error(std::string("line plan: hipMemcpy: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  g_line_plans[key] = {d, (int)h.size()};
+  *units = d;
+  *n_units = (int)h.size();
+  return TT_OK;
+}
+
+int line_grid(int64_t total) {
+  const int64_t b = (total + kLineBlock - 1) / kLineBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 16));
 }
 
 int check_line_args(int h, int wr, int img_h, int img_w, int64_t n_rot, const float *rot) {
   if (h < 1 || wr < 1) return fail_invalid("image_rfft dimensions must be >= 1");
   if (img_h < 2 || img_w < 1) return fail_invalid("image_shape must be (h >= 2, w >= 1)");
-  if (n_rot < 0 || n_rot > 0x7fffffffLL) return fail_invalid("rotation count out of range");
+  if (n_rot < 0 || n_rot > ((int64_t)1 << 40)) return fail_invalid("rotation count out of range");
   if (n_rot > 0 && !rot) return fail_invalid("rotation pointer is NULL");
   return TT_OK;
 }
@@ -221,8 +557,9 @@ int tt_extract2d_c64(const void *image, int h, int wr, int img_h, int img_w, con
   if (n_rot == 0) return TT_OK;
   if (!image || !lines) return fail_invalid("image / lines pointer is NULL");
   const LineGeom g = make_line_geom(h, wr, img_h, img_w, fftfreq_max);
-  hipLaunchKernelGGL(line_fwd_kernel, dim3((unsigned)n_rot), dim3(kLineBlock), 0, (hipStream_t)stream,
-                     (const f2v *)image, g, rot_yx, (f2v *)lines);
+  const int64_t total = n_rot * (int64_t)g.wr;
+  hipLaunchKernelGGL(line_fwd_kernel, dim3(line_grid(total)), dim3(kLineBlock), 0, (hipStream_t)stream,
+                     (const f2v *)image, g, rot_yx, total, (f2v *)lines);
   return check_launch("line_fwd_kernel");
 }
 
@@ -236,16 +573,43 @@ int tt_extract2d_bwd_c64(const void *grad_lines, const void *image, int h, int w
   if (grad_rot && !image) return fail_invalid("the image is required for the rotation gradient");
   const LineGeom g = make_line_geom(h, wr, img_h, img_w, fftfreq_max);
   hipStream_t s = (hipStream_t)stream;
-  const dim3 grid((unsigned)n_rot), block(kLineBlock);
-  if (grad_image && grad_rot)
+  const int64_t total = n_rot * (int64_t)g.wr;
+  const dim3 grid(line_grid(total)), block(kLineBlock);
+  // the image gradient by tiles (images of at least 2 x 2 samples and rotation
+  // counts the plan indexes with int), else by global atomics
+  const bool tiles = grad_image && h >= 2 && wr >= 2 && n_rot < ((int64_t)1 << 30);
+  if (tiles) {
+    LineUnit *units = nullptr;
+    int n_units = 0, ntx = 0;
+    if ((st = line_plan(g, n_rot, &units, &n_units, &ntx))) return st;
+    unsigned *word = nullptr;
+    if (hipError_t e = hipMallocAsync((void **)&word, 256, s)) {
+      set_error(std::string("tt_extract2d_bwd_c64: hipMallocAsync: ") + hipGetErrorString(e));
+      return TT_ERR_HIP;
+    }
+    hipLaunchKernelGGL(line_zero_word_kernel, dim3(1), dim3(64), 0, s, word);
+    hipLaunchKernelGGL(line_maxabs_kernel, dim3((unsigned)std::min<int64_t>(2048, (total + kLineBlock - 1) / kLineBlock)),
+                       block, 0, s, (const f2v *)grad_lines, total, word);
+    hipLaunchKernelGGL(line_adj_tile_kernel, dim3((unsigned)n_units), block, 0, s, (const f2v *)grad_lines,
+                       g, rot_yx, (const LineUnit *)units, ntx, (const unsigned *)word, (f2v *)grad_image);
+    st = check_launch("line_adj_tile_kernel");
+    if (hipError_t e = hipFreeAsync(word, s)) {
+      if (!st) {
+        set_error(std::string("tt_extract2d_bwd_c64: hipFreeAsync: ") + hipGetErrorString(e));
+        st = TT_ERR_HIP;
+      }
+    }
+    if (st) return st;
+  }
+  if (grad_image && grad_rot && !tiles)
     hipLaunchKernelGGL((line_bwd_kernel<true, true>), grid, block, 0, s, (const f2v *)grad_lines,
-                       (const f2v *)image, g, rot_yx, (f2v *)grad_image, grad_rot);
-  else if (grad_image)
+                       (const f2v *)image, g, rot_yx, total, (f2v *)grad_image, grad_rot);
+  else if (grad_image && !tiles)
     hipLaunchKernelGGL((line_bwd_kernel<true, false>), grid, block, 0, s, (const f2v *)grad_lines,
-                       (const f2v *)image, g, rot_yx, (f2v *)grad_image, grad_rot);
-  else
+                       (const f2v *)image, g, rot_yx, total, (f2v *)grad_image, grad_rot);
+  else if (grad_rot)
     hipLaunchKernelGGL((line_bwd_kernel<false, true>), grid, block, 0, s, (const f2v *)grad_lines,
-                       (const f2v *)image, g, rot_yx, (f2v *)grad_image, grad_rot);
+                       (const f2v *)image, g, rot_yx, total, (f2v *)grad_image, grad_rot);
   return check_launch("line_bwd_kernel");
 }
 

@@ -76,6 +76,24 @@ def test_lines_large(device):
     assert rel_l2(L[sub], ref) < EXT_TOL
 
 
+def test_lines_adjoint_large(device):
+    # the tiled image gradient (split tiles near DC, shared tile edges) against
+    # the oracle's adjoint, and the dot-product identity with the forward
+    rng = np.random.default_rng(15)
+    Fn = (rng.standard_normal((512, 257)) + 1j * rng.standard_normal((512, 257))).astype(np.complex64)
+    R = _rots(3000, 16)
+    Gn = (rng.standard_normal((3000, 257)) + 1j * rng.standard_normal((3000, 257))).astype(np.complex64)
+    F = _t(Fn, device).requires_grad_(True)
+    L = tt.extract_central_slices_rfft_2d(F, (512, 512), _t(R, device))
+    torch.real(torch.sum(torch.conj(_t(Gn, device)) * L)).backward()
+    ref = O.extract_central_lines_adjoint(Gn, Fn.shape, (512, 512), O.prepare_rotations_2d(R))
+    print("line adjoint 512^2 x 3000 rel_l2:", rel_l2(_np(F.grad), ref))
+    assert rel_l2(_np(F.grad), ref) < 1e-5
+    lhs = np.real(np.vdot(Gn.astype(np.complex128), _np(L).astype(np.complex128)))
+    rhs = np.real(np.vdot(_np(F.grad).astype(np.complex128), Fn.astype(np.complex128)))
+    assert abs(lhs - rhs) < 1e-4 * abs(lhs)
+
+
 def test_project_2d_to_1d_vs_golden(device):
     G = golden("extract_2d")
     img = _t(G["proj_image"], device)
