@@ -21,6 +21,7 @@
  *   torch-image-interpolation/src/torch_image_interpolation/image_interpolation_3d.py:11,104
  *   torch-fourier-slice/src/torch_fourier_slice/slice_extraction/_extract_central_slices_rfft_2d.py:9
  *   torch-image-interpolation/src/torch_image_interpolation/image_interpolation_2d.py:11,104
+ *   torch-transform-image/src/torch_transform_image/transforms_3d.py:10, transforms_2d.py:11
  */
 #ifndef TEAMTOMO_AMD_H
 #define TEAMTOMO_AMD_H
@@ -220,6 +221,29 @@ int tt_splat2d_f32(const float *values, const float *coords, int64_t n, int chan
 int tt_splat2d_f64(const double *values, const float *coords, int64_t n, int channels,
                    int is_complex, int h, int w, int mode,
                    double *image, void *weights, int weights_f64, void *stream);
+
+/*
+ * Affine warps: replace affine_transform_image_3d / _2d
+ * (torch-transform-image/src/torch_transform_image/transforms_3d.py:10-40,
+ * transforms_2d.py:11-41) with the coordinate grid computed in the kernel.
+ * Output voxel (z, y, x) of leading batch index t samples the image at
+ * M (z, y, x, 1)^T, M the block of matrix ((t md + z') mh + y') mw + x'
+ * (z' = z if md > 1 else 0, ...): the matrices broadcast against the output
+ * grid as the reference's matmul does (md, mh, mw each 1 or d, h, w).
+ *   image   (channels, D, H, W) / (channels, H, W) float32, or complex64
+ *           interleaved if is_complex
+ *   mats    (n_lead * md * mh * mw, 3, 4) / (n_lead * mh * mw, 2, 3) float32
+ *           zyx / yx blocks (row-major)
+ *   out     (n_lead, d, h, w, channels) / (n_lead, h, w, channels)
+ *   mode    TT_INTERP_TRILINEAR (= BILINEAR), TT_INTERP_NEAREST, or (2-D)
+ *           TT_INTERP_BICUBIC
+ */
+int tt_affine3d_f32(const float *image, int channels, int D, int H, int W, int is_complex,
+                    const float *mats, int64_t n_lead, int md, int mh, int mw,
+                    int d, int h, int w, int mode, float *out, void *stream);
+int tt_affine2d_f32(const float *image, int channels, int H, int W, int is_complex,
+                    const float *mats, int64_t n_lead, int mh, int mw, int h, int w, int mode,
+                    float *out, void *stream);
 
 /*
  * Batched extraction or insertion over many volumes of their own box size and

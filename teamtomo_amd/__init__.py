@@ -27,6 +27,12 @@ from .fourier_slice import (
 )
 from .batch import SliceBatch
 from .image_interpolation import insert_into_image_2d, insert_into_image_3d, sample_image_2d, sample_image_3d
+from .transform_image import (
+    affine_transform_image_2d,
+    affine_transform_image_3d,
+    rotate_then_shift_image_2d,
+    shift_then_rotate_image_2d,
+)
 
 __all__ = [
     "extract_central_slices_rfft_3d",
@@ -43,6 +49,10 @@ __all__ = [
     "insert_into_image_3d",
     "sample_image_2d",
     "insert_into_image_2d",
+    "affine_transform_image_3d",
+    "affine_transform_image_2d",
+    "rotate_then_shift_image_2d",
+    "shift_then_rotate_image_2d",
     "set_deterministic",
     "deterministic_enabled",
     "set_pixel_atomic",
@@ -67,8 +77,9 @@ _TFS_NAMES = (
 
 
 def install_dropin() -> None:
-    """Register ``torch_fourier_slice`` and ``torch_image_interpolation`` modules
-    (and their ``slice_extraction`` / ``slice_insertion`` submodules) backed by
+    """Register ``torch_fourier_slice``, ``torch_image_interpolation`` and
+    ``torch_transform_image`` modules (and the ``slice_extraction`` /
+    ``slice_insertion`` submodules) backed by
     this package, so ``from torch_fourier_slice import project_3d_to_2d`` in
     unchanged caller code resolves to the gfx950 implementation."""
     tfs = types.ModuleType("torch_fourier_slice")
@@ -85,7 +96,12 @@ def install_dropin() -> None:
     tii = types.ModuleType("torch_image_interpolation")
     for name in ("sample_image_3d", "insert_into_image_3d", "sample_image_2d", "insert_into_image_2d"):
         setattr(tii, name, globals()[name])
+    tti = types.ModuleType("torch_transform_image")
+    for name in ("affine_transform_image_3d", "affine_transform_image_2d", "rotate_then_shift_image_2d",
+                 "shift_then_rotate_image_2d"):
+        setattr(tti, name, globals()[name])
     sys.modules.update({
+        "torch_transform_image": tti,
         "torch_fourier_slice": tfs,
         "torch_fourier_slice.slice_extraction": ext,
         "torch_fourier_slice.slice_insertion": ins,

@@ -6,6 +6,8 @@
 //         (grid_sample, border, align_corners=True, zero outside [0, N-1]);
 //         (tri/bi)linear corners floor / floor+1; 2-D bicubic = Keys cubic
 //         convolution (A = -0.75) over 4x4 taps clipped to the image.
+// affine: torch-transform-image .../transforms_3d.py:10-40, transforms_2d.py:11-41
+//         (the sampler at M (z, y, x, 1)^T of every output voxel, fused).
 // splat:  image_interpolation_3d.py:104-275, image_interpolation_2d.py:104-275
 //         (index_put_ accumulate); linear corners floor / ceil with float32
 //         weights, nearest = round half to even.
@@ -45,45 +47,80 @@ __device__ __forceinline__ double nearest_index_f(double c, int n) {
   return rint(u);
 }
 
+// One 3-D sample (all channels) at array coordinate (cz, cy, cx) into o.
+template <typename T, bool CPLX>
+__device__ __forceinline__ void sample3d_point(const T *__restrict__ img, int C, int d, int h, int w, T cz,
+                                               T cy, T cx, int mode, T *__restrict__ o) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t plane = (int64_t)h * w, vol = (int64_t)d * plane;
+  const bool ok = cz >= 0 && cz <= (T)(d - 1) && cy >= 0 && cy <= (T)(h - 1) && cx >= 0 && cx <= (T)(w - 1);
+  if (!ok) {
+    for (int q = 0; q < C * E; ++q) o[q] = 0;
+    return;
+  }
+  if (mode == TT_INTERP_NEAREST) {
+    int z = (int)nearest_index_f(cz, d), y = (int)nearest_index_f(cy, h), x = (int)nearest_index_f(cx, w);
+    int64_t e = (int64_t)z * plane + (int64_t)y * w + x;
+    for (int ch = 0; ch < C; ++ch)
+      for (int q = 0; q < E; ++q) o[ch * E + q] = img[(ch * vol + e) * E + q];
+    return;
+  }
+  int z0 = (int)floor(cz), y0 = (int)floor(cy), x0 = (int)floor(cx);
+  T tz = cz - (T)z0, ty = cy - (T)y0, tx = cx - (T)x0;
+  for (int ch = 0; ch < C; ++ch) {
+    T acc[2] = {0, 0};
+    for (int q = 0; q < 8; ++q) {
+      int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
+      int z = z0 + dz, y = y0 + dy, x = x0 + dx;
+      if (z >= d || y >= h || x >= w) continue;  // weight is 0 there
+      T wt = (dz ? tz : (T)1 - tz) * (dy ? ty : (T)1 - ty) * (dx ? tx : (T)1 - tx);
+      int64_t e = ch * vol + (int64_t)z * plane + (int64_t)y * w + x;
+      for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
+    }
+    for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
+  }
+}
+
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(kBlock) void sample3d_kernel(const T *__restrict__ img, int C, int d,
                                                           int h, int w,
                                                           const T *__restrict__ coords, int64_t n,
                                                           int mode, T *__restrict__ out) {
   const int E = CPLX ? 2 : 1;
-  const int64_t plane = (int64_t)h * w, vol = (int64_t)d * plane;
   for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kBlock)
+    sample3d_point<T, CPLX>(img, C, d, h, w, coords[3 * p], coords[3 * p + 1], coords[3 * p + 2], mode,
+                            out + p * C * E);
+}
+
+// Fused affine warp (affine_transform_image_3d): output voxel (z, y, x) of
+// leading batch index t samples the image at M (z, y, x, 1)^T, M the 3x4 zyx
+// block (row-major, float32, as the reference's float32 matmul of the
+// coordinate grid) of matrix ((t md + z') mh + y') mw + x', where z' = z if
+// md > 1 else 0 (and so on): the matrices broadcast against (d, h, w) as in
+// the reference's matmul.  No coordinate grid is materialised.
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void affine3d_kernel(const float *__restrict__ img, int C, int D, int H,
+                                                          int W, const float *__restrict__ mats, int64_t n_lead,
+                                                          int md, int mh, int mw, int d, int h, int w, int mode,
+                                                          float *__restrict__ out) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t per = (int64_t)d * h * w, total = n_lead * per;
+  for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < total;
        p += (int64_t)gridDim.x * kBlock) {
-    T cz = coords[3 * p], cy = coords[3 * p + 1], cx = coords[3 * p + 2];
-    bool ok = cz >= 0 && cz <= (T)(d - 1) && cy >= 0 && cy <= (T)(h - 1) && cx >= 0 &&
-              cx <= (T)(w - 1);
-    T *o = out + p * C * E;
-    if (!ok) {
-      for (int q = 0; q < C * E; ++q) o[q] = 0;
-      continue;
-    }
-    if (mode == TT_INTERP_NEAREST) {
-      int z = (int)nearest_index_f(cz, d), y = (int)nearest_index_f(cy, h),
-          x = (int)nearest_index_f(cx, w);
-      int64_t e = (int64_t)z * plane + (int64_t)y * w + x;
-      for (int ch = 0; ch < C; ++ch)
-        for (int q = 0; q < E; ++q) o[ch * E + q] = img[(ch * vol + e) * E + q];
-      continue;
-    }
-    int z0 = (int)floor(cz), y0 = (int)floor(cy), x0 = (int)floor(cx);
-    T tz = cz - (T)z0, ty = cy - (T)y0, tx = cx - (T)x0;
-    for (int ch = 0; ch < C; ++ch) {
-      T acc[2] = {0, 0};
-      for (int q = 0; q < 8; ++q) {
-        int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
-        int z = z0 + dz, y = y0 + dy, x = x0 + dx;
-        if (z >= d || y >= h || x >= w) continue;  // weight is 0 there
-        T wt = (dz ? tz : (T)1 - tz) * (dy ? ty : (T)1 - ty) * (dx ? tx : (T)1 - tx);
-        int64_t e = ch * vol + (int64_t)z * plane + (int64_t)y * w + x;
-        for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
-      }
-      for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
-    }
+    const int64_t t = p / per;
+    const int64_t v = p - t * per;
+    const int z = (int)(v / ((int64_t)h * w)), rem = (int)(v - (int64_t)z * h * w);
+    const int y = rem / w, x = rem - y * w;
+    const int64_t mi = ((t * md + (md > 1 ? z : 0)) * mh + (mh > 1 ? y : 0)) * mw + (mw > 1 ? x : 0);
+    const float *M = mats + mi * 12;
+    const float fz = (float)z, fy = (float)y, fx = (float)x;
+    float c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      c[i] = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(M[4 * i], fz), __fmul_rn(M[4 * i + 1], fy)),
+                                 __fmul_rn(M[4 * i + 2], fx)), M[4 * i + 3]);
+    sample3d_point<float, CPLX>(img, C, D, H, W, c[0], c[1], c[2], mode, out + p * C * E);
   }
 }
 
@@ -138,60 +175,88 @@ __device__ __forceinline__ void cubic_weights(T t, T (&wt)[4]) {
   wt[3] = ((A * x3 - (T)5 * A) * x3 + (T)8 * A) * x3 - (T)4 * A;
 }
 
+// One 2-D sample (all channels) at array coordinate (cy, cx) into o.
+template <typename T, bool CPLX>
+__device__ __forceinline__ void sample2d_point(const T *__restrict__ img, int C, int h, int w, T cy, T cx,
+                                               int mode, T *__restrict__ o) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t plane = (int64_t)h * w;
+  const bool ok = cy >= 0 && cy <= (T)(h - 1) && cx >= 0 && cx <= (T)(w - 1);
+  if (!ok) {
+    for (int q = 0; q < C * E; ++q) o[q] = 0;
+    return;
+  }
+  if (mode == TT_INTERP_NEAREST) {
+    const int64_t e = (int64_t)nearest_index_f(cy, h) * w + (int64_t)nearest_index_f(cx, w);
+    for (int ch = 0; ch < C; ++ch)
+      for (int q = 0; q < E; ++q) o[ch * E + q] = img[(ch * plane + e) * E + q];
+    return;
+  }
+  const int y0 = (int)floor(cy), x0 = (int)floor(cx);
+  const T ty = cy - (T)y0, tx = cx - (T)x0;
+  if (mode == TT_INTERP_BICUBIC) {
+    T wy[4], wx[4];
+    cubic_weights(ty, wy);
+    cubic_weights(tx, wx);
+    for (int ch = 0; ch < C; ++ch) {
+      T acc[2] = {0, 0};
+      for (int a = 0; a < 4; ++a) {
+        const int yy = min(max(y0 - 1 + a, 0), h - 1);
+        for (int b = 0; b < 4; ++b) {
+          const int xx = min(max(x0 - 1 + b, 0), w - 1);
+          const T wt = wy[a] * wx[b];
+          const int64_t e = ch * plane + (int64_t)yy * w + xx;
+          for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
+        }
+      }
+      for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
+    }
+    return;
+  }
+  for (int ch = 0; ch < C; ++ch) {
+    T acc[2] = {0, 0};
+    for (int q = 0; q < 4; ++q) {
+      const int dy = q >> 1, dx = q & 1;
+      const int y = y0 + dy, x = x0 + dx;
+      if (y >= h || x >= w) continue;  // weight is 0 there
+      const T wt = (dy ? ty : (T)1 - ty) * (dx ? tx : (T)1 - tx);
+      const int64_t e = ch * plane + (int64_t)y * w + x;
+      for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
+    }
+    for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
+  }
+}
+
 template <typename T, bool CPLX>
 __global__ __launch_bounds__(kBlock) void sample2d_kernel(const T *__restrict__ img, int C, int h, int w,
                                                           const T *__restrict__ coords, int64_t n,
                                                           int mode, T *__restrict__ out) {
   const int E = CPLX ? 2 : 1;
-  const int64_t plane = (int64_t)h * w;
   for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * kBlock)
+    sample2d_point<T, CPLX>(img, C, h, w, coords[2 * p], coords[2 * p + 1], mode, out + p * C * E);
+}
+
+// Fused affine warp (affine_transform_image_2d): output pixel (y, x) of
+// leading batch index t samples the image at M (y, x, 1)^T, M the 2x3 yx
+// block of matrix (t mh + y') mw + x' (broadcast as in the 3-D warp).
+template <bool CPLX>
+__global__ __launch_bounds__(kBlock) void affine2d_kernel(const float *__restrict__ img, int C, int H, int W,
+                                                          const float *__restrict__ mats, int64_t n_lead, int mh,
+                                                          int mw, int h, int w, int mode, float *__restrict__ out) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t per = (int64_t)h * w, total = n_lead * per;
+  for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < total;
        p += (int64_t)gridDim.x * kBlock) {
-    T cy = coords[2 * p], cx = coords[2 * p + 1];
-    bool ok = cy >= 0 && cy <= (T)(h - 1) && cx >= 0 && cx <= (T)(w - 1);
-    T *o = out + p * C * E;
-    if (!ok) {
-      for (int q = 0; q < C * E; ++q) o[q] = 0;
-      continue;
-    }
-    if (mode == TT_INTERP_NEAREST) {
-      const int64_t e = (int64_t)nearest_index_f(cy, h) * w + (int64_t)nearest_index_f(cx, w);
-      for (int ch = 0; ch < C; ++ch)
-        for (int q = 0; q < E; ++q) o[ch * E + q] = img[(ch * plane + e) * E + q];
-      continue;
-    }
-    const int y0 = (int)floor(cy), x0 = (int)floor(cx);
-    const T ty = cy - (T)y0, tx = cx - (T)x0;
-    if (mode == TT_INTERP_BICUBIC) {
-      T wy[4], wx[4];
-      cubic_weights(ty, wy);
-      cubic_weights(tx, wx);
-      for (int ch = 0; ch < C; ++ch) {
-        T acc[2] = {0, 0};
-        for (int a = 0; a < 4; ++a) {
-          const int yy = min(max(y0 - 1 + a, 0), h - 1);
-          for (int b = 0; b < 4; ++b) {
-            const int xx = min(max(x0 - 1 + b, 0), w - 1);
-            const T wt = wy[a] * wx[b];
-            const int64_t e = ch * plane + (int64_t)yy * w + xx;
-            for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
-          }
-        }
-        for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
-      }
-      continue;
-    }
-    for (int ch = 0; ch < C; ++ch) {
-      T acc[2] = {0, 0};
-      for (int q = 0; q < 4; ++q) {
-        const int dy = q >> 1, dx = q & 1;
-        const int y = y0 + dy, x = x0 + dx;
-        if (y >= h || x >= w) continue;  // weight is 0 there
-        const T wt = (dy ? ty : (T)1 - ty) * (dx ? tx : (T)1 - tx);
-        const int64_t e = ch * plane + (int64_t)y * w + x;
-        for (int k = 0; k < E; ++k) acc[k] += wt * img[e * E + k];
-      }
-      for (int k = 0; k < E; ++k) o[ch * E + k] = acc[k];
-    }
+    const int64_t t = p / per;
+    const int v = (int)(p - t * per);
+    const int y = v / w, x = v - y * w;
+    const int64_t mi = (t * mh + (mh > 1 ? y : 0)) * mw + (mw > 1 ? x : 0);
+    const float *M = mats + mi * 6;
+    const float fy = (float)y, fx = (float)x;
+    const float cy = __fadd_rn(__fadd_rn(__fmul_rn(M[0], fy), __fmul_rn(M[1], fx)), M[2]);
+    const float cx = __fadd_rn(__fadd_rn(__fmul_rn(M[3], fy), __fmul_rn(M[4], fx)), M[5]);
+    sample2d_point<float, CPLX>(img, C, H, W, cy, cx, mode, out + p * C * E);
   }
 }
 
@@ -337,6 +402,44 @@ int splat2d_impl(const T *values, const float *coords, int64_t n, int channels, 
   return check_launch("splat2d_kernel");
 }
 
+int affine_impl(int dims, const float *image, int channels, const int *in_shape, int is_complex, const float *mats,
+                int64_t n_lead, const int *mshape, const int *out_shape, int mode, float *out, void *stream) {
+  for (int i = 0; i < dims; ++i) {
+    if (in_shape[i] < 1 || out_shape[i] < 1) return fail_invalid("image and output dimensions must be >= 1");
+    if (mshape[i] != 1 && mshape[i] != out_shape[i])
+      return fail_invalid("matrix batch dimensions must be 1 or the output size (broadcasting)");
+  }
+  if (channels < 1) return fail_invalid("channels must be >= 1");
+  if (mode != TT_INTERP_NEAREST && mode != TT_INTERP_TRILINEAR && !(dims == 2 && mode == TT_INTERP_BICUBIC))
+    return fail_invalid("unknown interpolation mode");
+  if (n_lead <= 0) return TT_OK;
+  if (!image || !mats || !out) return fail_invalid("NULL pointer");
+  int64_t per = 1;
+  for (int i = 0; i < dims; ++i) per *= out_shape[i];
+  if (per >= ((int64_t)1 << 40) || n_lead > ((int64_t)1 << 40) / per) return fail_invalid("output too large");
+  const int64_t total = n_lead * per;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid(grid_for(total)), block(kBlock);
+  if (dims == 3) {
+    if (is_complex)
+      hipLaunchKernelGGL(affine3d_kernel<true>, grid, block, 0, s, image, channels, in_shape[0], in_shape[1],
+                         in_shape[2], mats, n_lead, mshape[0], mshape[1], mshape[2], out_shape[0], out_shape[1],
+                         out_shape[2], mode, out);
+    else
+      hipLaunchKernelGGL(affine3d_kernel<false>, grid, block, 0, s, image, channels, in_shape[0], in_shape[1],
+                         in_shape[2], mats, n_lead, mshape[0], mshape[1], mshape[2], out_shape[0], out_shape[1],
+                         out_shape[2], mode, out);
+    return check_launch("affine3d_kernel");
+  }
+  if (is_complex)
+    hipLaunchKernelGGL(affine2d_kernel<true>, grid, block, 0, s, image, channels, in_shape[0], in_shape[1], mats,
+                       n_lead, mshape[0], mshape[1], out_shape[0], out_shape[1], mode, out);
+  else
+    hipLaunchKernelGGL(affine2d_kernel<false>, grid, block, 0, s, image, channels, in_shape[0], in_shape[1], mats,
+                       n_lead, mshape[0], mshape[1], out_shape[0], out_shape[1], mode, out);
+  return check_launch("affine2d_kernel");
+}
+
 }  // namespace
 
 }  // namespace tt
@@ -389,6 +492,19 @@ int tt_splat2d_f64(const double *values, const float *coords, int64_t n, int cha
                    int h, int w, int mode, double *image, void *weights, int weights_f64, void *stream) {
   return splat2d_impl<double>(values, coords, n, channels, is_complex, h, w, mode, image, weights,
                               weights_f64, stream);
+}
+
+int tt_affine3d_f32(const float *image, int channels, int D, int H, int W, int is_complex, const float *mats,
+                    int64_t n_lead, int md, int mh, int mw, int d, int h, int w, int mode, float *out,
+                    void *stream) {
+  const int in_shape[3] = {D, H, W}, mshape[3] = {md, mh, mw}, out_shape[3] = {d, h, w};
+  return affine_impl(3, image, channels, in_shape, is_complex, mats, n_lead, mshape, out_shape, mode, out, stream);
+}
+
+int tt_affine2d_f32(const float *image, int channels, int H, int W, int is_complex, const float *mats,
+                    int64_t n_lead, int mh, int mw, int h, int w, int mode, float *out, void *stream) {
+  const int in_shape[2] = {H, W}, mshape[2] = {mh, mw}, out_shape[2] = {h, w};
+  return affine_impl(2, image, channels, in_shape, is_complex, mats, n_lead, mshape, out_shape, mode, out, stream);
 }
 
 }  // extern "C"

@@ -345,7 +345,7 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 5; }
+int tt_abi_version(void) { return 6; }
 
 static int g_gather_path = TT_GATHER_AUTO;
 

@@ -2,7 +2,8 @@
 
 Run once, on a machine that has the reference sources, with
 ``python tests/golden/make_golden.py [REFERENCE_ROOT] [--2d]`` (default
-``/root/reference``; ``--2d`` writes only the fixtures of the 2-D analogues).  It imports the reference's pure-Python packages
+``/root/reference``; ``--2d`` / ``--transform`` write only the fixtures of the 2-D
+analogues / of the affine warps).  It imports the reference's pure-Python packages
 (torch-fourier-slice, torch-image-interpolation, torch-grid-utils) on the CPU
 and stores inputs and outputs as small ``.npz`` fixtures next to this script.
 Nothing at test time reads the reference: the tests load the fixtures.
@@ -25,7 +26,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def _import_reference(root):
     pkgs = os.path.join(root, "packages", "primitives")
-    for name in ("torch-fourier-slice", "torch-image-interpolation", "torch-grid-utils"):
+    for name in ("torch-fourier-slice", "torch-image-interpolation", "torch-grid-utils", "torch-affine-utils",
+                 "torch-transform-image"):
         sys.path.insert(0, os.path.join(pkgs, name, "src"))
     import torch_fourier_slice as tfs  # noqa: E402
     import torch_image_interpolation as tii  # noqa: E402
@@ -335,11 +337,101 @@ def main_2d(root):
          val_c2=val_c2.numpy(), val_m1=val_m1.numpy(), val_m2=val_m2.numpy(), **out)
 
 
+def affine_matrices_3d(n, seed, shape):
+    """(n, 4, 4) float32 xyz homogeneous matrices: a rotation about the centre
+    with a random scale and shift, plus a pure half-voxel translation first."""
+    import torch
+    from scipy.spatial.transform import Rotation
+
+    rng = np.random.default_rng(seed)
+    c = (np.array(shape[::-1], dtype=np.float64) - 1) / 2  # xyz centre
+    out = [np.eye(4)]
+    out[0][:3, 3] = [0.5, -0.5, 0.5]
+    for k in range(n - 1):
+        A = Rotation.random(random_state=seed + k).as_matrix() * rng.uniform(0.8, 1.2)
+        t = c - A @ c + rng.uniform(-1.5, 1.5, 3)
+        M = np.eye(4)
+        M[:3, :3], M[:3, 3] = A, t
+        out.append(M)
+    return torch.from_numpy(np.stack(out)).to(torch.float32)
+
+
+def affine_matrices_2d(n, seed, shape):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    c = (np.array(shape[::-1], dtype=np.float64) - 1) / 2
+    out = [np.eye(3)]
+    out[0][:2, 2] = [0.5, -0.5]
+    for _ in range(n - 1):
+        a = rng.uniform(0, 2 * np.pi)
+        A = np.array([[np.cos(a), -np.sin(a)], [np.sin(a), np.cos(a)]]) * rng.uniform(0.8, 1.2)
+        M = np.eye(3)
+        M[:2, :2], M[:2, 2] = A, c - A @ c + rng.uniform(-1.5, 1.5, 2)
+        out.append(M)
+    return torch.from_numpy(np.stack(out)).to(torch.float32)
+
+
+def main_transform(root):
+    """Affine warps of torch-transform-image (transforms_3d.py, transforms_2d.py)."""
+    import torch
+
+    torch.set_num_threads(8)
+    _import_reference(root)
+    import torch_transform_image as tti  # noqa: E402
+
+    def save(name, **arrays):
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
+        print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+    out = {}
+    g = torch.Generator().manual_seed(31)
+    imgs3 = {"real": torch.randn(12, 10, 9, generator=g),
+             "complex": torch.complex(torch.randn(12, 10, 9, generator=g), torch.randn(12, 10, 9, generator=g)),
+             "multi": torch.randn(2, 12, 10, 9, generator=g)}
+    M3 = affine_matrices_3d(4, 32, (12, 10, 9)).reshape(4, 1, 1, 1, 4, 4)  # broadcast over (d, h, w)
+    for key, img in imgs3.items():
+        out[f"img3_{key}"] = img.numpy()
+        for mode in ("trilinear", "nearest"):
+            out[f"a3_{key}_{mode}"] = tti.affine_transform_image_3d(img, M3.clone(), mode).numpy()
+            out[f"a3_{key}_{mode}_zyx_out"] = tti.affine_transform_image_3d(
+                img, M3.clone(), mode, output_shape=(8, 7, 6), zyx_matrices=True).numpy()
+    out["M3"] = M3.numpy()
+    imgs2 = {"real": torch.randn(14, 11, generator=g),
+             "complex": torch.complex(torch.randn(14, 11, generator=g), torch.randn(14, 11, generator=g)),
+             "multi": torch.randn(3, 14, 11, generator=g)}
+    M2 = affine_matrices_2d(5, 33, (14, 11)).reshape(5, 1, 1, 3, 3)
+    for key, img in imgs2.items():
+        out[f"img2_{key}"] = img.numpy()
+        for mode in ("bilinear", "nearest", "bicubic"):
+            out[f"a2_{key}_{mode}"] = tti.affine_transform_image_2d(img, M2.clone(), mode).numpy()
+            out[f"a2_{key}_{mode}_yx_out"] = tti.affine_transform_image_2d(
+                img, M2.clone(), mode, output_shape=(9, 8), yx_matrices=True).numpy()
+    out["M2"] = M2.numpy()
+    img = imgs2["real"]
+    for mode in ("bilinear", "bicubic"):
+        out[f"rts_{mode}"] = tti.rotate_then_shift_image_2d(img, rotate=30.0, shift=(2.5, -1.5),
+                                                            interpolation=mode).numpy()
+        out[f"str_{mode}"] = tti.shift_then_rotate_image_2d(img, rotate=30.0, shift=(2.5, -1.5),
+                                                            interpolation=mode).numpy()
+    out["shift_only"] = tti.rotate_then_shift_image_2d(img, shift=(1.0, 2.0), interpolation="bilinear").numpy()
+    # one matrix per output column: the matrices broadcast against the grid
+    Mcol = affine_matrices_2d(11, 34, (14, 11))
+    out["Mcol"] = Mcol.numpy()
+    out["a2_percol"] = tti.affine_transform_image_2d(img, Mcol.clone(), "bilinear").numpy()
+    out["a3_single"] = tti.affine_transform_image_3d(imgs3["real"], M3[1, 0, 0, 0].clone(), "trilinear").numpy()
+    save("transform_image", **out)
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     root = args[0] if args else "/root/reference"
     if "--2d" in sys.argv:  # only the fixtures of the 2-D analogues
         main_2d(root)
+    elif "--transform" in sys.argv:  # only the affine-warp fixtures
+        main_transform(root)
     else:
         main(root)
         main_2d(root)
+        main_transform(root)
