@@ -82,6 +82,18 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr,
                    void *slices, void *stream);
 
 /*
+ * Multichannel extraction: replaces extract_central_slices_rfft_3d_multichannel
+ * (torch-fourier-slice/.../slice_extraction/_extract_central_slices_rfft_3d.py:225-367).
+ *   volume  (channels, vol_d, vol_h, vol_wr) complex64
+ *   slices  (n_rot, channels, img_h, img_w/2+1) complex64, written
+ * The brick gather serves two channels per launch (one enumeration of the
+ * slice pixels for both); otherwise as tt_extract_c64.
+ */
+int tt_extract_mc_c64(const void *volume, int channels, int vol_d, int vol_h, int vol_wr,
+                      int img_d, int img_h, int img_w, const float *rot_zyx, int64_t n_rot,
+                      double fftfreq_max, double x_tolerance, void *slices, void *stream);
+
+/*
  * Backward of the extraction: the adjoint scatter into grad_volume
  * (complex64 (vol_d, vol_h, vol_wr); accumulated, or written with
  * TT_FLAG_OVERWRITE) and/or the analytic rotation gradient (accumulated into

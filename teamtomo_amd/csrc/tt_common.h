@@ -50,6 +50,8 @@ struct SliceGeom {
   double xtol, xtol10;    // float32(x_tol), float32(10 x_tol) as doubles
   double tolx;            // xtol * ax: the x' tolerance in half-axis array units
   float invWr_f;          // 1/Wr for the flat-index split
+  int64_t ostride;        // gather output elements per rotation (P; channels x P when multichannel)
+  int64_t vstride;        // input volume elements per channel (vD vH vW)
 };
 
 // Flat pixel index -> (row, column) without an integer division.

@@ -16,6 +16,12 @@ bool engine_supports(const SliceGeom &g, int64_t n_rot);
 int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
                    hipStream_t s);
 
+// Two channels of a multichannel volume (channel stride g.vstride) into a
+// multichannel slice stack (rotation stride g.ostride, channel stride g.P),
+// sharing one plan, plane-record pass and brick launch.
+int engine_extract2(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                    hipStream_t s);
+
 // gV (+)= A^T gS, A the extraction of geometry g.  64-bit fixed-point LDS
 // accumulation per brick: bitwise reproducible.
 int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot, f2v *gV,

@@ -391,7 +391,7 @@ __device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, cons
 // ---------------------------------------------------------------------------
 // One unit (a brick and a range of rotations) of the gather, the adjoint or
 // the insertion: shared by the one-call kernel and the batched persistent one.
-template <int BZ, int BY, int BX, int MODE, bool MASK>
+template <int BZ, int BY, int BX, int MODE, bool MASK, int NCH = 1>
 __device__ __forceinline__ void brick_body(
     const f2v *__restrict__ src, const SliceGeom &g, const BrickGrid &bg, const PlaneRec *__restrict__ pre,
     const PlaneTest *__restrict__ ptest, const Unit &un, int n_rot, f2v *__restrict__ out,
@@ -401,8 +401,9 @@ __device__ __forceinline__ void brick_body(
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
   static_assert(CH >= 1 && CH <= 4, "the chunk list holds pixel counts 1 to 4");
-  // gather: NT complex voxels; scatter: NACC fixed-point planes of NT words
-  constexpr int NWORDS = SCATTER ? C::NACC * NT : NT;
+  static_assert(NCH == 1 || (NCH == 2 && MODE == kGather), "two channels: gather only");
+  // gather: NCH x NT complex voxels; scatter: NACC fixed-point planes of NT words
+  constexpr int NWORDS = SCATTER ? C::NACC * NT : NCH * NT;
   // one LDS block, the tile first: the trilinear corner offsets then fit the
   // offset fields of the LDS instructions
   struct Smem {
@@ -436,13 +437,15 @@ __device__ __forceinline__ void brick_body(
 
   if (!SCATTER) {
     f2v *tv = reinterpret_cast<f2v *>(s_tile);
-    for (int v = tid; v < NT; v += C::T) {
-      const int z = v / (SY * SX), y = (v / SX) % SY, x = v % SX;
-      const int gz = lo0 + z, gy = lo1 + y, gx = lo2 + x;
-      f2v val = {0.0f, 0.0f};
-      if (gz < g.vD && gy < g.vH && gx < g.vW) val = src[((int64_t)gz * g.vH + gy) * g.vW + gx];
-      tv[v] = val;
-    }
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch)
+      for (int v = tid; v < NT; v += C::T) {
+        const int z = v / (SY * SX), y = (v / SX) % SY, x = v % SX;
+        const int gz = lo0 + z, gy = lo1 + y, gx = lo2 + x;
+        f2v val = {0.0f, 0.0f};
+        if (gz < g.vD && gy < g.vH && gx < g.vW) val = src[ch * g.vstride + ((int64_t)gz * g.vH + gy) * g.vW + gx];
+        tv[ch * NT + v] = val;
+      }
   } else {
     for (int v = tid; v < NWORDS; v += C::T) s_tile[v] = 0ull;
   }
@@ -797,26 +800,33 @@ __device__ __forceinline__ void brick_body(
             e[q] = own[q] ? (int)ef + (HZ * SY * SX + HY * SX + HX) : 0;
           }
           if (MODE == kGather) {
-            const f2v *tv = reinterpret_cast<const f2v *>(s_tile);
-            f2v cv[CH][8];
+            // channel ch of the output: rotation stride g.ostride, channel stride g.P
+            int64_t obase = base;
+            if (NCH > 1)
+              obase = (int64_t)((u64)(unsigned)rr * (u64)g.ostride + (u64)(unsigned)(i * g.Wr + j0));
 #pragma unroll
-            for (int q = 0; q < CH; ++q) {
-              const f2v *b = tv + e[q];
-              cv[q][0] = b[0];
-              cv[q][1] = b[1];
-              cv[q][2] = b[SX];
-              cv[q][3] = b[SX + 1];
-              cv[q][4] = b[SY * SX];
-              cv[q][5] = b[SY * SX + 1];
-              cv[q][6] = b[SY * SX + SX];
-              cv[q][7] = b[SY * SX + SX + 1];
-            }
+            for (int ch = 0; ch < NCH; ++ch) {
+              const f2v *tv = reinterpret_cast<const f2v *>(s_tile) + ch * NT;
+              f2v cv[CH][8];
 #pragma unroll
-            for (int q = 0; q < CH; ++q) {
-              f2v val = trilerp(cv[q][0], cv[q][1], cv[q][2], cv[q][3], cv[q][4], cv[q][5], cv[q][6],
-                                cv[q][7], t3[q][0], t3[q][1], t3[q][2]);
-              if (neg) val.y = -val.y;
-              if (own[q]) out[base + q] = val;
+              for (int q = 0; q < CH; ++q) {
+                const f2v *b = tv + e[q];
+                cv[q][0] = b[0];
+                cv[q][1] = b[1];
+                cv[q][2] = b[SX];
+                cv[q][3] = b[SX + 1];
+                cv[q][4] = b[SY * SX];
+                cv[q][5] = b[SY * SX + 1];
+                cv[q][6] = b[SY * SX + SX];
+                cv[q][7] = b[SY * SX + SX + 1];
+              }
+#pragma unroll
+              for (int q = 0; q < CH; ++q) {
+                f2v val = trilerp(cv[q][0], cv[q][1], cv[q][2], cv[q][3], cv[q][4], cv[q][5], cv[q][6],
+                                  cv[q][7], t3[q][0], t3[q][1], t3[q][2]);
+                if (neg) val.y = -val.y;
+                if (own[q]) out[obase + ch * g.P + q] = val;
+              }
             }
           } else {
 #pragma unroll
@@ -865,7 +875,7 @@ __device__ __forceinline__ void brick_body(
                                halo, tid, C::T);
 }
 
-template <int BZ, int BY, int BX, int MODE, bool MASK>
+template <int BZ, int BY, int BX, int MODE, bool MASK, int NCH = 1>
 __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
     const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
@@ -873,7 +883,7 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
     int accumulate) {
   const Unit un = units[blockIdx.x];
-  brick_body<BZ, BY, BX, MODE, MASK>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
+  brick_body<BZ, BY, BX, MODE, MASK, NCH>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
                                      halo, sa, accumulate);
 }
 
@@ -1080,7 +1090,8 @@ __device__ __forceinline__ void lin_interval(double A, double B, double iB, doub
 }
 
 __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
-                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block) {
+                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block,
+                                                 int n_chan = 1) {
   constexpr int CH = 4;
   __shared__ ZeroRow s_zr[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1129,7 +1140,7 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
     if (a2 <= b2 && a2 < a1) { int ta = a1, tb = b1; a1 = a2; b1 = b2; a2 = ta; b2 = tb; }
     if (a2 <= b2 && a2 <= b1 + 1) { b1 = max(b1, b2); a2 = 0; b2 = -1; }
     ZeroRow z;
-    z.base = (long long)r * g.P + (long long)i * g.Wr;
+    z.base = (long long)r * g.ostride + (long long)i * g.Wr;
     if (a1 > b1) {  // nothing certain: the whole row
       z.a1 = 0; z.n1 = g.Wr; z.a2 = 0; z.n2 = 0; z.a3 = 0; z.n3 = 0;
     } else if (a2 > b2) {
@@ -1159,14 +1170,15 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
       if (qq < z.n1) j = z.a1 + qq;
       else if (qq < z.n1 + z.n2) j = z.a2 + (qq - z.n1);
       else j = z.a3 + (qq - z.n1 - z.n2);
-      if (live && qq < z.n1 + z.n2 + z.n3) out[z.base + j] = f2v{0.0f, 0.0f};
+      if (live && qq < z.n1 + z.n2 + z.n3)
+        for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
     }
   }
 }
 
 __global__ __launch_bounds__(256) void gather_zero_kernel(SliceGeom g, const float *__restrict__ rot,
-                                                          int64_t n_rows, f2v *__restrict__ out) {
-  gather_zero_body(g, rot, n_rows, out, (int64_t)blockIdx.x);
+                                                          int64_t n_rows, f2v *__restrict__ out, int n_chan) {
+  gather_zero_body(g, rot, n_rows, out, (int64_t)blockIdx.x, n_chan);
 }
 
 // x=0-plane Friedel mirror of the insertion, from the plane of direct splats
@@ -1970,7 +1982,7 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
   const int64_t n_rows = n_rot * g.H;
   const int64_t nblk = (n_rows + 255) / 256;
   if (nblk >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
-  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out);
+  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out, 1);
   if (int st = check_launch("gather_zero_kernel")) return st;
   PlanEntry pe;
   if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
@@ -1996,6 +2008,45 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
                        dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
                        (u64 *)nullptr, (u64 *)nullptr, sa, 0);
+  return check_launch("brick_kernel");
+}
+
+// Two channels of a multichannel volume (channel stride g.vstride) into a
+// multichannel slice stack (rotation stride g.ostride, channel stride g.P):
+// one plan, one plane-record pass and one brick launch serve both channels.
+int engine_extract2(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                    hipStream_t s) {
+  if (int st = check_engine(g, n_rot)) return st;
+  if (n_rot == 0) return TT_OK;
+  const int64_t n_rows = n_rot * g.H;
+  const int64_t nblk = (n_rows + 255) / 256;
+  if (nblk >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
+  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out, 2);
+  if (int st = check_launch("gather_zero_kernel")) return st;
+  PlanEntry pe;
+  if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
+  if (pe.n_units == 0) return TT_OK;
+  Scratch sc(s);
+  const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
+  if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
+  PlaneRec *pre = (PlaneRec *)sc.p;
+  PlaneTest *ptest = (PlaneTest *)((char *)sc.p + off_test);
+  hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
+                     (int)n_rot, pre, ptest);
+  if (int st = check_launch("plane_rec_kernel")) return st;
+  const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
+  ScaleArgs sa = {nullptr, nullptr, 0, 0};
+  constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
+  if (g.use_mask)
+    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true, 2>), dim3((unsigned)pe.n_units), dim3(T),
+                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
+                       (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
+                       (u64 *)nullptr, sa, 0);
+  else
+    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false, 2>), dim3((unsigned)pe.n_units), dim3(T),
+                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
+                       (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
+                       (u64 *)nullptr, sa, 0);
   return check_launch("brick_kernel");
 }
 

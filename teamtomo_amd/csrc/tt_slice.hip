@@ -64,6 +64,8 @@ SliceGeom make_geom(int img_d, int img_h, int img_w, int vol_d, int vol_h, int v
   g.xtol10 = (double)(float)(x_tolerance * 10.0);
   g.tolx = g.xtol * g.ax;
   g.invWr_f = 1.0f / (float)g.Wr;
+  g.ostride = (int64_t)g.P;
+  g.vstride = (int64_t)vol_d * vol_h * vol_wr;
   return g;
 }
 
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void extract_fwd_kernel(
                   k.ty, k.tx);
     if (c.conj) res.y = -res.y;
   }
-  out[r_idx * g.P + p] = res;
+  out[r_idx * g.ostride + p] = res;
 }
 
 // Block-wide sum of 6 doubles, lane 0 of the block adds them to gR.
@@ -345,7 +347,7 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 6; }
+int tt_abi_version(void) { return 7; }
 
 static int g_gather_path = TT_GATHER_AUTO;
 
@@ -379,6 +381,45 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img
                        l.nb_padded, (f2v *)slices + r0 * g.P);
     if ((st = check_launch("extract_fwd_kernel"))) return st;
   }
+  return TT_OK;
+}
+
+int tt_extract_mc_c64(const void *volume, int channels, int vol_d, int vol_h, int vol_wr, int img_d,
+                      int img_h, int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                      double x_tolerance, void *slices, void *stream) {
+  int st = check_common(vol_d, vol_h, vol_wr, img_d, img_h, img_w, n_rot, rot_zyx);
+  if (st) return st;
+  if (channels < 1) return fail_invalid("channels must be >= 1");
+  if (n_rot == 0) return TT_OK;
+  if (!volume || !slices) return fail_invalid("volume / slices pointer is NULL");
+  SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, x_tolerance);
+  if (channels == 1)
+    return tt_extract_c64(volume, vol_d, vol_h, vol_wr, img_d, img_h, img_w, rot_zyx, n_rot, fftfreq_max,
+                          x_tolerance, slices, stream);
+  g.ostride = (int64_t)channels * g.P;
+  hipStream_t s = (hipStream_t)stream;
+  const f2v *vol = (const f2v *)volume;
+  f2v *out = (f2v *)slices;
+  if (g_gather_path == TT_GATHER_BRICK || (g_gather_path == TT_GATHER_AUTO && engine_supports(g, n_rot))) {
+    // channels two at a time through one brick launch each; an odd last
+    // channel is taken with its predecessor (which is written again, equal)
+    for (int c0 = 0; c0 < channels; c0 += 2) {
+      const int cc = std::min(c0, channels - 2);
+      if ((st = engine_extract2(g, vol + cc * g.vstride, rot_zyx, n_rot, out + (int64_t)cc * g.P, s))) return st;
+    }
+    return TT_OK;
+  }
+  int64_t bpr = (g.P + kBlock - 1) / kBlock;
+  int64_t rots_per_launch = std::max<int64_t>(1, kMaxBlocksPerLaunch / bpr);
+  for (int c = 0; c < channels; ++c)
+    for (int64_t r0 = 0; r0 < n_rot; r0 += rots_per_launch) {
+      int64_t nr = std::min(rots_per_launch, n_rot - r0);
+      Launch l = plan(g, nr);
+      hipLaunchKernelGGL(extract_fwd_kernel, dim3((unsigned)l.nb_padded), dim3(kBlock), 0, s,
+                         vol + c * g.vstride, g, rot_zyx + r0 * 9, l.blocks_per_rot, l.n_blocks, l.nb_padded,
+                         out + r0 * g.ostride + (int64_t)c * g.P);
+      if ((st = check_launch("extract_fwd_kernel"))) return st;
+    }
   return TT_OK;
 }
 

@@ -138,6 +138,44 @@ def extract_central_slices_rfft_3d(
     return out.reshape(*stack_shape, *out.shape[-2:])
 
 
+class _ExtractMCFn(torch.autograd.Function):
+    """slices (n, c, h, w) = gather of a (c, d, h, w) volume, two channels per
+    brick launch; backward = per-channel adjoint scatter + rotation grad."""
+
+    @staticmethod
+    def forward(ctx, volume, R, image_shape, fftfreq_max, x_tolerance):
+        n = R.shape[0]
+        C, D, H, Wr = volume.shape
+        img_d, img_h, img_w = (int(s) for s in image_shape)
+        out = torch.empty((n, C, img_h, _rfft_width(img_w)), dtype=volume.dtype, device=volume.device)
+        _lib.call(
+            "tt_extract_mc_c64", _lib.ptr(volume), C, D, H, Wr, img_d, img_h, img_w, _lib.ptr(R), n,
+            _fmax_arg(fftfreq_max), float(x_tolerance), _lib.ptr(out), _lib.stream_of(volume),
+        )
+        ctx.save_for_backward(volume, R)
+        ctx.geom = (img_d, img_h, img_w, fftfreq_max, x_tolerance)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_slices):
+        volume, R = ctx.saved_tensors
+        img_d, img_h, img_w, fftfreq_max, x_tolerance = ctx.geom
+        need_v, need_r = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_v or need_r):
+            return None, None, None, None, None
+        C, D, H, Wr = volume.shape
+        gV = torch.empty_like(volume) if need_v else None
+        gR = torch.zeros((R.shape[0], 3, 3), dtype=torch.float64, device=R.device) if need_r else None
+        for c in range(C):
+            g = grad_slices[:, c].to(torch.complex64).contiguous()
+            _lib.call(
+                "tt_extract_bwd_c64", _lib.ptr(g), _lib.ptr(volume[c]), D, H, Wr, img_d, img_h, img_w,
+                _lib.ptr(R), R.shape[0], _fmax_arg(fftfreq_max), float(x_tolerance),
+                _lib.ptr(gV[c]) if need_v else None, _lib.ptr(gR), _scatter_flags(), _lib.stream_of(volume),
+            )
+        return gV, (gR.to(R.dtype) if gR is not None else None), None, None, None
+
+
 def extract_central_slices_rfft_3d_multichannel(
     volume_rfft: torch.Tensor,
     image_shape: tuple[int, int, int],
@@ -150,17 +188,21 @@ def extract_central_slices_rfft_3d_multichannel(
     """Multichannel extraction: (c, d, h, w//2+1) -> (..., c, h, w//2+1).
 
     Reference: slice_extraction/_extract_central_slices_rfft_3d.py:225-367.
-    Each channel is gathered with the same geometry.
+    All channels share the geometry; the brick gather serves two channels per
+    launch, so the slice pixels are enumerated once per pair of channels.
     """
     assert volume_rfft.ndim == 4, "Input volume_rfft must be 4D tensor."
-    outs = [
-        extract_central_slices_rfft_3d(
-            volume_rfft[c], image_shape, rotation_matrices, fftfreq_max, zyx_matrices,
-            x_tolerance, rot_tolerance,
+    _lib.require_gpu(volume_rfft)
+    if volume_rfft.dtype != torch.complex64:
+        raise RuntimeError(
+            "extract_central_slices_rfft_3d_multichannel: volume_rfft must be complex64 "
+            f"(grid_sample needs input and grid of the same dtype), got {volume_rfft.dtype}"
         )
-        for c in range(volume_rfft.shape[0])
-    ]
-    return torch.stack(outs, dim=-3)
+    R = _prepare_rotations(rotation_matrices, volume_rfft.device, zyx_matrices, rot_tolerance)
+    stack_shape = tuple(R.shape[:-2])
+    R = R.reshape(-1, 3, 3).contiguous()
+    out = _ExtractMCFn.apply(volume_rfft.contiguous(), R, tuple(image_shape), fftfreq_max, x_tolerance)
+    return out.reshape(*stack_shape, *out.shape[-3:])
 
 
 class _InsertFn(torch.autograd.Function):
