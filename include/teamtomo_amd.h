@@ -22,6 +22,7 @@
  *   torch-fourier-slice/src/torch_fourier_slice/slice_extraction/_extract_central_slices_rfft_2d.py:9
  *   torch-image-interpolation/src/torch_image_interpolation/image_interpolation_2d.py:11,104
  *   torch-transform-image/src/torch_transform_image/transforms_3d.py:10, transforms_2d.py:11
+ *   torch-subpixel-crop/src/torch_subpixel_crop/subpixel_crop_3d.py:12, subpixel_crop_2d.py:12
  */
 #ifndef TEAMTOMO_AMD_H
 #define TEAMTOMO_AMD_H
@@ -256,6 +257,29 @@ int tt_affine3d_f32(const float *image, int channels, int D, int H, int W, int i
 int tt_affine2d_f32(const float *image, int channels, int H, int W, int is_complex,
                     const float *mats, int64_t n_lead, int mh, int mw, int h, int w, int mode,
                     float *out, void *stream);
+
+/*
+ * Subpixel crops: replace subpixel_crop_3d / _2d
+ * (torch-subpixel-crop/src/torch_subpixel_crop/subpixel_crop_3d.py:12-108,
+ * subpixel_crop_2d.py:12-163) together with torch-fourier-shift's phase ramp
+ * (torch-fourier-shift/src/torch_fourier_shift/phase_shift_grids.py:60-127);
+ * the FFTs in between stay with the caller.
+ * tt_crop3d_f32: patches (n, p, p, p) = image (D, H, W) around the int32
+ *   positions (n, 3) zyx (patch centre at index p/2, zeros outside), times
+ *   mask (p, p, p) unless NULL.
+ * tt_crop2d_f32: patches (n, B, p, p) = images (B, H, W), patch (m, b) of image
+ *   b around positions[m, b] (n, B, 2) yx, times mask (p, p) unless NULL.
+ * tt_phase_shift_rfft_c64: in place, n rffts (n, [d,] h, w/2+1) complex64 of
+ *   (ndim = 3: d, h, w; ndim = 2: h, w) images times exp(-2 pi i f . s_t),
+ *   f the fftfreq / rfftfreq frequencies (not fftshifted), s_t = shifts[t]
+ *   (n, ndim) float32.
+ */
+int tt_crop3d_f32(const float *image, int D, int H, int W, const int *positions, int64_t n,
+                  int sidelength, const float *mask, float *patches, void *stream);
+int tt_crop2d_f32(const float *images, int B, int H, int W, const int *positions, int64_t n,
+                  int sidelength, const float *mask, float *patches, void *stream);
+int tt_phase_shift_rfft_c64(void *dft, int ndim, const float *shifts, int64_t n, int d, int h,
+                            int w, void *stream);
 
 /*
  * Batched extraction or insertion over many volumes of their own box size and

@@ -27,6 +27,7 @@ from .fourier_slice import (
 )
 from .batch import SliceBatch
 from .image_interpolation import insert_into_image_2d, insert_into_image_3d, sample_image_2d, sample_image_3d
+from .subpixel_crop import subpixel_crop_2d, subpixel_crop_3d
 from .transform_image import (
     affine_transform_image_2d,
     affine_transform_image_3d,
@@ -53,6 +54,8 @@ __all__ = [
     "affine_transform_image_2d",
     "rotate_then_shift_image_2d",
     "shift_then_rotate_image_2d",
+    "subpixel_crop_3d",
+    "subpixel_crop_2d",
     "set_deterministic",
     "deterministic_enabled",
     "set_pixel_atomic",
@@ -77,8 +80,8 @@ _TFS_NAMES = (
 
 
 def install_dropin() -> None:
-    """Register ``torch_fourier_slice``, ``torch_image_interpolation`` and
-    ``torch_transform_image`` modules (and the ``slice_extraction`` /
+    """Register ``torch_fourier_slice``, ``torch_image_interpolation``,
+    ``torch_transform_image`` and ``torch_subpixel_crop`` modules (and the ``slice_extraction`` /
     ``slice_insertion`` submodules) backed by
     this package, so ``from torch_fourier_slice import project_3d_to_2d`` in
     unchanged caller code resolves to the gfx950 implementation."""
@@ -100,7 +103,10 @@ def install_dropin() -> None:
     for name in ("affine_transform_image_3d", "affine_transform_image_2d", "rotate_then_shift_image_2d",
                  "shift_then_rotate_image_2d"):
         setattr(tti, name, globals()[name])
+    tsc = types.ModuleType("torch_subpixel_crop")
+    tsc.subpixel_crop_3d, tsc.subpixel_crop_2d = subpixel_crop_3d, subpixel_crop_2d
     sys.modules.update({
+        "torch_subpixel_crop": tsc,
         "torch_transform_image": tti,
         "torch_fourier_slice": tfs,
         "torch_fourier_slice.slice_extraction": ext,

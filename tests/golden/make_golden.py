@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def _import_reference(root):
     pkgs = os.path.join(root, "packages", "primitives")
     for name in ("torch-fourier-slice", "torch-image-interpolation", "torch-grid-utils", "torch-affine-utils",
-                 "torch-transform-image"):
+                 "torch-transform-image", "torch-fourier-shift", "torch-subpixel-crop"):
         sys.path.insert(0, os.path.join(pkgs, name, "src"))
     import torch_fourier_slice as tfs  # noqa: E402
     import torch_image_interpolation as tii  # noqa: E402
@@ -422,6 +422,31 @@ def main_transform(root):
     out["a2_percol"] = tti.affine_transform_image_2d(img, Mcol.clone(), "bilinear").numpy()
     out["a3_single"] = tti.affine_transform_image_3d(imgs3["real"], M3[1, 0, 0, 0].clone(), "trilinear").numpy()
     save("transform_image", **out)
+
+    # G16: subpixel crops (torch-subpixel-crop), real and rfft outputs
+    import torch_subpixel_crop as tsc  # noqa: E402
+
+    out = {}
+    g = torch.Generator().manual_seed(35)
+    vol = torch.randn(20, 18, 16, generator=g)
+    pos3 = torch.rand(2, 3, 3, generator=g) * torch.tensor([20.0, 18.0, 16.0]) - 1.0  # some patches cross the edge
+    pos3[0, 0] = torch.tensor([9.5, 8.5, 7.25])  # round-half-to-even ties
+    mask3 = torch.rand(8, 8, 8, generator=g)
+    out["vol"], out["pos3"], out["mask3"] = vol.numpy(), pos3.numpy(), mask3.numpy()
+    out["c3"] = tsc.subpixel_crop_3d(vol, pos3, 8).numpy()
+    out["c3_mask"] = tsc.subpixel_crop_3d(vol, pos3, 8, mask=mask3).numpy()
+    out["c3_rfft"] = tsc.subpixel_crop_3d(vol, pos3, 8, return_rfft=True).numpy()
+    out["c3_rfft_dec"] = tsc.subpixel_crop_3d(vol, pos3, 8, return_rfft=True, decenter=True).numpy()
+    imgs = torch.randn(3, 30, 26, generator=g)
+    pos2 = torch.rand(4, 3, 2, generator=g) * torch.tensor([30.0, 26.0]) - 1.0
+    pos2[0, 0] = torch.tensor([10.5, 11.5])
+    mask2 = torch.rand(10, 10, generator=g)
+    out["imgs"], out["pos2"], out["mask2"] = imgs.numpy(), pos2.numpy(), mask2.numpy()
+    out["c2"] = tsc.subpixel_crop_2d(imgs, pos2, 10).numpy()
+    out["c2_mask"] = tsc.subpixel_crop_2d(imgs, pos2, 10, mask=mask2).numpy()
+    out["c2_rfft_dec"] = tsc.subpixel_crop_2d(imgs, pos2, 10, return_rfft=True, decenter=True).numpy()
+    out["c2_single"] = tsc.subpixel_crop_2d(imgs[1], pos2[:, 1], 9).numpy()  # unbatched, odd sidelength
+    save("subpixel_crop", **out)
 
 
 if __name__ == "__main__":
