@@ -282,6 +282,25 @@ int tt_phase_shift_rfft_c64(void *dft, int ndim, const float *shifts, int64_t n,
                             int w, void *stream);
 
 /*
+ * The elementwise steps around the FFTs of project_3d_to_2d and
+ * backproject_2d_to_3d (torch-fourier-slice/.../project.py:44-70,
+ * backproject.py:75-91), one pass each.  sinc^2 is sinc(|f|)^2 on the centred
+ * grid f_i = (i - N/2)/N of the padded N^3 volume.
+ * tt_pad_sinc2_fftshift_*: out (N^3, N = d + 2 pad) = fftshift(pad(vol) * sinc^2).
+ * tt_ifftshift_sinc2_crop_*: out (d^3) = crop(ifftshift(x) / sinc^2) (divide
+ *   != 0: the backprojection's gridding correction) or crop(ifftshift(x) *
+ *   sinc^2) (divide == 0: the adjoint of tt_pad_sinc2_fftshift_*).
+ * tt_wdiv_ifftshift_*: out = ifftshift over the two full axes of
+ *   vol / clamp(w, 1), (D, H, Wr) rfft volume and weights.
+ */
+int tt_pad_sinc2_fftshift_f32(const float *vol, int d, int pad, float *out, void *stream);
+int tt_pad_sinc2_fftshift_f64(const double *vol, int d, int pad, double *out, void *stream);
+int tt_ifftshift_sinc2_crop_f32(const float *x, int N, int pad, int divide, float *out, void *stream);
+int tt_ifftshift_sinc2_crop_f64(const double *x, int N, int pad, int divide, double *out, void *stream);
+int tt_wdiv_ifftshift_c64(const void *vol, const float *w, int D, int H, int Wr, void *out, void *stream);
+int tt_wdiv_ifftshift_c128(const void *vol, const double *w, int D, int H, int Wr, void *out, void *stream);
+
+/*
  * Batched extraction or insertion over many volumes of their own box size and
  * rotation count.  No single reference function corresponds: it runs the
  * work of a loop of extract_central_slices_rfft_3d

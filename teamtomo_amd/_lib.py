@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -65,6 +65,12 @@ _SIGNATURES = {
     "tt_crop3d_f32": [_p, _i, _i, _i, _p, _l, _i, _p, _p, _p],
     "tt_crop2d_f32": [_p, _i, _i, _i, _p, _l, _i, _p, _p, _p],
     "tt_phase_shift_rfft_c64": [_p, _i, _p, _l, _i, _i, _i, _p],
+    "tt_pad_sinc2_fftshift_f32": [_p, _i, _i, _p, _p],
+    "tt_pad_sinc2_fftshift_f64": [_p, _i, _i, _p, _p],
+    "tt_ifftshift_sinc2_crop_f32": [_p, _i, _i, _i, _p, _p],
+    "tt_ifftshift_sinc2_crop_f64": [_p, _i, _i, _i, _p, _p],
+    "tt_wdiv_ifftshift_c64": [_p, _p, _i, _i, _i, _p, _p],
+    "tt_wdiv_ifftshift_c128": [_p, _p, _i, _i, _i, _p, _p],
     "tt_batch_create": [_i, _p, _i, _i, _p, ctypes.POINTER(ctypes.c_void_p)],
     "tt_batch_run": [_p, _p],
     "tt_batch_tile": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],

@@ -311,6 +311,34 @@ def _sinc2_grid(shape, device):
     return torch.sinc(r) ** 2
 
 
+class _PadSinc2Shift(torch.autograd.Function):
+    """fftshift(pad(volume) * sinc^2) in one pass (TFS/project.py:44-66); the
+    backward is the adjoint, crop(ifftshift(g) * sinc^2), also one pass."""
+
+    @staticmethod
+    def forward(ctx, volume, pad):
+        _lib.require_gpu(volume)
+        if volume.dtype not in (torch.float32, torch.float64):
+            raise TypeError(f"project_3d_to_2d: volume must be float32 or float64, got {volume.dtype}")
+        v = volume.contiguous()
+        d = v.shape[-1]
+        N = d + 2 * pad
+        out = torch.empty((N, N, N), dtype=v.dtype, device=v.device)
+        fn = "tt_pad_sinc2_fftshift_f32" if v.dtype == torch.float32 else "tt_pad_sinc2_fftshift_f64"
+        _lib.call(fn, _lib.ptr(v), d, pad, _lib.ptr(out), _lib.stream_of(v))
+        ctx.pad = pad
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        g = grad.contiguous()
+        N = g.shape[-1]
+        out = torch.empty((N - 2 * ctx.pad,) * 3, dtype=g.dtype, device=g.device)
+        fn = "tt_ifftshift_sinc2_crop_f32" if g.dtype == torch.float32 else "tt_ifftshift_sinc2_crop_f64"
+        _lib.call(fn, _lib.ptr(g), N, ctx.pad, 0, _lib.ptr(out), _lib.stream_of(g))
+        return out, None
+
+
 def project_3d_to_2d(
     volume: torch.Tensor,
     rotation_matrices: torch.Tensor,
@@ -331,10 +359,8 @@ def project_3d_to_2d(
     p = 0
     if pad_factor > 1.0:
         p = int((volume.shape[-1] * (pad_factor - 1.0)) // 2)
-        volume = F.pad(volume, pad=[p] * 6)
-    volume_shape = tuple(volume.shape[-3:])
-    volume = volume * _sinc2_grid(volume_shape, volume.device)
-    dft = torch.fft.fftshift(volume, dim=(-3, -2, -1))
+    volume_shape = (d + 2 * p,) * 3
+    dft = _PadSinc2Shift.apply(volume, p)  # pad, x sinc^2, fftshift: one pass
     dft = torch.fft.rfftn(dft, dim=(-3, -2, -1))
     dft = torch.fft.fftshift(dft, dim=(-3, -2))
     projections = extract_central_slices_rfft_3d(
@@ -393,15 +419,29 @@ def backproject_2d_to_3d(
         image_rfft=images, volume_shape=volume_shape, rotation_matrices=rotation_matrices,
         fftfreq_max=fftfreq_max, zyx_matrices=zyx_matrices,
     )
-    weights = torch.clamp(weights, min=1.0)
-    dft = dft / weights
-    dft = torch.fft.ifftshift(dft, dim=(-3, -2))
-    dft = torch.fft.irfftn(dft, dim=(-3, -2, -1))
-    dft = torch.fft.ifftshift(dft, dim=(-3, -2, -1))
-    dft = dft / _sinc2_grid(dft.shape[-3:], dft.device)
-    if pad_factor > 1.0:
-        dft = F.pad(dft, pad=[-p] * 6)
-    return torch.real(dft)
+    if torch.is_grad_enabled() and (dft.requires_grad or weights.requires_grad):
+        # differentiable path: the same steps as torch operations
+        weights = torch.clamp(weights, min=1.0)
+        dft = dft / weights
+        dft = torch.fft.ifftshift(dft, dim=(-3, -2))
+        dft = torch.fft.irfftn(dft, dim=(-3, -2, -1))
+        dft = torch.fft.ifftshift(dft, dim=(-3, -2, -1))
+        dft = dft / _sinc2_grid(dft.shape[-3:], dft.device)
+        if pad_factor > 1.0:
+            dft = F.pad(dft, pad=[-p] * 6)
+        return torch.real(dft)
+    # fused: divide by clamp(weights, 1) with the ifftshift, then ifftshift,
+    # / sinc^2 and the crop after the inverse FFT, one pass each
+    D, H, Wr = dft.shape
+    shifted = torch.empty_like(dft)
+    fn = "tt_wdiv_ifftshift_c64" if dft.dtype == torch.complex64 else "tt_wdiv_ifftshift_c128"
+    _lib.call(fn, _lib.ptr(dft), _lib.ptr(weights.contiguous()), D, H, Wr, _lib.ptr(shifted), _lib.stream_of(dft))
+    real = torch.fft.irfftn(shifted, dim=(-3, -2, -1)).contiguous()
+    N = real.shape[-1]
+    out = torch.empty((N - 2 * p,) * 3, dtype=real.dtype, device=real.device)
+    fn = "tt_ifftshift_sinc2_crop_f32" if real.dtype == torch.float32 else "tt_ifftshift_sinc2_crop_f64"
+    _lib.call(fn, _lib.ptr(real), N, p, 1, _lib.ptr(out), _lib.stream_of(real))
+    return out
 
 
 def backproject_2d_to_3d_multichannel(
