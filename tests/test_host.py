@@ -72,3 +72,73 @@ def test_install_dropin_registers_reference_names():
     for m in ("torch_fourier_slice", "torch_fourier_slice.slice_extraction",
               "torch_fourier_slice.slice_insertion", "torch_image_interpolation"):
         sys.modules.pop(m, None)
+
+
+def test_widened_modules_refuse_cpu_and_mirror_errors():
+    """The 2-D analogues, affine warps and subpixel crops: CPU tensors raise,
+    and the reference's argument errors come first where it raises them."""
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        tt.extract_central_slices_rfft_2d(torch.zeros(8, 5, dtype=torch.complex64), (8, 8), torch.eye(2)[None])
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        tt.sample_image_2d(torch.zeros(4, 4), torch.zeros(2, 2))
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        tt.affine_transform_image_3d(torch.zeros(4, 4, 4), torch.eye(4), "trilinear")
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        tt.subpixel_crop_3d(torch.zeros(8, 8, 8), torch.zeros(1, 3), 4)
+    with pytest.raises(ValueError, match="all dimensions of image must be equal"):
+        tt.project_2d_to_1d(torch.zeros(4, 5), torch.eye(2)[None])
+    with pytest.raises(ValueError, match="pad_factor"):
+        tt.project_2d_to_1d(torch.zeros(4, 4), torch.eye(2)[None], pad_factor=0.5)
+    with pytest.raises(ValueError, match="same dtype"):
+        tt.insert_into_image_2d(torch.zeros(3, dtype=torch.complex128), torch.zeros(3, 2),
+                                torch.zeros(4, 4, dtype=torch.complex64))
+    with pytest.raises(ValueError, match="One coordinate pair"):
+        tt.insert_into_image_2d(torch.zeros(3), torch.zeros(4, 2), torch.zeros(4, 4))
+    with pytest.raises(ValueError, match=r"\(h, w\)"):
+        tt.sample_image_2d(torch.zeros(4), torch.zeros(2, 2))
+
+
+def test_affine_matrix_broadcasting_matches_matmul():
+    """The host layout of broadcast matrix batches (transform_image._matrices)
+    indexes the same matrix the reference's matmul of the grid would use."""
+    from teamtomo_amd.transform_image import _matrices
+
+    g = torch.Generator().manual_seed(0)
+    for bshape, grid in (((), (3, 4, 5)), ((2, 1, 1, 1), (3, 4, 5)), ((5,), (3, 4, 5)), ((4, 1), (3, 4, 5)),
+                         ((2, 3, 1, 5), (3, 4, 5))):
+        M = torch.randn(*bshape, 4, 4, generator=g)
+        blk, lead, n, (md, mh, mw) = _matrices(M, "cpu", grid, flip=False)
+        full = torch.broadcast_shapes(bshape, grid)
+        Mf = M.expand(*full, 4, 4).reshape(-1, *grid, 4, 4)
+        assert n == Mf.shape[0] and lead == tuple(full[:-3])
+        for t in range(n):
+            for z, y, x in ((0, 0, 0), (2, 3, 4), (1, 2, 3)):
+                mi = ((t * md + (z if md > 1 else 0)) * mh + (y if mh > 1 else 0)) * mw + (x if mw > 1 else 0)
+                assert torch.equal(blk[mi], Mf[t, z, y, x, :3, :])
+    with pytest.raises(RuntimeError):
+        _matrices(torch.randn(7, 4, 4), "cpu", (3, 4, 5), flip=False)  # does not broadcast
+    with pytest.raises(RuntimeError, match="same dtype"):
+        _matrices(torch.eye(4, dtype=torch.float64), "cpu", (3, 4, 5), flip=False)
+
+
+def test_install_dropin_registers_widened_modules():
+    import sys
+
+    tt.install_dropin()
+    try:
+        from torch_fourier_slice import extract_central_slices_rfft_2d, project_2d_to_1d
+        from torch_image_interpolation import insert_into_image_2d, sample_image_2d
+        from torch_subpixel_crop import subpixel_crop_2d, subpixel_crop_3d
+        from torch_transform_image import affine_transform_image_3d, rotate_then_shift_image_2d
+
+        assert project_2d_to_1d is tt.project_2d_to_1d
+        assert extract_central_slices_rfft_2d is tt.extract_central_slices_rfft_2d
+        assert sample_image_2d is tt.sample_image_2d and insert_into_image_2d is tt.insert_into_image_2d
+        assert affine_transform_image_3d is tt.affine_transform_image_3d
+        assert rotate_then_shift_image_2d is tt.rotate_then_shift_image_2d
+        assert subpixel_crop_3d is tt.subpixel_crop_3d and subpixel_crop_2d is tt.subpixel_crop_2d
+    finally:
+        for m in ("torch_fourier_slice", "torch_fourier_slice.slice_extraction",
+                  "torch_fourier_slice.slice_insertion", "torch_image_interpolation",
+                  "torch_transform_image", "torch_subpixel_crop"):
+            sys.modules.pop(m, None)
