@@ -310,6 +310,25 @@ def run_lines2d(args, device):
     print(json.dumps(line), flush=True)
 
 
+def copy_bandwidth(device, mib=1024, reps=5):
+    """Measured HBM copy rate (GB/s, read + write bytes) of a 1 GiB
+    device-to-device copy: the practical ceiling next to the 8 TB/s peak."""
+    src = torch.ones(mib << 18, device=device)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * (mib << 20) / ms / 1e6
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -398,6 +417,7 @@ def main():
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes": step_bytes},
         "kernels": kernels,
         "wall_s": wall,
+        "copy_GBs": copy_bandwidth(device),
     }
     if args.torch:
         try:
