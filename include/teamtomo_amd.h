@@ -269,17 +269,46 @@ int tt_affine2d_f32(const float *image, int channels, int H, int W, int is_compl
  *   mask (p, p, p) unless NULL.
  * tt_crop2d_f32: patches (n, B, p, p) = images (B, H, W), patch (m, b) of image
  *   b around positions[m, b] (n, B, 2) yx, times mask (p, p) unless NULL.
- * tt_phase_shift_rfft_c64: in place, n rffts (n, [d,] h, w/2+1) complex64 of
- *   (ndim = 3: d, h, w; ndim = 2: h, w) images times exp(-2 pi i f . s_t),
- *   f the fftfreq / rfftfreq frequencies (not fftshifted), s_t = shifts[t]
- *   (n, ndim) float32.
+ * The phase ramp is tt_fourier_shift_c64 (in place, rfft, not fftshifted).
  */
 int tt_crop3d_f32(const float *image, int D, int H, int W, const int *positions, int64_t n,
                   int sidelength, const float *mask, float *patches, void *stream);
 int tt_crop2d_f32(const float *images, int B, int H, int W, const int *positions, int64_t n,
                   int sidelength, const float *mask, float *patches, void *stream);
-int tt_phase_shift_rfft_c64(void *dft, int ndim, const float *shifts, int64_t n, int d, int h,
-                            int w, void *stream);
+
+/*
+ * Fourier shifts: replace fourier_shift_dft_1d/_2d/_3d
+ * (torch-fourier-shift/src/torch_fourier_shift/fourier_shift_dft.py:10-115,
+ * with the phase grids of phase_shift_grids.py:9-127) and are the middle step
+ * of fourier_shift_image_1d/_2d/_3d (fourier_shift_image.py:10-91).
+ * tt_fourier_shift_c64 / _c128: out[t] = dft[t] * exp(-2 pi i f . s_t) for
+ *   t < n, s_t = shifts[t] (n, ndim) in (z,) (y,) x order.  One DFT holds
+ *   ([d,] [h,] ws) elements, ws = w/2 + 1 if rfft else w, of an image of shape
+ *   ([d,] [h,] w) (d, h ignored below ndim 3, 2).  f is fftfreq on every axis
+ *   (rfftfreq on the last if rfft); fftshifted != 0: the DFTs are fftshifted
+ *   over every axis but an rfft's last.  dft_stride is 0 (one DFT for every
+ *   t) or the element count of one DFT.  out may equal dft.
+ * tt_fourier_shift_bwd_c64 / _c128: the backward of that product for the
+ *   incoming gradient grad_out (n DFTs): grad_dft[t] = grad_out[t] *
+ *   conj(ramp) (n DFTs, skipped when NULL) and grad_shifts[t, a] =
+ *   2 pi sum_v f_a(v) Im(conj(grad_out) out) (n, ndim), skipped when NULL,
+ *   which reads dft.  Stream-ordered scratch (hipMallocAsync) holds the
+ *   per-block partial sums.
+ */
+int tt_fourier_shift_c64(const void *dft, int64_t dft_stride, const float *shifts, int64_t n,
+                         int ndim, int d, int h, int w, int rfft, int fftshifted, void *out,
+                         void *stream);
+int tt_fourier_shift_c128(const void *dft, int64_t dft_stride, const double *shifts, int64_t n,
+                          int ndim, int d, int h, int w, int rfft, int fftshifted, void *out,
+                          void *stream);
+int tt_fourier_shift_bwd_c64(const void *grad_out, const void *dft, int64_t dft_stride,
+                             const float *shifts, int64_t n, int ndim, int d, int h, int w,
+                             int rfft, int fftshifted, void *grad_dft, float *grad_shifts,
+                             void *stream);
+int tt_fourier_shift_bwd_c128(const void *grad_out, const void *dft, int64_t dft_stride,
+                              const double *shifts, int64_t n, int ndim, int d, int h, int w,
+                              int rfft, int fftshifted, void *grad_dft, double *grad_shifts,
+                              void *stream);
 
 /*
  * The elementwise steps around the FFTs of project_3d_to_2d and

@@ -26,6 +26,14 @@ from .fourier_slice import (
     project_3d_to_2d_multichannel,
 )
 from .batch import SliceBatch
+from .fourier_shift import (
+    fourier_shift_dft_1d,
+    fourier_shift_dft_2d,
+    fourier_shift_dft_3d,
+    fourier_shift_image_1d,
+    fourier_shift_image_2d,
+    fourier_shift_image_3d,
+)
 from .image_interpolation import insert_into_image_2d, insert_into_image_3d, sample_image_2d, sample_image_3d
 from .subpixel_crop import subpixel_crop_2d, subpixel_crop_3d
 from .transform_image import (
@@ -56,6 +64,12 @@ __all__ = [
     "shift_then_rotate_image_2d",
     "subpixel_crop_3d",
     "subpixel_crop_2d",
+    "fourier_shift_dft_1d",
+    "fourier_shift_dft_2d",
+    "fourier_shift_dft_3d",
+    "fourier_shift_image_1d",
+    "fourier_shift_image_2d",
+    "fourier_shift_image_3d",
     "set_deterministic",
     "deterministic_enabled",
     "set_pixel_atomic",
@@ -81,8 +95,9 @@ _TFS_NAMES = (
 
 def install_dropin() -> None:
     """Register ``torch_fourier_slice``, ``torch_image_interpolation``,
-    ``torch_transform_image`` and ``torch_subpixel_crop`` modules (and the ``slice_extraction`` /
-    ``slice_insertion`` submodules) backed by
+    ``torch_transform_image``, ``torch_subpixel_crop`` and ``torch_fourier_shift``
+    modules (and the ``slice_extraction`` / ``slice_insertion`` and
+    ``fourier_shift_dft`` / ``fourier_shift_image`` submodules) backed by
     this package, so ``from torch_fourier_slice import project_3d_to_2d`` in
     unchanged caller code resolves to the gfx950 implementation."""
     tfs = types.ModuleType("torch_fourier_slice")
@@ -105,7 +120,18 @@ def install_dropin() -> None:
         setattr(tti, name, globals()[name])
     tsc = types.ModuleType("torch_subpixel_crop")
     tsc.subpixel_crop_3d, tsc.subpixel_crop_2d = subpixel_crop_3d, subpixel_crop_2d
+    tfsh = types.ModuleType("torch_fourier_shift")
+    fsd = types.ModuleType("torch_fourier_shift.fourier_shift_dft")
+    fsi = types.ModuleType("torch_fourier_shift.fourier_shift_image")
+    for k in (1, 2, 3):
+        for mod, name in ((fsd, f"fourier_shift_dft_{k}d"), (fsi, f"fourier_shift_image_{k}d")):
+            setattr(mod, name, globals()[name])
+            setattr(tfsh, name, globals()[name])
+    tfsh.fourier_shift_dft, tfsh.fourier_shift_image = fsd, fsi
     sys.modules.update({
+        "torch_fourier_shift": tfsh,
+        "torch_fourier_shift.fourier_shift_dft": fsd,
+        "torch_fourier_shift.fourier_shift_image": fsi,
         "torch_subpixel_crop": tsc,
         "torch_transform_image": tti,
         "torch_fourier_slice": tfs,

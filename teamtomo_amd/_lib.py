@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -64,7 +64,10 @@ _SIGNATURES = {
     "tt_affine2d_f32": [_p, _i, _i, _i, _i, _p, _l, _i, _i, _i, _i, _i, _p, _p],
     "tt_crop3d_f32": [_p, _i, _i, _i, _p, _l, _i, _p, _p, _p],
     "tt_crop2d_f32": [_p, _i, _i, _i, _p, _l, _i, _p, _p, _p],
-    "tt_phase_shift_rfft_c64": [_p, _i, _p, _l, _i, _i, _i, _p],
+    "tt_fourier_shift_c64": [_p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p],
+    "tt_fourier_shift_c128": [_p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p],
+    "tt_fourier_shift_bwd_c64": [_p, _p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _p],
+    "tt_fourier_shift_bwd_c128": [_p, _p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _p],
     "tt_pad_sinc2_fftshift_f32": [_p, _i, _i, _p, _p],
     "tt_pad_sinc2_fftshift_f64": [_p, _i, _i, _p, _p],
     "tt_ifftshift_sinc2_crop_f32": [_p, _i, _i, _i, _p, _p],

@@ -1,6 +1,6 @@
 // Subpixel crops (torch-subpixel-crop drop-ins): an integer-position crop with
-// zero fill and an optional real-space mask, and the Fourier phase shift that
-// moves each patch by its subpixel remainder.
+// zero fill and an optional real-space mask.  The Fourier phase shift that
+// moves each patch by its subpixel remainder is tt_fourier_shift_c64.
 //
 // Reference: packages/primitives/torch-subpixel-crop/src/torch_subpixel_crop/
 //   subpixel_crop_3d.py:12-108, subpixel_crop_2d.py:12-163: patches of
@@ -9,7 +9,7 @@
 //   mask, then torch-fourier-shift's phase ramp exp(-2 pi i f . s) on the
 //   rfft (packages/primitives/torch-fourier-shift/src/torch_fourier_shift/
 //   phase_shift_grids.py:60-127, f from fftfreq / rfftfreq, not fftshifted).
-// The FFTs stay on torch; the crop and the ramp are one memory pass each.
+// The FFTs stay on torch; the crop is one memory pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,36 +62,6 @@ __global__ __launch_bounds__(kCropBlock) void crop_kernel(const float *__restric
   }
 }
 
-// In-place phase ramp of n rffts of (d, h, w) images ((n, d, h, w//2+1), d = 1
-// for 2-D): multiply by exp(-2 pi i (f_z s_z + f_y s_y + f_x s_x)) with the
-// fftfreq values of each axis (rfftfreq on the last), angle in FP64.
-template <int NDIM>
-__global__ __launch_bounds__(kCropBlock) void phase_shift_kernel(f2v *__restrict__ dft, const float *__restrict__ shifts,
-                                                                int64_t n, int d, int h, int w) {
-  const int wr = w / 2 + 1;
-  const int64_t per = (int64_t)d * h * wr, total = n * per;
-  const double two_pi = 6.283185307179586476925286766559;
-  for (int64_t q = (int64_t)blockIdx.x * kCropBlock + threadIdx.x; q < total;
-       q += (int64_t)gridDim.x * kCropBlock) {
-    const int64_t t = q / per;
-    const int v = (int)(q - t * per);
-    const int kz = v / (h * wr), rem = v - kz * h * wr;
-    const int ky = rem / wr, kx = rem - ky * wr;
-    // fftfreq: k / n for k < (n + 1) / 2, else (k - n) / n
-    const double fz = (double)(kz < (d + 1) / 2 ? kz : kz - d) / d;
-    const double fy = (double)(ky < (h + 1) / 2 ? ky : ky - h) / h;
-    const double fx = (double)kx / w;
-    const float *s = shifts + t * NDIM;
-    double a = fy * (double)s[NDIM - 2] + fx * (double)s[NDIM - 1];
-    if (NDIM == 3) a += fz * (double)s[0];
-    double sn, cs;
-    sincos(-two_pi * a, &sn, &cs);
-    const float c = (float)cs, si = (float)sn;
-    f2v z = dft[q];
-    dft[q] = f2v{z.x * c - z.y * si, z.x * si + z.y * c};
-  }
-}
-
 }  // namespace
 
 }  // namespace tt
@@ -120,22 +90,6 @@ int tt_crop2d_f32(const float *images, int B, int H, int W, const int *positions
   hipLaunchKernelGGL(crop_kernel<2>, dim3(crop_grid(total)), dim3(kCropBlock), 0, (hipStream_t)stream, images, B, 1,
                      H, W, positions, n, sidelength, mask, patches);
   return check_launch("crop_kernel<2>");
-}
-
-int tt_phase_shift_rfft_c64(void *dft, int ndim, const float *shifts, int64_t n, int d, int h, int w, void *stream) {
-  if (ndim != 2 && ndim != 3) return fail_invalid("ndim must be 2 or 3");
-  if ((ndim == 3 && d < 1) || h < 1 || w < 1) return fail_invalid("image dimensions must be >= 1");
-  if (n <= 0) return TT_OK;
-  if (!dft || !shifts) return fail_invalid("NULL pointer");
-  const int dd = ndim == 3 ? d : 1;
-  const int64_t total = n * (int64_t)dd * h * (w / 2 + 1);
-  if (ndim == 3)
-    hipLaunchKernelGGL(phase_shift_kernel<3>, dim3(crop_grid(total)), dim3(kCropBlock), 0, (hipStream_t)stream,
-                       (f2v *)dft, shifts, n, dd, h, w);
-  else
-    hipLaunchKernelGGL(phase_shift_kernel<2>, dim3(crop_grid(total)), dim3(kCropBlock), 0, (hipStream_t)stream,
-                       (f2v *)dft, shifts, n, dd, h, w);
-  return check_launch("phase_shift_kernel");
 }
 
 }  // extern "C"

@@ -3,8 +3,8 @@
 Reference: packages/primitives/torch-subpixel-crop/src/torch_subpixel_crop/
 subpixel_crop_3d.py:12-108 and subpixel_crop_2d.py:12-163.  Patches are cut
 at round(position) (zeros outside the image) by one HIP kernel that also
-applies a (p, p[, p]) mask, and moved by the subpixel remainder with a phase
-ramp kernel on the rfft (torch-fourier-shift phase_shift_grids.py:60-127);
+applies a (p, p[, p]) mask, and moved by the subpixel remainder with the
+Fourier-shift kernel on the rfft (torch-fourier-shift phase_shift_grids.py:60-127);
 the FFTs stay on torch.  Forward only: an input that requires grad raises.
 """
 
@@ -15,6 +15,7 @@ from typing import Optional
 import torch
 
 from . import _lib
+from .fourier_shift import _fourier_shift
 from .image_interpolation import _no_grad
 
 __all__ = ["subpixel_crop_3d", "subpixel_crop_2d"]
@@ -50,10 +51,7 @@ def _shift(patches, shifts, ndim, p, return_rfft, decenter):
     dft = torch.fft.rfftn(patches, dim=dims)
     if return_rfft and decenter:
         shifts = shifts + p / 2
-    s = shifts.to(torch.float32).reshape(-1, ndim).contiguous()
-    dft = dft.contiguous()
-    _lib.call("tt_phase_shift_rfft_c64", _lib.ptr(dft), ndim, _lib.ptr(s), s.shape[0],
-              p if ndim == 3 else 1, p, p, _lib.stream_of(dft))
+    dft = _fourier_shift(dft, (p,) * ndim, shifts.to(torch.float32), True, False, ndim, inplace=True)
     if return_rfft:
         return dft
     return torch.real(torch.fft.irfftn(dft, dim=dims))

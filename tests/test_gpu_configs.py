@@ -158,9 +158,10 @@ def _phase_shift(images, shifts):
 def test_config5_tomogram_backprojection_and_shift_fit(device):
     """61 tilts -60..60 step 2, 512^2 float32 images = N(0,1) + 500 Gaussian
     features (projections of 500 seeded 3-D blobs), per-tilt shifts
-    U(-5,5) px applied as phase ramps.  The insertion into (512,512,257) is
-    checked against the FP64 C oracle; backprojection runs with
-    pad_factor=1; 40 Adam steps on the per-tilt shifts through the backward,
+    U(-5,5) px applied as phase ramps (torch, FP64).  The insertion into
+    (512,512,257) is checked against the FP64 C oracle; backprojection runs
+    with pad_factor=1; 40 Adam steps on the per-tilt shifts through the
+    backward (fourier_shift_image_2d, then backproject_2d_to_3d),
     fitting the low-passed backprojection of the unshifted series, must cut
     the loss tenfold and the shift error fourfold."""
     n = 512
@@ -203,12 +204,14 @@ def test_config5_tomogram_backprojection_and_shift_fit(device):
     err0 = float(torch.sqrt(torch.mean((s.detach().cpu() - shifts_true) ** 2)))
     for _ in range(40):
         opt.zero_grad()
-        rec = tt.backproject_2d_to_3d(_phase_shift(images, -s), R, pad_factor=1.0, fftfreq_max=fm)
+        # the shifts' gradient comes from the Fourier-shift kernel's backward
+        moved = tt.fourier_shift_image_2d(images, -s.float())
+        rec = tt.backproject_2d_to_3d(moved, R, pad_factor=1.0, fftfreq_max=fm)
         loss = torch.sum((rec - target).double() ** 2)
         loss.backward()
         opt.step()
         losses.append(float(loss.detach()))
-        del rec, loss
+        del rec, loss, moved
     err1 = float(torch.sqrt(torch.mean((s.detach().cpu() - shifts_true) ** 2)))
     print(f"config-5 shift fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}, shift error rms {err0:.3f} -> {err1:.3f} px")
     assert losses[-1] < 0.1 * losses[0]

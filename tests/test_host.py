@@ -130,7 +130,12 @@ def test_install_dropin_registers_widened_modules():
         from torch_image_interpolation import insert_into_image_2d, sample_image_2d
         from torch_subpixel_crop import subpixel_crop_2d, subpixel_crop_3d
         from torch_transform_image import affine_transform_image_3d, rotate_then_shift_image_2d
+        from torch_fourier_shift import fourier_shift_dft_2d, fourier_shift_image_3d
+        from torch_fourier_shift.fourier_shift_dft import fourier_shift_dft_3d
 
+        assert fourier_shift_dft_2d is tt.fourier_shift_dft_2d
+        assert fourier_shift_dft_3d is tt.fourier_shift_dft_3d
+        assert fourier_shift_image_3d is tt.fourier_shift_image_3d
         assert project_2d_to_1d is tt.project_2d_to_1d
         assert extract_central_slices_rfft_2d is tt.extract_central_slices_rfft_2d
         assert sample_image_2d is tt.sample_image_2d and insert_into_image_2d is tt.insert_into_image_2d
@@ -140,5 +145,19 @@ def test_install_dropin_registers_widened_modules():
     finally:
         for m in ("torch_fourier_slice", "torch_fourier_slice.slice_extraction",
                   "torch_fourier_slice.slice_insertion", "torch_image_interpolation",
-                  "torch_transform_image", "torch_subpixel_crop"):
+                  "torch_transform_image", "torch_subpixel_crop", "torch_fourier_shift",
+                  "torch_fourier_shift.fourier_shift_dft", "torch_fourier_shift.fourier_shift_image"):
             sys.modules.pop(m, None)
+
+
+def test_fourier_shift_refuses_cpu_and_checks_shapes():
+    dft = torch.zeros((4, 3), dtype=torch.complex64)
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        tt.fourier_shift_dft_2d(dft, (4, 4), torch.zeros(2), True, False)
+    from teamtomo_amd.fourier_shift import _result_dtype
+
+    c64, c128 = torch.complex64, torch.complex128
+    assert _result_dtype(torch.zeros(1, dtype=c64), torch.zeros(1, dtype=torch.int64)) == c64
+    assert _result_dtype(torch.zeros(1, dtype=c64), torch.zeros(1, dtype=torch.float64)) == c128
+    assert _result_dtype(torch.zeros(1, dtype=c128), torch.zeros(1)) == c128
+    assert _result_dtype(torch.zeros(1, dtype=torch.float32), torch.zeros(1)) == c64
