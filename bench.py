@@ -196,10 +196,13 @@ def time_torch_comparator(V, R, gS, steps=1, chunk=256):
     return N_ROT / dt, dt * 1e3, chunk
 
 
-def cpu_baseline(V, R, gS, min_s=10.0, chunk=256):
+def cpu_baseline(V, R, gS, step, min_s=10.0, chunk=256):
     """FP64 C oracle (OpenMP over rotations) on the host CPU: extract + adjoint of
     the rotations in chunks of `chunk` until `min_s` seconds have passed or all
-    N_ROT are done (a bounded sample of the workload), reported per slice."""
+    N_ROT are done (a bounded sample of the workload), reported per slice.
+    The oracle's results for the first chunk also check the GPU's on the same
+    inputs at full size: the slices of the last timed step, and an adjoint of
+    that chunk's gradients."""
     from oracle import c_oracle as C
     from oracle import fourier_slice as O
 
@@ -207,17 +210,43 @@ def cpu_baseline(V, R, gS, min_s=10.0, chunk=256):
     Rn = O.prepare_rotations(R.cpu().numpy(), False, 1e-8)
     g = gS.cpu().numpy()
     C.extract(Vn, (N, N, N), Rn[:2])  # load + first-touch outside the timing
-    done, dt = 0, 0.0
+    done, dt, first = 0, 0.0, None
     while done < N_ROT and dt < min_s:
         sl = slice(done, min(done + chunk, N_ROT))
         t0 = time.perf_counter()
-        C.extract(Vn, (N, N, N), Rn[sl])
-        C.extract_adjoint(g[sl], Vn.shape, (N, N, N), Rn[sl])
+        S = C.extract(Vn, (N, N, N), Rn[sl])
+        G = C.extract_adjoint(g[sl], Vn.shape, (N, N, N), Rn[sl])
         dt += time.perf_counter() - t0
+        if first is None:
+            first = (sl, S, G)
         done = sl.stop
+        del S, G
     return {"value": done / dt, "unit": "slices/s", "cores": C.threads(), "kind": "port",
             "sample": f"FP64 C oracle (oracle/c/tt_oracle.c, OpenMP), extract+adjoint of {done} of the "
-                      f"{N_ROT} rotations at {N}^3 in {dt:.1f} s"}
+                      f"{N_ROT} rotations at {N}^3 in {dt:.1f} s",
+            "parity": parity_check(step, *first)}
+
+
+def parity_check(step, sl, S_ref, G_ref):
+    """Relative L2 and max |error| (over max |reference|) of the GPU's slices
+    and volume gradient against the FP64 oracle for the rotations in `sl`."""
+    L = step.lib
+    n = sl.stop - sl.start
+    gS = step.gS[sl].contiguous()
+    Rz = step.Rz[sl].contiguous()
+    gV = torch.empty_like(step.V)
+    L.call("tt_extract_bwd_c64", L.ptr(gS), L.ptr(step.V), N, N, N // 2 + 1, N, N, N, L.ptr(Rz), n, -1.0, 1e-8,
+           L.ptr(gV), None, step.flags, step.stream)
+    torch.cuda.synchronize()
+
+    def cmp(a, b):
+        den = float(np.linalg.norm(b))
+        return {"rel_l2": float(np.linalg.norm(a - b)) / den, "max_abs_over_max": float(np.max(np.abs(a - b))) /
+                float(np.max(np.abs(b)))}
+
+    return {"rotations": [sl.start, sl.stop], "slices": cmp(step.out[sl].cpu().numpy(), S_ref),
+            "volume_gradient": cmp(gV.cpu().numpy(), G_ref),
+            "bar": "rel_l2 <= 1e-5 (slices), <= 1e-4 (gradient)"}
 
 
 L2D_N = 1024      # 2-D image side: rfft (1024, 513)
@@ -427,7 +456,7 @@ def main():
         except Exception as exc:  # report, do not hide the main result
             line["torch_same_gpu"] = {"error": repr(exc)[:300]}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(V, R, gS)
+        line["cpu_baseline"] = cpu_baseline(V, R, gS, step)
     print(json.dumps(line), flush=True)
 
 
