@@ -39,7 +39,20 @@ struct ShiftGeom {
   int rfft, fftshifted;
   int per;         // d * h * ws
   double inv_D, inv_H, inv_W;  // 1/D, 1/Hh, 1/Wf
+  double inv_hw, inv_ws;       // 1/(h ws), 1/ws
 };
+
+// floor(v / d) for 0 <= v < 2^31 without an integer division: the FP64
+// product with 1/d is within one of the quotient, and the remainder fixes it.
+__device__ __forceinline__ int div_fix(int v, int d, double inv_d) {
+  int q = (int)((double)v * inv_d);
+  const int r = v - q * d;
+  if (r < 0)
+    --q;
+  else if (r >= d)
+    ++q;
+  return q;
+}
 
 // fftfreq value of stored index k on a full axis of size n (inv_n = 1/n).
 __device__ __forceinline__ double full_freq(int k, int n, double inv_n, int shifted) {
@@ -50,8 +63,8 @@ __device__ __forceinline__ double full_freq(int k, int n, double inv_n, int shif
 // Frequencies (fz, fy, fx) of stored element v of one batch entry.
 __device__ __forceinline__ void shift_freqs(const ShiftGeom &g, int v, double &fz, double &fy, double &fx) {
   const int hw = g.h * g.ws;
-  const int kz = v / hw, rem = v - kz * hw;
-  const int ky = rem / g.ws, kx = rem - ky * g.ws;
+  const int kz = g.ndim == 3 ? div_fix(v, hw, g.inv_hw) : 0, rem = v - kz * hw;
+  const int ky = div_fix(rem, g.ws, g.inv_ws), kx = rem - ky * g.ws;
   fz = g.ndim == 3 ? full_freq(kz, g.D, g.inv_D, g.fftshifted) : 0.0;
   fy = g.ndim >= 2 ? full_freq(ky, g.Hh, g.inv_H, g.fftshifted) : 0.0;
   fx = g.rfft ? (double)kx * g.inv_W : full_freq(kx, g.Wf, g.inv_W, g.fftshifted);
@@ -188,6 +201,8 @@ int make_shift_geom(int ndim, int d, int h, int w, int rfft, int fftshifted, Shi
   g.inv_D = 1.0 / g.D;
   g.inv_H = 1.0 / g.Hh;
   g.inv_W = 1.0 / g.Wf;
+  g.inv_hw = 1.0 / ((double)g.h * g.ws);
+  g.inv_ws = 1.0 / g.ws;
   return TT_OK;
 }
 
