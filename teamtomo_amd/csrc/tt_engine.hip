@@ -2083,7 +2083,9 @@ int batch_check(int kind, const tt_batch_item &it) {
   if ((int64_t)it.img_h * (it.img_w / 2 + 1) >= (1 << 22)) return fail_invalid("batch item: slice too large");
   if ((int64_t)it.vol_d * it.vol_h * it.vol_wr >= ((int64_t)1 << 31)) return fail_invalid("batch item: volume too large");
   if (it.n_rot > 0 && (!it.rot_zyx || !it.input)) return fail_invalid("batch item: NULL rotations or input");
-  if (!it.output || (kind == TT_BATCH_INSERT && !it.weights)) return fail_invalid("batch item: NULL output");
+  // an extraction item without rotations writes nothing (its output may be empty)
+  if ((kind == TT_BATCH_INSERT || it.n_rot > 0) && !it.output) return fail_invalid("batch item: NULL output");
+  if (kind == TT_BATCH_INSERT && !it.weights) return fail_invalid("batch item: NULL weights");
   const SliceGeom g = batch_geom(it);
   if (!engine_supports(g, it.n_rot)) return fail_invalid("batch item: beyond the brick engine's limits");
   return TT_OK;

@@ -429,7 +429,12 @@ int tt_extract_bwd_c64(const void *grad_slices, const void *volume, int vol_d, i
                        double *grad_rot, int flags, void *stream) {
   int st = check_common(vol_d, vol_h, vol_wr, img_d, img_h, img_w, n_rot, rot_zyx);
   if (st) return st;
-  if (n_rot == 0 || (!grad_volume && !grad_rot)) return TT_OK;
+  if (n_rot == 0) {  // no slices: an overwritten volume gradient is zero
+    if (grad_volume && (flags & TT_FLAG_OVERWRITE))
+      return zero_fill<float>((float *)grad_volume, 2 * (int64_t)vol_d * vol_h * vol_wr, (hipStream_t)stream);
+    return TT_OK;
+  }
+  if (!grad_volume && !grad_rot) return TT_OK;
   if (!grad_slices) return fail_invalid("grad_slices pointer is NULL");
   if (grad_rot && !volume) return fail_invalid("volume is required for the rotation gradient");
   SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, x_tolerance);
@@ -464,7 +469,13 @@ static int insert_impl(const void *slices, int vol_d, int vol_h, int vol_w, cons
   int vol_wr = vol_w / 2 + 1;
   int st = check_common(vol_d, vol_h, vol_wr, vol_d, vol_h, vol_w, n_rot, rot_zyx);
   if (st) return st;
-  if (n_rot == 0) return TT_OK;
+  if (n_rot == 0) {  // no slices: overwritten outputs are zero
+    if (!(flags & TT_FLAG_OVERWRITE)) return TT_OK;
+    if (!volume || !weights) return fail_invalid("volume / weights pointer is NULL");
+    const int64_t nv = (int64_t)vol_d * vol_h * vol_wr;
+    if ((st = zero_fill<T>((T *)volume, 2 * nv, (hipStream_t)stream))) return st;
+    return zero_fill<T>(weights, nv, (hipStream_t)stream);
+  }
   if (!slices || !volume || !weights) return fail_invalid("slices / volume / weights pointer is NULL");
   SliceGeom g = make_geom(vol_d, vol_h, vol_w, vol_d, vol_h, vol_wr, fftfreq_max, 1e-8);
   return scatter_insert<T>(g, (const typename Cx<T>::type *)slices, rot_zyx, n_rot,

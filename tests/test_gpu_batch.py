@@ -127,3 +127,30 @@ def test_batched_extract_and_insert_match_one_call_and_replay(device):
           f"batched graph replay {t_batch:.2f} ms ({t_loop / t_batch:.2f}x)")
     bx.close()
     bi.close()
+
+
+def test_batch_items_without_rotations(device):
+    """An item with no rotations: its extraction output is empty and its
+    insertion writes a zero volume and zero weights; the other items match the
+    one-call results."""
+    n = 64
+    vols = [volume_rfft(blob_volume(n, 20, seed=300 + k, device=device)).contiguous() for k in range(2)]
+    rots = [uniform_rotations(50, 301, device=device), torch.zeros(0, 3, 3, device=device)]
+    bx = tt.SliceBatch("extract", vols, rots, autotune=False)
+    out = [o.clone() for o in bx.run()]
+    assert out[1].shape == (0, n, n // 2 + 1)
+    ref = tt.extract_central_slices_rfft_3d(vols[0], (n, n, n), rots[0])
+    assert float((out[0] - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
+    bx.close()
+    g = torch.Generator(device=device).manual_seed(6)
+    slices = [torch.complex(torch.randn(ref.shape, generator=g, device=device),
+                            torch.randn(ref.shape, generator=g, device=device)).contiguous(),
+              torch.zeros((0, n, n // 2 + 1), dtype=torch.complex64, device=device)]
+    for autotune in (False, True):
+        bi = tt.SliceBatch("insert", slices, rots, shapes=[(n, n, n)] * 2, autotune=autotune)
+        (v0, w0), (v1, w1) = bi.run()
+        torch.cuda.synchronize()
+        rv, rw = tt.insert_central_slices_rfft_3d(slices[0], (n, n, n), rots[0])
+        assert rel_l2(v0.cpu().numpy(), rv.cpu().numpy()) < 1e-5 and rel_l2(w0.cpu().numpy(), rw.cpu().numpy()) < 1e-5
+        assert not bool(v1.abs().amax()) and not bool(w1.abs().amax())
+        bi.close()

@@ -104,6 +104,29 @@ def test_extract_shapes_dtypes_and_errors(device):
         tt.extract_central_slices_rfft_3d(V.cpu(), (16, 16, 16), Rf.cpu())
 
 
+def test_empty_batches_give_zero_outputs(device):
+    """No slices: insertion returns a zero volume and zero weights, and the
+    volume gradient of an empty extraction is zero (the outputs are freshly
+    allocated, so nothing may be left unwritten)."""
+    n = 16
+    for dtype in (torch.complex64, torch.complex128):
+        for _ in range(2):  # fresh allocations may reuse the same (dirty) memory
+            junk = torch.full((4 * n * n * n,), 7.0, device=device)
+            del junk
+            vol, w = tt.insert_central_slices_rfft_3d(torch.zeros(0, n, n // 2 + 1, dtype=dtype, device=device),
+                                                      (n, n, n), torch.zeros(0, 3, 3, device=device))
+            assert vol.shape == (n, n, n // 2 + 1) and vol.dtype == dtype
+            assert not bool(vol.abs().amax()) and not bool(w.abs().amax())
+    V = volume_rfft(torch.randn(n, n, n)).to(device).requires_grad_(True)
+    S = tt.extract_central_slices_rfft_3d(V, (n, n, n), torch.zeros(0, 3, 3, device=device))
+    torch.real(torch.sum(S)).backward()
+    assert V.grad is not None and not bool(V.grad.abs().amax())
+    img = torch.zeros(8, 8, device=device)
+    out, wts = tt.insert_into_image_2d(torch.zeros(0, device=device), torch.zeros(0, 2, device=device), img)
+    assert out is img and not bool(wts.abs().amax())
+    assert tt.sample_image_3d(torch.randn(4, 4, 4, device=device), torch.zeros(0, 3, device=device)).shape == (0,)
+
+
 # ------------------------------------------------------------------ insertion
 def test_insert_config1_vs_oracle_and_golden(device):
     G = golden("extract_insert_64")
