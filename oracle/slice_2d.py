@@ -10,7 +10,10 @@ Covered (reference roots as in ``oracle/fourier_slice.py``):
 * ``extract_central_lines_adjoint`` -- its adjoint w.r.t. the image (what torch
   autograd through grid_sample computes);
 * ``project_2d_to_1d`` -- TFS/project.py:180-260;
-* ``sample_image_2d`` / ``insert_into_image_2d`` -- TII/image_interpolation_2d.py:11-275.
+* ``sample_image_2d`` / ``insert_into_image_2d`` -- TII/image_interpolation_2d.py:11-275;
+* ``sample_image_1d`` / ``insert_into_image_1d`` -- TII/image_interpolation_1d.py:11-244
+  (grid_sample on the image repeated to height 2 at y = 0, which reduces to
+  the 1-D rules written out here).
 
 Conventions: a 2-D rfft image is ``fftshift(rfftn(fftshift(img)), axes=(-2,))``
 with DC at ``(H//2, 0)``; ``R_yx`` are (n, 2, 2) rotations after the float32 cast
@@ -25,7 +28,10 @@ against the R = I known answer (the line is row H//2 of the image), the
 dot-product test; the samplers by exact reproduction of affine functions
 (bilinear), of constants by the bicubic sampler and of quadratics by its
 weight formula at A = -0.5 (Keys 1981), interpolation at integers, and the
-reference tests' known answers (TII tests/test_image_interpolation_2d.py:65-160).
+reference tests' known answers (TII tests/test_image_interpolation_2d.py:65-160
+and tests/test_image_interpolation_1d.py:66-160); the 1-D sampler also
+against the golden-pinned 2-D sampler on the repeated image, the
+composition the reference runs.
 ``fftfreq_max`` of the line extraction: parity unpinned (the reference raises
 einops.EinopsError for every value, see ``extract_central_lines``).
 """
@@ -43,6 +49,8 @@ __all__ = [
     "project_2d_to_1d",
     "sample_image_2d",
     "insert_into_image_2d",
+    "sample_image_1d",
+    "insert_into_image_1d",
     "image_rfft_2d",
 ]
 
@@ -292,4 +300,89 @@ def insert_into_image_2d(values, coordinates, image, weights=None, interpolation
             for ch in range(img3.shape[0]):
                 np.add.at(img3[ch], (yy, xx), wgt * v[:, ch])
             np.add.at(weights, (yy, xx), wgt)
+    return img, weights
+
+
+def sample_image_1d(image, coordinates, interpolation="linear"):
+    """Generic 1-D sampler (TII/image_interpolation_1d.py:11-107).
+
+    image (w,) or (c, w), real or complex; coordinates (...) in array units.
+    The reference samples the image repeated to height 2 at y = 0 with border
+    padding, so every y tap reads the same row: linear is (1 - t) v[x0] +
+    t v[x0 + 1]; nearest rounds grid_sample's float32 round trip half to
+    even; cubic is Keys A=-0.75 over the taps x0-1..x0+2 clipped to [0, w-1].
+    Points outside [0, w-1] give 0.  ``interpolation`` is 'linear' (or
+    'bilinear'), 'cubic' (or 'bicubic') or 'nearest'.  Returns (...) or (..., c).
+    """
+    img = np.asarray(image)
+    if img.ndim not in (1, 2):
+        raise ValueError("image should have shape (w, ) or (c, w)")
+    multich = img.ndim == 2
+    if not multich:
+        img = img[None]
+    dt = np.complex128 if np.iscomplexobj(img) else np.float64
+    img = img.astype(dt)
+    coords = np.asarray(coordinates, dtype=np.float64)
+    lead = coords.shape
+    c = coords.reshape(-1)
+    w = img.shape[1]
+    ok = (c >= 0) & (c <= w - 1)
+    cc = np.where(ok, c, 0.0)
+    mode = {"bilinear": "linear", "bicubic": "cubic"}.get(interpolation, interpolation)
+    if mode == "nearest":
+        ix = np.rint(np.clip(_grid_sample_unnormalize_f32(cc, w), 0, w - 1)).astype(np.int64)
+        out = img[:, ix].T
+    elif mode == "cubic":
+        x0 = np.floor(cc).astype(np.int64)
+        wx = _cubic_weights(cc - x0)
+        out = np.zeros((c.shape[0], img.shape[0]), dtype=dt)
+        for b in range(4):
+            out += wx[b][:, None] * img[:, np.clip(x0 - 1 + b, 0, w - 1)].T
+    elif mode == "linear":
+        x0 = np.floor(cc).astype(np.int64)
+        x1 = np.minimum(x0 + 1, w - 1)
+        t = (cc - x0)[:, None]
+        out = (1.0 - t) * img[:, x0].T + t * img[:, x1].T
+    else:
+        raise ValueError(f"unknown interpolation {interpolation!r}")
+    out = np.where(ok[:, None], out, 0.0).reshape(*lead, img.shape[0])
+    return out if multich else out[..., 0]
+
+
+def insert_into_image_1d(values, coordinates, image, weights=None, interpolation="linear"):
+    """Generic 1-D splat, in place (TII/image_interpolation_1d.py:110-244).
+
+    linear: floor/ceil corners with weights 1 - t and t, image += w*v,
+    weights += w; nearest: round half to even, weight 1.  Points outside
+    [0, w-1] are dropped; an unknown mode changes nothing.  ``image`` (w,) or
+    (c, w) and ``weights`` (w,) are float64 / complex128 arrays modified in
+    place and returned.
+    """
+    img = image
+    multich = img.ndim == 2
+    vals = np.asarray(values)
+    coords = np.asarray(coordinates, dtype=np.float64)
+    vshape = vals.shape[:-1] if multich else vals.shape
+    if vshape != coords.shape:
+        raise ValueError("One coordinate is required for each value in data.")
+    w = img.shape[-1]
+    if weights is None:
+        weights = np.zeros((w,), dtype=np.float64)
+    img2 = img if multich else img[None]
+    v = vals.reshape(-1, img2.shape[0]) if multich else vals.reshape(-1, 1)
+    c = coords.reshape(-1)
+    ok = (c >= 0) & (c <= w - 1)
+    v, c = v[ok], c[ok]
+    if interpolation == "nearest":
+        ix = np.rint(c).astype(np.int64)
+        for ch in range(img2.shape[0]):
+            np.add.at(img2[ch], ix, v[:, ch])
+        np.add.at(weights, ix, 1.0)
+    elif interpolation == "linear":
+        x0, x1 = np.floor(c).astype(np.int64), np.ceil(c).astype(np.int64)
+        t = c - x0
+        for xx, wgt in ((x0, 1 - t), (x1, t)):
+            for ch in range(img2.shape[0]):
+                np.add.at(img2[ch], xx, wgt * v[:, ch])
+            np.add.at(weights, xx, wgt)
     return img, weights

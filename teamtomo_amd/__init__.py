@@ -34,7 +34,14 @@ from .fourier_shift import (
     fourier_shift_image_2d,
     fourier_shift_image_3d,
 )
-from .image_interpolation import insert_into_image_2d, insert_into_image_3d, sample_image_2d, sample_image_3d
+from .image_interpolation import (
+    insert_into_image_1d,
+    insert_into_image_2d,
+    insert_into_image_3d,
+    sample_image_1d,
+    sample_image_2d,
+    sample_image_3d,
+)
 from .subpixel_crop import subpixel_crop_2d, subpixel_crop_3d
 from .transform_image import (
     affine_transform_image_2d,
@@ -58,6 +65,8 @@ __all__ = [
     "insert_into_image_3d",
     "sample_image_2d",
     "insert_into_image_2d",
+    "sample_image_1d",
+    "insert_into_image_1d",
     "affine_transform_image_3d",
     "affine_transform_image_2d",
     "rotate_then_shift_image_2d",
@@ -112,7 +121,8 @@ def install_dropin() -> None:
         setattr(tfs, name, globals()[name])
     tfs.slice_extraction, tfs.slice_insertion = ext, ins
     tii = types.ModuleType("torch_image_interpolation")
-    for name in ("sample_image_3d", "insert_into_image_3d", "sample_image_2d", "insert_into_image_2d"):
+    for name in ("sample_image_3d", "insert_into_image_3d", "sample_image_2d", "insert_into_image_2d",
+                 "sample_image_1d", "insert_into_image_1d"):
         setattr(tii, name, globals()[name])
     tti = types.ModuleType("torch_transform_image")
     for name in ("affine_transform_image_3d", "affine_transform_image_2d", "rotate_then_shift_image_2d",

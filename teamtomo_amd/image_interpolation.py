@@ -1,7 +1,8 @@
-"""Drop-in replacements for torch_image_interpolation's 3-D and 2-D entry points.
+"""Drop-in replacements for torch_image_interpolation's 3-D, 2-D and 1-D entry points.
 
 Reference: packages/primitives/torch-image-interpolation/src/
-torch_image_interpolation/image_interpolation_3d.py and image_interpolation_2d.py.
+torch_image_interpolation/image_interpolation_3d.py, image_interpolation_2d.py
+and image_interpolation_1d.py.
 Sampling and splatting run as gfx950 HIP kernels through the C ABI
 (include/teamtomo_amd.h).  These generic samplers are forward-only: a call
 whose image or values require grad raises, rather than returning a result
@@ -16,7 +17,14 @@ import torch
 
 from . import _lib
 
-__all__ = ["sample_image_3d", "insert_into_image_3d", "sample_image_2d", "insert_into_image_2d"]
+__all__ = [
+    "sample_image_3d",
+    "insert_into_image_3d",
+    "sample_image_2d",
+    "insert_into_image_2d",
+    "sample_image_1d",
+    "insert_into_image_1d",
+]
 
 _REAL_OF = {
     torch.float32: torch.float32,
@@ -215,4 +223,64 @@ def insert_into_image_2d(
               _lib.TT_INTERP_BILINEAR if interpolation == "bilinear" else _lib.TT_INTERP_NEAREST,
               _lib.ptr(image), _lib.ptr(weights), int(weights.dtype == torch.float64),
               _lib.stream_of(image))
+    return image, weights
+
+
+def sample_image_1d(
+    image: torch.Tensor,
+    coordinates: torch.Tensor,
+    interpolation: Literal["nearest", "linear", "cubic"] = "linear",
+) -> torch.Tensor:
+    """Sample a (w,) or (c, w) image at (...) array coordinates.
+
+    Reference: image_interpolation_1d.py:11-107, which runs grid_sample on the
+    image repeated to height 2 at y = 0 with border padding.  Here the same
+    composition runs on the 2-D sampling kernel: every y tap reads the one row,
+    so linear, cubic (Keys, border-clipped taps) and nearest are the 1-D rules.
+    Returns (...) or (..., c); points outside [0, w-1] give 0.
+    """
+    if image.ndim not in (1, 2):
+        raise ValueError(f"image should have shape (w, ) or (c, w), got {image.shape}")
+    mode = {"linear": "bilinear", "cubic": "bicubic"}.get(interpolation, interpolation)
+    multichannel = image.ndim == 2
+    img = (image if multichannel else image.unsqueeze(0)).unsqueeze(1).expand(-1, 2, -1)
+    crd = torch.stack([torch.zeros_like(coordinates), coordinates], dim=-1)
+    out = sample_image_2d(img, crd, interpolation=mode)  # (..., c)
+    return out if multichannel else out[..., 0]
+
+
+def insert_into_image_1d(
+    values: torch.Tensor,
+    coordinates: torch.Tensor,
+    image: torch.Tensor,
+    weights: torch.Tensor | None = None,
+    interpolation: Literal["nearest", "linear"] = "linear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """Splat (...) or (..., c) values at (...) coordinates into a (w,) or
+    (c, w) ``image`` and (w,) ``weights``, in place; returns both.
+
+    Reference: image_interpolation_1d.py:110-244.  Linear splats use floor/ceil
+    corners with weights 1 - t and t, nearest rounds half to even with weight
+    1; points outside [0, w-1] are dropped and an unknown mode changes
+    nothing.  Runs on the 2-D splat kernel with the image seen as (c, 1, w).
+    """
+    multichannel = image.ndim == 2
+    w = image.shape[-1]
+    values_shape = values.shape[:-1] if multichannel else values.shape
+    if values_shape != coordinates.shape:
+        raise ValueError("One coordinate is required for each value in data.")
+    if image.dtype != values.dtype:
+        raise ValueError("Image and values must have the same dtype.")
+    _lib.require_gpu(image, values)
+    if weights is None:
+        weights = torch.zeros(size=(w,), dtype=torch.float32, device=image.device)
+    if interpolation not in ("nearest", "linear"):
+        return image, weights
+    if not image.is_contiguous() or not weights.is_contiguous():
+        raise ValueError("insert_into_image_1d: image and weights must be contiguous (updated in place)")
+    C = image.shape[0] if multichannel else 1
+    vals = values if multichannel else values.unsqueeze(-1)
+    crd = torch.stack([torch.zeros_like(coordinates, dtype=torch.float32), coordinates.to(torch.float32)], dim=-1)
+    insert_into_image_2d(vals, crd, image.view(C, 1, w), weights.view(1, w),
+                         interpolation="bilinear" if interpolation == "linear" else "nearest")
     return image, weights

@@ -215,3 +215,84 @@ def test_insert_into_image_2d_known_answers_and_types(device):
         tt.insert_into_image_2d(torch.ones((3, 4), dtype=torch.complex128, device=device),
                                 torch.rand((3, 4, 2), device=device),
                                 torch.zeros((4, 4), dtype=torch.complex64, device=device))
+
+
+# ---------------------------------------------------------------- 1-D samplers (on the 2-D kernels)
+
+
+@pytest.mark.parametrize("mode", ["linear", "cubic", "nearest"])
+def test_sample_image_1d_vs_oracle(device, mode):
+    rng = np.random.default_rng(60)
+    img = rng.standard_normal(40).astype(np.float32)
+    mc = (rng.standard_normal((3, 40)) + 1j * rng.standard_normal((3, 40))).astype(np.complex64)
+    c = rng.uniform(-2, 41, (5, 70)).astype(np.float32)
+    c[0, :6] = [0.0, 39.0, 12.5, 3.5, 7.5, 20.0]  # edges and nearest ties
+    out = tt.sample_image_1d(_t(img, device), _t(c, device), mode)
+    assert out.shape == (5, 70)
+    assert max_abs(_np(out), O.sample_image_1d(img, c, mode)) < 1e-5
+    out = tt.sample_image_1d(_t(mc, device), _t(c, device), mode)
+    assert out.shape == (5, 70, 3) and out.dtype == torch.complex64
+    assert max_abs(_np(out), O.sample_image_1d(mc, c, mode)) < 1e-5
+
+
+def test_sample_image_1d_reference_cases(device):
+    """TII tests/test_image_interpolation_1d.py:9-63: integer coordinates of
+    any (...) shape, complex and multichannel images."""
+    rng = np.random.default_rng(61)
+    coords = torch.tensor(rng.integers(0, 27, size=(6, 7, 8)), device=device)
+    image = torch.rand(28, device=device)
+    for mode in ("nearest", "linear", "cubic"):
+        s = tt.sample_image_1d(image, coords, interpolation=mode)
+        assert s.shape == (6, 7, 8)
+        assert torch.allclose(s, image[coords])  # integer positions read the pixel
+    cimg = torch.complex(torch.rand(28, device=device), torch.rand(28, device=device))
+    assert tt.sample_image_1d(cimg, coords).shape == (6, 7, 8)
+    assert tt.sample_image_1d(torch.rand(3, 28, device=device), coords).shape == (6, 7, 8, 3)
+    assert tt.sample_image_1d(cimg.expand(3, 28).contiguous(), coords).shape == (6, 7, 8, 3)
+    with pytest.raises(ValueError):
+        tt.sample_image_1d(torch.rand(2, 3, 28, device=device), coords)
+
+
+def test_insert_into_image_1d_reference_cases(device):
+    """TII tests/test_image_interpolation_1d.py:66-195: known answers, in-place
+    accumulation, multichannel shapes, dtype consistency and the dtype error."""
+    image = torch.zeros(28, device=device)
+    out, w = tt.insert_into_image_1d(torch.tensor([1.0], device=device), torch.tensor([14.28], device=device), image)
+    want = torch.zeros(28, device=device)
+    want[14], want[15] = 0.72, 0.28
+    assert out is image and torch.allclose(out, want) and torch.allclose(w, want)
+    image = torch.zeros(28, device=device)
+    out, _ = tt.insert_into_image_1d(torch.tensor([5.0], device=device), torch.tensor([10.7], device=device), image,
+                                     interpolation="nearest")
+    want = torch.zeros(28, device=device)
+    want[11] = 5
+    assert torch.allclose(out, want)
+    rng = np.random.default_rng(62)
+    coords = torch.tensor(rng.integers(0, 27, size=(6, 7, 8)), device=device)
+    for mode in ("linear", "nearest"):
+        img3 = torch.zeros(3, 28, device=device)
+        img3, w3 = tt.insert_into_image_1d(torch.ones(6, 7, 8, 3, device=device), coords, img3, interpolation=mode)
+        assert img3.shape == (3, 28) and w3.shape == (28,) and img3[0, coords[0, 0, 0]] > 0
+    for dtype in (torch.float32, torch.float64, torch.complex64, torch.complex128):
+        img = torch.rand(4, dtype=dtype, device=device)
+        wts = torch.zeros(4, dtype=torch.float64, device=device)
+        cc = torch.tensor(rng.uniform(0, 3, size=(3, 4)), device=device)
+        for mode in ("linear", "nearest"):
+            img, wts = tt.insert_into_image_1d(torch.rand(3, 4, dtype=dtype, device=device), cc, img, wts, mode)
+            assert img.dtype == dtype and wts.dtype == torch.float64
+    with pytest.raises(ValueError):
+        tt.insert_into_image_1d(torch.rand(3, 4, dtype=torch.complex128, device=device),
+                                torch.zeros(3, 4, device=device), torch.rand(4, dtype=torch.complex64, device=device))
+
+
+@pytest.mark.parametrize("mode", ["linear", "nearest"])
+def test_insert_into_image_1d_vs_oracle(device, mode):
+    rng = np.random.default_rng(63)
+    vals = (rng.standard_normal((500, 2)) + 1j * rng.standard_normal((500, 2))).astype(np.complex64)
+    c = rng.uniform(-1, 50, 500).astype(np.float32)
+    c[:4] = [0.0, 49.0, 10.5, 11.5]
+    img = torch.zeros(2, 50, dtype=torch.complex64, device=device)
+    img, w = tt.insert_into_image_1d(_t(vals, device), _t(c, device), img, interpolation=mode)
+    ri, rw = O.insert_into_image_1d(vals.astype(np.complex128), c, np.zeros((2, 50), np.complex128),
+                                    interpolation=mode)
+    assert max_abs(_np(img), ri) < 1e-4 and max_abs(_np(w), rw) < 1e-4

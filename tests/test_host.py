@@ -127,7 +127,7 @@ def test_install_dropin_registers_widened_modules():
     tt.install_dropin()
     try:
         from torch_fourier_slice import extract_central_slices_rfft_2d, project_2d_to_1d
-        from torch_image_interpolation import insert_into_image_2d, sample_image_2d
+        from torch_image_interpolation import insert_into_image_1d, insert_into_image_2d, sample_image_1d, sample_image_2d
         from torch_subpixel_crop import subpixel_crop_2d, subpixel_crop_3d
         from torch_transform_image import affine_transform_image_3d, rotate_then_shift_image_2d
         from torch_fourier_shift import fourier_shift_dft_2d, fourier_shift_image_3d
@@ -139,6 +139,7 @@ def test_install_dropin_registers_widened_modules():
         assert project_2d_to_1d is tt.project_2d_to_1d
         assert extract_central_slices_rfft_2d is tt.extract_central_slices_rfft_2d
         assert sample_image_2d is tt.sample_image_2d and insert_into_image_2d is tt.insert_into_image_2d
+        assert sample_image_1d is tt.sample_image_1d and insert_into_image_1d is tt.insert_into_image_1d
         assert affine_transform_image_3d is tt.affine_transform_image_3d
         assert rotate_then_shift_image_2d is tt.rotate_then_shift_image_2d
         assert subpixel_crop_3d is tt.subpixel_crop_3d and subpixel_crop_2d is tt.subpixel_crop_2d

@@ -158,3 +158,53 @@ def test_project_2d_identity_is_column_sum():
     P = O.project_2d_to_1d(img, np.eye(2)[None], pad_factor=1.0)
     ref = (img * O._sinc2_grid_2d((16, 16))).sum(axis=0)
     assert max_abs(P[0], ref) < 1e-12
+
+
+# ---------------------------------------------------------------- 1-D samplers
+
+
+def test_1d_splat_known_answers():
+    """TII tests/test_image_interpolation_1d.py:66-160: 1 at 14.28 splits
+    0.72 / 0.28 over pixels 14 and 15; 5 at 10.7 (nearest) lands on 11."""
+    img, w = O.insert_into_image_1d(np.array([1.0]), np.array([14.28]), np.zeros(28), interpolation="linear")
+    want = np.zeros(28)
+    want[14], want[15] = 0.72, 0.28
+    np.testing.assert_allclose(img, want, atol=1e-12)
+    np.testing.assert_allclose(w, want, atol=1e-12)
+    img, _ = O.insert_into_image_1d(np.array([5.0]), np.array([10.7]), np.zeros(28), interpolation="nearest")
+    want = np.zeros(28)
+    want[11] = 5.0
+    np.testing.assert_allclose(img, want)
+    # an unknown mode changes nothing; points outside [0, w-1] are dropped
+    img, w = O.insert_into_image_1d(np.ones(3), np.array([1.5, -0.1, 27.01]), np.zeros(28), interpolation="cubic")
+    assert not img.any() and not w.any()
+    img, w = O.insert_into_image_1d(np.ones(3), np.array([1.5, -0.1, 27.01]), np.zeros(28))
+    assert img.sum() == pytest.approx(1.0) and w.sum() == pytest.approx(1.0)
+
+
+def test_1d_samplers_reproduce_lines_and_constants():
+    rng = np.random.default_rng(4)
+    w = 30
+    x = np.arange(w, dtype=np.float64)
+    line = 0.7 * x - 3.0
+    c = rng.uniform(0, w - 1, 500)
+    np.testing.assert_allclose(O.sample_image_1d(line, c, "linear"), 0.7 * c - 3.0, atol=1e-12)
+    np.testing.assert_allclose(O.sample_image_1d(np.full(w, 2.5), c, "cubic"), 2.5, atol=1e-12)
+    ints = rng.integers(0, w, 50).astype(np.float64)
+    img = rng.standard_normal(w)
+    for mode in ("linear", "cubic", "nearest"):
+        np.testing.assert_allclose(O.sample_image_1d(img, ints, mode), img[ints.astype(int)], atol=1e-12)
+        assert not O.sample_image_1d(img, np.array([-0.01, w - 0.99]), mode).any()
+
+
+def test_1d_sampler_is_the_2d_sampler_on_the_repeated_image():
+    """The reference's 1-D sampler is grid_sample on the image repeated to
+    height 2 at y = 0; the 2-D oracle is pinned against its goldens."""
+    rng = np.random.default_rng(5)
+    img = rng.standard_normal((3, 25)) + 1j * rng.standard_normal((3, 25))
+    c = rng.uniform(-2, 26, (4, 60))
+    c[0, :5] = [0.0, 24.0, 12.5, 3.5, 7.5]  # ties for nearest
+    img2 = np.repeat(img[:, None, :], 2, axis=1)
+    c2 = np.stack([np.zeros_like(c), c], -1)
+    for m1, m2 in (("linear", "bilinear"), ("cubic", "bicubic"), ("nearest", "nearest")):
+        np.testing.assert_allclose(O.sample_image_1d(img, c, m1), O.sample_image_2d(img2, c2, m2), atol=1e-12)
