@@ -334,13 +334,15 @@ __device__ __forceinline__ BrickBox brick_box(const SliceGeom &g, const BrickGri
 // is last) are converted and stored (the insertion's x = 0 plane into
 // plane/wplane for the Friedel mirror pass); the upper shell (l = B on an
 // axis where the brick is not last) belongs to the next brick and goes to
-// the halo buffer as fixed-point partial sums.
+// the halo buffer as fixed-point partial sums (unless write_shell is false:
+// a brick no slice touched, whose shell the halo pass then skips).
 template <int BZ, int BY, int BX, int MODE>
 __device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astride, int v,
                                             const BrickBox &bb, const SliceGeom &g, int brick,
                                             float inv_v, int accumulate, f2v *__restrict__ out,
                                             float *__restrict__ wout, f2v *__restrict__ plane,
-                                            float *__restrict__ wplane, u64 *__restrict__ halo) {
+                                            float *__restrict__ wplane, u64 *__restrict__ halo,
+                                            bool write_shell = true) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int SX = C::SX, SY = C::SY, NA = C::NACC, NSH = C::NSH;
   const int lz = v / (SY * SX), ly = (v / SX) % SY, lx = v % SX;
@@ -348,6 +350,7 @@ __device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astri
   const bool shell = (lz == bb.ext[0] + 1 && !bb.last[0]) || (ly == bb.ext[1] + 1 && !bb.last[1]) ||
                      (lx == bb.ext[2] + 1 && !bb.last[2]);
   if (shell) {
+    if (!write_shell) return;
     u64 *hb = halo + (int64_t)brick * NA * NSH;
     const int si = shell_index<BZ, BY, BX>(lz, ly, lx);
 #pragma unroll
@@ -377,13 +380,14 @@ template <int BZ, int BY, int BX, int MODE>
 __device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, const BrickGrid &bg,
                            int brick, float vscale, int accumulate, f2v *__restrict__ out,
                            float *__restrict__ wout, f2v *__restrict__ plane,
-                           float *__restrict__ wplane, u64 *__restrict__ halo, int tid, int nthreads) {
+                           float *__restrict__ wplane, u64 *__restrict__ halo, int tid, int nthreads,
+                           bool write_shell) {
   constexpr int NT = Cfg<BZ, BY, BX, MODE>::NT;
   const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
   const float inv_v = 1.0f / vscale;
   for (int v = tid; v < NT; v += nthreads)
     flush_voxel<BZ, BY, BX, MODE>(acc + v, NT, v, bb, g, brick, inv_v, accumulate, out, wout, plane,
-                                  wplane, halo);
+                                  wplane, halo, write_shell);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,7 +400,8 @@ __device__ __forceinline__ void brick_body(
     const f2v *__restrict__ src, const SliceGeom &g, const BrickGrid &bg, const PlaneRec *__restrict__ pre,
     const PlaneTest *__restrict__ ptest, const Unit &un, int n_rot, f2v *__restrict__ out,
     float *__restrict__ wout, f2v *__restrict__ plane, float *__restrict__ wplane,
-    u64 *__restrict__ segbuf, u64 *__restrict__ halo, const ScaleArgs &sa, int accumulate) {
+    u64 *__restrict__ segbuf, u64 *__restrict__ halo, const ScaleArgs &sa, int accumulate,
+    int *__restrict__ touched) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
@@ -505,6 +510,7 @@ __device__ __forceinline__ void brick_body(
                                fmaxf(fabsf(flo[2]), fabsf(fhi[2])) + 1.0f);
 
   int rd = r_beg, ncross = 0;
+  bool had_items = false;  // this wave listed a slice item (the tile may have been written)
   while (have_cells) {
     // (1a) plane test, 64 rotations per pass, up to 8 passes whose data are
     // loaded at once: rotations whose plane meets the brick's coordinate box
@@ -660,6 +666,7 @@ __device__ __forceinline__ void brick_body(
     if (h) items[rank_below(bh)] = it;
     __builtin_amdgcn_wave_barrier();
     if (nitems == 0) continue;
+    had_items = true;
 
     // (2) inclusive prefix of the items' row counts: item k's end in lane k
     const int ia = wave_incl_scan(lane < nitems ? (items[lane].rows >> 16) : 0);
@@ -865,14 +872,21 @@ __device__ __forceinline__ void brick_body(
 
   if (!SCATTER) return;
   if (__ballot(bad) && lane == 0) atomicOr(sa.flag, 1u);
-  __syncthreads();
+  const bool any = __syncthreads_or((int)had_items) != 0;
   if (nseg > 1) {  // partial sums of a rotation segment: combined by split_combine_kernel
     u64 *dst = segbuf + ((int64_t)un.slot + seg) * (C::NACC * NT);
     for (int v = tid; v < C::NACC * NT; v += C::T) dst[v] = s_tile[v];
     return;
   }
+  // touched[brick]: whether any slice item reached the tile.  An untouched
+  // tile is zero: its shell is not written to the halo buffer (the halo pass
+  // skips it) and, when accumulating, nothing at all is written.
+  if (touched) {
+    if (tid == 0) touched[brick] = any ? 1 : 0;
+    if (!any && accumulate) return;
+  }
   flush_tile<BZ, BY, BX, MODE>(s_tile, g, bg, brick, vscale, accumulate, out, wout, plane, wplane,
-                               halo, tid, C::T);
+                               halo, tid, C::T, any || !touched);
 }
 
 template <int BZ, int BY, int BX, int MODE, bool MASK, int NCH = 1>
@@ -881,10 +895,10 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
     f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
     float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
-    int accumulate) {
+    int accumulate, int *__restrict__ touched) {
   const Unit un = units[blockIdx.x];
   brick_body<BZ, BY, BX, MODE, MASK, NCH>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
-                                     halo, sa, accumulate);
+                                     halo, sa, accumulate, touched);
 }
 
 // Sums the rotation segments of a split brick (integer sums: the result does
@@ -894,11 +908,13 @@ template <int BZ, int BY, int BX, int MODE>
 __device__ __forceinline__ void split_body(
     const SliceGeom &g, const BrickGrid &bg, const SplitRec &sp, const u64 *__restrict__ segbuf,
     const ScaleArgs &sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo, int v) {
+    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo, int v,
+    int *__restrict__ touched) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NT = C::NT, NA = C::NACC;
   if (sa.guard && *sa.flag == 0u) return;
   if (v >= NT) return;
+  if (touched && v == 0) touched[sp.brick] = 1;  // split bricks are the busiest ones
   u64 acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = 0ull;
@@ -918,21 +934,23 @@ template <int BZ, int BY, int BX, int MODE>
 __global__ __launch_bounds__(256) void split_combine_kernel(
     SliceGeom g, BrickGrid bg, const SplitRec *__restrict__ splits, const u64 *__restrict__ segbuf,
     ScaleArgs sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo) {
+    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo, int *__restrict__ touched) {
   const SplitRec sp = splits[blockIdx.x];
   split_body<BZ, BY, BX, MODE>(g, bg, sp, segbuf, sa, accumulate, out, wout, plane, wplane, halo,
-                               (int)(blockIdx.y * 256 + threadIdx.x));
+                               (int)(blockIdx.y * 256 + threadIdx.x), touched);
 }
 
 // Adds the halo partial sums of the lower neighbours to the voxels of a
 // brick's lower shell (l = 0 on an axis where the brick has a neighbour
 // below): a voxel on a face, edge or corner receives 1, 3 or 7 halos.
+// With touched set, the halos of untouched neighbours (not written, zero) are
+// skipped, and so is a voxel that receives none.
 template <int BZ, int BY, int BX, int MODE>
 __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &bg,
                                           const u64 *__restrict__ halo, const ScaleArgs &sa,
                                           f2v *__restrict__ out, float *__restrict__ wout,
                                           f2v *__restrict__ plane, float *__restrict__ wplane, int brick,
-                                          int u) {
+                                          int u, const int *__restrict__ touched) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int SX = C::SX, SY = C::SY, NA = C::NACC, NSH = C::NSH;
   if (sa.guard && *sa.flag == 0u) return;
@@ -971,6 +989,7 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
     }
     if (!owned || amask == 0) return;
     u64 acc[3] = {0ull, 0ull, 0ull};
+    int nread = 0;
     // the nonempty subsets of amask, unrolled so that every load is in flight at once
 #pragma unroll
     for (int sbits = 1; sbits < 8; ++sbits) {
@@ -982,11 +1001,14 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
         nb -= in ? bstride[d] : 0;
         lp[d] = l[d] + (in ? B[d] : 0);
       }
+      if (touched && !touched[nb]) continue;
+      ++nread;
       const int si = shell_index<BZ, BY, BX>(lp[0], lp[1], lp[2]);
       const u64 *hb = halo + (int64_t)nb * NA * NSH;
 #pragma unroll
       for (int a = 0; a < NA; ++a) acc[a] += hb[a * NSH + si];
     }
+    if (touched && nread == 0) return;
     const int gz = bb.lo[0] + l[0], gy = bb.lo[1] + l[1], gx = bb.lo[2] + l[2];
     const f2v val = {(float)(long long)acc[0] * inv_v, (float)(long long)acc[1] * inv_v};
     if (MODE == kInsert) {
@@ -1013,9 +1035,10 @@ __global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGri
                                                            f2v *__restrict__ out,
                                                            float *__restrict__ wout,
                                                            f2v *__restrict__ plane,
-                                                           float *__restrict__ wplane) {
+                                                           float *__restrict__ wplane,
+                                                           const int *__restrict__ touched) {
   halo_body<BZ, BY, BX, MODE>(g, bg, halo, sa, out, wout, plane, wplane, (int)blockIdx.x,
-                              (int)(blockIdx.y * 256 + threadIdx.x));
+                              (int)(blockIdx.y * 256 + threadIdx.x), touched);
 }
 
 // max(|re|, |im|) as float bits into *res, over chunks of 8192 consecutive
@@ -1451,6 +1474,9 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   off = (off + 255) & ~(size_t)255;
   const size_t off_halo = off;
   off += (size_t)n_bricks * C::NACC * C::NSH * sizeof(u64);
+  off = (off + 255) & ~(size_t)255;
+  const size_t off_touched = off;  // per brick: reached by a slice (written by every brick)
+  off += (size_t)n_bricks * sizeof(int);
   Scratch sc(s);
   if (int st = sc.alloc(off)) return st;
   char *base = (char *)sc.p;
@@ -1461,6 +1487,7 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   PlaneTest *ptest = (PlaneTest *)(base + off_test);
   u64 *segbuf = (u64 *)(base + off_seg);
   u64 *halo = (u64 *)(base + off_halo);
+  int *touched = (int *)(base + off_touched);
   if (int st = fill_zero<unsigned>(words, (int64_t)(zero_end / sizeof(unsigned)), s)) return st;
   const int64_t n = n_rot * (int64_t)g.P;
   // accumulate: exact max (the outputs cannot be rewritten); otherwise a
@@ -1492,22 +1519,22 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
       if (g.use_mask)
         hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, true>), dim3((unsigned)pe.n_units), dim3(C::T),
                            0, s, src, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot,
-                           out, wout, plane, wplane, segbuf, halo, sa, accumulate);
+                           out, wout, plane, wplane, segbuf, halo, sa, accumulate, touched);
       else
         hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, false>), dim3((unsigned)pe.n_units), dim3(C::T),
                            0, s, src, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot,
-                           out, wout, plane, wplane, segbuf, halo, sa, accumulate);
+                           out, wout, plane, wplane, segbuf, halo, sa, accumulate, touched);
       if (int st = check_launch("brick_kernel")) return st;
     }
     if (pe.n_splits > 0) {
       hipLaunchKernelGGL((split_combine_kernel<BZ, BY, BX, MODE>),
                          dim3((unsigned)pe.n_splits, (unsigned)((C::NT + 255) / 256)), dim3(256), 0, s, g, bg, (const SplitRec *)pe.splits, (const u64 *)segbuf, sa, accumulate,
-                         out, wout, plane, wplane, halo);
+                         out, wout, plane, wplane, halo, touched);
       if (int st = check_launch("split_combine_kernel")) return st;
     }
     hipLaunchKernelGGL((halo_combine_kernel<BZ, BY, BX, MODE>),
                        dim3((unsigned)n_bricks, (unsigned)((C::NSH + 255) / 256)), dim3(256), 0, s,
-                       g, bg, (const u64 *)halo, sa, out, wout, plane, wplane);
+                       g, bg, (const u64 *)halo, sa, out, wout, plane, wplane, (const int *)touched);
     if (int st = check_launch("halo_combine_kernel")) return st;
   }
   if (MODE == kInsert) {
@@ -1576,7 +1603,7 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_batch_ker
                   __builtin_amdgcn_readfirstlane(bu.slot)};
     const ScaleArgs sa = batch_scale(it, pass);
     brick_body<BZ, BY, BX, MODE, true>(it.src, it.g, it.bg, it.pre, it.ptest, un, it.n_rot, it.out, it.wout,
-                                       it.plane, it.wplane, it.segbuf, it.halo, sa, 0);
+                                       it.plane, it.wplane, it.segbuf, it.halo, sa, 0, nullptr);
     __syncthreads();  // the tile is reused by the next unit
   }
 }
@@ -1612,7 +1639,7 @@ __global__ __launch_bounds__(256) void split_batch_kernel(const BatchItemDev *__
   const BatchItemDev &it = items[sp.item];
   const SplitRec rec{sp.brick, sp.seg, sp.slot};
   split_body<BZ, BY, BX, MODE>(it.g, it.bg, rec, it.segbuf, batch_scale(it, pass), 0, it.out, it.wout,
-                               it.plane, it.wplane, it.halo, (int)(blockIdx.y * 256 + threadIdx.x));
+                               it.plane, it.wplane, it.halo, (int)(blockIdx.y * 256 + threadIdx.x), nullptr);
 }
 
 template <int BZ, int BY, int BX, int MODE>
@@ -1621,7 +1648,7 @@ __global__ __launch_bounds__(256) void halo_batch_kernel(const BatchItemDev *__r
   const int2 b = bricks[blockIdx.x];  // item, brick
   const BatchItemDev &it = items[b.x];
   halo_body<BZ, BY, BX, MODE>(it.g, it.bg, it.halo, batch_scale(it, pass), it.out, it.wout, it.plane,
-                              it.wplane, b.y, (int)(blockIdx.y * 256 + threadIdx.x));
+                              it.wplane, b.y, (int)(blockIdx.y * 256 + threadIdx.x), nullptr);
 }
 
 __global__ __launch_bounds__(256) void mirror_batch_kernel(const BatchItemDev *__restrict__ items,
@@ -2002,12 +2029,12 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)pe.n_units), dim3(T),
                        0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot, out,
                        (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (u64 *)nullptr, sa, 0);
+                       (u64 *)nullptr, sa, 0, (int *)nullptr);
   else
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)pe.n_units),
                        dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
-                       (u64 *)nullptr, (u64 *)nullptr, sa, 0);
+                       (u64 *)nullptr, (u64 *)nullptr, sa, 0, (int *)nullptr);
   return check_launch("brick_kernel");
 }
 
@@ -2041,12 +2068,12 @@ int engine_extract2(const SliceGeom &g, const f2v *vol, const float *rot, int64_
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true, 2>), dim3((unsigned)pe.n_units), dim3(T),
                        0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (u64 *)nullptr, sa, 0);
+                       (u64 *)nullptr, sa, 0, (int *)nullptr);
   else
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false, 2>), dim3((unsigned)pe.n_units), dim3(T),
                        0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (u64 *)nullptr, sa, 0);
+                       (u64 *)nullptr, sa, 0, (int *)nullptr);
   return check_launch("brick_kernel");
 }
 
