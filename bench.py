@@ -13,11 +13,13 @@ roofline and the FP64 oracle timed on the host CPU on a bounded sample.  The
 plain-torch comparator on the same GPU is timed with --torch.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu] [--copy-probe MIB]
-                      [--workload slices3d|lines2d]
+                      [--workload slices3d|lines2d|insert3d]
 
 ``--workload lines2d`` measures the 2-D analogue instead (central lines of a
 (1024, 513) complex64 2-D rfft for 65536 rotations, forward + image
-gradient) and prints its own JSON line; the default is the headline above.
+gradient) and ``--workload insert3d`` the insertion (4096 N(0,1) complex64
+slices into (256, 256, 129), and 61 tilts of 512^2 into (512, 512, 257));
+each prints its own JSON line.  The default is the headline above.
 """
 
 from __future__ import annotations
@@ -51,8 +53,9 @@ def parse():
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--gather", choices=["auto", "pixel", "brick"], default="auto",
                     help="extraction gather kernel (default: auto)")
-    ap.add_argument("--workload", choices=["slices3d", "lines2d"], default="slices3d",
-                    help="slices3d: the headline (default); lines2d: the 2-D central-line analogue")
+    ap.add_argument("--workload", choices=["slices3d", "lines2d", "insert3d"], default="slices3d",
+                    help="slices3d: the headline (default); lines2d: the 2-D central-line analogue; "
+                         "insert3d: central-slice insertion (4096 slices into 256^3, 61 tilts into 512^3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
@@ -358,6 +361,97 @@ def copy_bandwidth(device, mib=1024, reps=5):
     return 2 * (mib << 20) / ms / 1e6
 
 
+def run_insert3d(args, device):
+    """Central-slice insertion as ``insert_central_slices_rfft_3d`` runs it
+    (volume and weights written): 4096 N(0,1) complex64 slices with uniform
+    rotations into (256, 256, 129), and 61 tilts (-60..60 step 2 about y) of
+    512^2 into (512, 512, 257)."""
+    from teamtomo_amd import _lib
+    from teamtomo_amd.fourier_slice import _prepare_rotations, _scatter_flags
+    from teamtomo_amd.synthetic import uniform_rotations
+
+    def case(n, R, seed):
+        g = torch.Generator(device=device).manual_seed(seed)
+        Rz = _prepare_rotations(R, device, False, None).reshape(-1, 3, 3).contiguous()
+        k = Rz.shape[0]
+        S = torch.complex(torch.randn((k, n, n // 2 + 1), generator=g, device=device),
+                          torch.randn((k, n, n // 2 + 1), generator=g, device=device)).contiguous()
+        vol = torch.empty((n, n, n // 2 + 1), dtype=torch.complex64, device=device)
+        w = torch.empty((n, n, n // 2 + 1), dtype=torch.float32, device=device)
+        st = _lib.stream_of(S)
+
+        def run(S=S, Rz=Rz, k=k):
+            _lib.call("tt_insert_c64", _lib.ptr(S), n, n, n, _lib.ptr(Rz), k, -1.0, _lib.ptr(vol), _lib.ptr(w),
+                      _scatter_flags(), st)
+
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        P, V = n * (n // 2 + 1), n * n * (n // 2 + 1)
+        nbytes = k * (8 * P + 36) + 12 * V  # slices + rotations read, volume + weights written
+        return {"ms": ms, "bytes": nbytes, "GBs": nbytes / ms / 1e6, "slices": k, "volume": [n, n, n // 2 + 1],
+                "slices_per_s": k / (ms / 1e3)}, (S, Rz, vol, w, run)
+
+    c2, (S2, Rz2, vol2, w2, run2) = case(N, uniform_rotations(N_ROT, seed=0, device=device), 3)
+    t = np.deg2rad(np.arange(-60, 61, 2, dtype=np.float64))
+    Ry = np.zeros((t.size, 3, 3), dtype=np.float32)
+    Ry[:, 0, 0], Ry[:, 0, 2], Ry[:, 1, 1], Ry[:, 2, 0], Ry[:, 2, 2] = np.cos(t), np.sin(t), 1.0, -np.sin(t), np.cos(t)
+    c5, _ = case(512, torch.from_numpy(Ry).to(device), 5)
+    line = {
+        "metric": "slices/s central-slice insertion at 256^3 c64 (4096 slices); % HBM roofline",
+        "value": c2["slices_per_s"], "unit": "slices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": c2["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+        "data": "synthetic (N(0,1) c64 slices, uniform SO(3) rotations; 61 tilts about y for the 512^3 case)",
+        "config": {"workload": "insert_central_slices_rfft_3d, 4096 slices (256, 129) into (256, 256, 129) c64 + "
+                               "f32 weights", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "insert", "achieved": c2["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": c2["GBs"] / HBM_PEAK_GBS, "traffic": None},
+        "kernels": {"insert_256": c2, "insert_512_tilts": dict(c5, frac=c5["GBs"] / HBM_PEAK_GBS)},
+    }
+    if not args.no_cpu:
+        from oracle import c_oracle as C
+        from oracle import fourier_slice as O
+
+        Sn = S2.cpu().numpy()
+        Rn = O.prepare_rotations(Rz2.cpu().numpy(), True, None)
+        C.insert(Sn[:2], (N, N, N), Rn[:2])
+        done, dt, first = 0, 0.0, None
+        while done < N_ROT and dt < 10.0:
+            sl = slice(done, min(done + 256, N_ROT))
+            t0 = time.perf_counter()
+            res = C.insert(Sn[sl], (N, N, N), Rn[sl])
+            dt += time.perf_counter() - t0
+            if first is None:
+                first = (sl, res)
+            done = sl.stop
+        sl, (rv, rw) = first
+        gv = torch.empty_like(vol2)
+        gw = torch.empty_like(w2)
+        k = sl.stop - sl.start
+        _lib.call("tt_insert_c64", _lib.ptr(S2[sl].contiguous()), N, N, N, _lib.ptr(Rz2[sl].contiguous()), k, -1.0,
+                  _lib.ptr(gv), _lib.ptr(gw), _scatter_flags(), _lib.stream_of(S2))
+        torch.cuda.synchronize()
+
+        def cmp(a, b):
+            return {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                    "max_abs_over_max": float(np.max(np.abs(a - b)) / np.max(np.abs(b)))}
+
+        line["cpu_baseline"] = {
+            "value": done / dt, "unit": "slices/s", "cores": C.threads(), "kind": "port",
+            "sample": f"FP64 C oracle (oracle/c/tt_oracle.c, OpenMP), insertion of {done} of the {N_ROT} slices "
+                      f"at {N}^3 in {dt:.1f} s",
+            "parity": {"rotations": [sl.start, sl.stop], "volume": cmp(gv.cpu().numpy(), rv),
+                       "weights": cmp(gw.cpu().numpy(), rw), "bar": "rel_l2 <= 1e-4"}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -377,6 +471,9 @@ def main():
         return
     if args.workload == "lines2d":
         run_lines2d(args, device)
+        return
+    if args.workload == "insert3d":
+        run_insert3d(args, device)
         return
     from teamtomo_amd import _lib, set_gather_path
 
