@@ -333,12 +333,26 @@ def run_lines2d(args, device):
         gn = gL[:512].cpu().numpy()
         O2.extract_central_lines(Fn, (n, n), Rn[:2])
         t0 = time.perf_counter()
-        O2.extract_central_lines(Fn, (n, n), Rn)
-        O2.extract_central_lines_adjoint(gn, Fn.shape, (n, n), Rn)
+        ref_l = O2.extract_central_lines(Fn, (n, n), Rn)
+        ref_g = O2.extract_central_lines_adjoint(gn, Fn.shape, (n, n), Rn)
         dt = time.perf_counter() - t0
+        # the same 512 lines on the GPU: the timed step's forward output, and
+        # an image gradient of those lines alone
+        gI1 = torch.zeros_like(F)
+        _lib.call("tt_extract2d_bwd_c64", _lib.ptr(gL[:512].contiguous()), _lib.ptr(F), n, wr, n, n,
+                  _lib.ptr(Ryx[:512].contiguous()), 512, -1.0, _lib.ptr(gI1), None, st)
+        torch.cuda.synchronize()
+
+        def cmp(a, b):
+            return {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                    "max_abs_over_max": float(np.max(np.abs(a - b)) / np.max(np.abs(b)))}
+
         line["cpu_baseline"] = {"value": 512 / dt, "unit": "lines/s", "cores": 1, "kind": "port",
                                 "sample": f"FP64 NumPy oracle (oracle/slice_2d.py), forward + image adjoint of "
-                                          f"512 of the {L2D_ROT} lines in {dt:.2f} s"}
+                                          f"512 of the {L2D_ROT} lines in {dt:.2f} s",
+                                "parity": {"rotations": [0, 512], "lines": cmp(out[:512].cpu().numpy(), ref_l),
+                                           "image_gradient": cmp(gI1.cpu().numpy(), ref_g),
+                                           "bar": "rel_l2 <= 1e-5"}}
     print(json.dumps(line), flush=True)
 
 
