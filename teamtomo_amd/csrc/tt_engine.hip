@@ -1277,7 +1277,11 @@ struct PlanEntry {
 std::mutex g_plan_mutex;
 std::map<PlanKey, PlanEntry> g_plans;
 
-constexpr int kTargetSlots = 256 * 2 * 4;  // CUs x resident blocks x units per block slot
+// Work per unit is about the total estimate / kTargetSlots: two units per
+// resident block slot (256 CUs x 2 blocks).  Measured at 256^3 x 4096
+// (extract + adjoint): 512 -> 2.021 ms, 1024 -> 2.009, 2048 -> 2.049,
+// 4096 -> 2.343 (smaller units re-read their tiles; larger ones unbalance).
+constexpr int kTargetSlots = 256 * 2 * 2;
 
 // Units of the bricks, heaviest first.  The work of a brick is estimated for
 // uniformly oriented planes: a plane through DC crosses a brick of
