@@ -230,6 +230,63 @@ def cpu_baseline(V, R, gS, step, min_s=10.0, chunk=256):
             "parity": parity_check(step, *first)}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def torch_cpu_baseline(min_s=5.0):
+    """The plain-torch restatement (torch_extract / torch_adjoint above) on the
+    host CPU with torch's intra-op threads: config 1 in full (64^3 N(0,1)
+    volume, 100 rotations) and the 256^3 workload on rotation chunks of 16
+    until `min_s` seconds have passed, both reported per slice; plus the FP64
+    C oracle on config 1."""
+    from oracle import c_oracle as C
+    from oracle import fourier_slice as O
+    from teamtomo_amd.synthetic import blob_volume, uniform_rotations, volume_rfft
+
+    cpu = torch.device("cpu")
+    out = {"threads": torch.get_num_threads(), "cpu": _cpu_model()}
+    n1 = 64
+    V1 = volume_rfft(torch.randn((n1,) * 3, generator=torch.Generator().manual_seed(1))).contiguous()
+    R1 = uniform_rotations(100, seed=1, device=cpu)
+    g1 = torch.complex(torch.randn((100, n1, n1 // 2 + 1), generator=torch.Generator().manual_seed(2)),
+                       torch.randn((100, n1, n1 // 2 + 1), generator=torch.Generator().manual_seed(3)))
+    torch_extract(V1, (n1,) * 3, R1[:2])
+    t0 = time.perf_counter()
+    torch_extract(V1, (n1,) * 3, R1)
+    torch_adjoint(g1, torch.zeros_like(V1), (n1,) * 3, R1)
+    dt = time.perf_counter() - t0
+    out["config1_64cubed_100rot"] = {"value": 100 / dt, "unit": "slices/s", "seconds": dt}
+    Rn = O.prepare_rotations(R1.numpy(), False, 1e-8)
+    Vn = V1.numpy().astype(np.complex128)
+    t0 = time.perf_counter()
+    C.extract(Vn, (n1,) * 3, Rn)
+    C.extract_adjoint(g1.numpy(), Vn.shape, (n1,) * 3, Rn)
+    dt = time.perf_counter() - t0
+    out["config1_c_oracle"] = {"value": 100 / dt, "unit": "slices/s", "seconds": dt, "threads": C.threads()}
+    V2 = volume_rfft(blob_volume(N, 200, seed=2, device=cpu)).contiguous()
+    R2 = uniform_rotations(N_ROT, seed=0, device=cpu)
+    gen = torch.Generator().manual_seed(4)
+    done, dt = 0, 0.0
+    while dt < min_s and done < N_ROT:
+        k = 16
+        gS = torch.complex(torch.randn((k, N, N // 2 + 1), generator=gen), torch.randn((k, N, N // 2 + 1), generator=gen))
+        t0 = time.perf_counter()
+        torch_extract(V2, (N, N, N), R2[done:done + k])
+        torch_adjoint(gS, torch.zeros_like(V2), (N, N, N), R2[done:done + k])
+        dt += time.perf_counter() - t0
+        done += k
+    out["config2_sample"] = {"value": done / dt, "unit": "slices/s", "rotations": done, "seconds": dt}
+    return out
+
+
 def parity_check(step, sl, S_ref, G_ref):
     """Relative L2 and max |error| (over max |reference|) of the GPU's slices
     and volume gradient against the FP64 oracle for the rotations in `sl`."""
@@ -568,6 +625,7 @@ def main():
             line["torch_same_gpu"] = {"error": repr(exc)[:300]}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(V, R, gS, step)
+        line["cpu_baseline"]["torch_cpu"] = torch_cpu_baseline()
     print(json.dumps(line), flush=True)
 
 
