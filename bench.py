@@ -13,13 +13,15 @@ roofline and the FP64 oracle timed on the host CPU on a bounded sample.  The
 plain-torch comparator on the same GPU is timed with --torch.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu] [--copy-probe MIB]
-                      [--workload slices3d|lines2d|insert3d]
+                      [--workload slices3d|lines2d|insert3d|batch]
 
 ``--workload lines2d`` measures the 2-D analogue instead (central lines of a
 (1024, 513) complex64 2-D rfft for 65536 rotations, forward + image
 gradient) and ``--workload insert3d`` the insertion (4096 N(0,1) complex64
-slices into (256, 256, 129), and 61 tilts of 512^2 into (512, 512, 257));
-each prints its own JSON line.  The default is the headline above.
+slices into (256, 256, 129), and 61 tilts of 512^2 into (512, 512, 257)) and
+``--workload batch`` 32 volumes of mixed size as two graph-captured batched
+launches against a loop of one-call extract + insert; each prints its own
+JSON line.  The default is the headline above.
 """
 
 from __future__ import annotations
@@ -53,9 +55,10 @@ def parse():
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--gather", choices=["auto", "pixel", "brick"], default="auto",
                     help="extraction gather kernel (default: auto)")
-    ap.add_argument("--workload", choices=["slices3d", "lines2d", "insert3d"], default="slices3d",
+    ap.add_argument("--workload", choices=["slices3d", "lines2d", "insert3d", "batch"], default="slices3d",
                     help="slices3d: the headline (default); lines2d: the 2-D central-line analogue; "
-                         "insert3d: central-slice insertion (4096 slices into 256^3, 61 tilts into 512^3)")
+                         "insert3d: central-slice insertion (4096 slices into 256^3, 61 tilts into 512^3); "
+                         "batch: 32 volumes of mixed size, batched extract + insert against one-call loops")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
@@ -523,6 +526,93 @@ def run_insert3d(args, device):
     print(json.dumps(line), flush=True)
 
 
+def run_batch(args, device):
+    """32 volumes with box sizes from {64, 96, 128, 160, 192} and rotation
+    counts log-uniform in [16, 2048] (seed 4): extraction and insertion of
+    every volume as two graph-captured SliceBatch launches, against a loop of
+    one-call extract + insert per volume."""
+    import math
+
+    import teamtomo_amd as tt
+    from teamtomo_amd.synthetic import blob_volume, uniform_rotations, volume_rfft
+
+    rng = np.random.default_rng(4)
+    sizes = rng.choice([64, 96, 128, 160, 192], size=32)
+    counts = np.floor(np.exp(rng.uniform(math.log(16), math.log(2048), size=32))).astype(int)
+    vols = [volume_rfft(blob_volume(int(n), 40, seed=100 + k, device=device)).contiguous() for k, n in enumerate(sizes)]
+    rots = [uniform_rotations(int(c), 200 + k, device=device) for k, c in enumerate(counts)]
+    shapes = [(int(n),) * 3 for n in sizes]
+    g = torch.Generator(device=device).manual_seed(5)
+    slices = [torch.complex(torch.randn((int(c), int(n), int(n) // 2 + 1), generator=g, device=device),
+                            torch.randn((int(c), int(n), int(n) // 2 + 1), generator=g, device=device)).contiguous()
+              for n, c in zip(sizes, counts)]
+    bx = tt.SliceBatch("extract", vols, rots, autotune=True)
+    bi = tt.SliceBatch("insert", slices, rots, shapes=shapes, autotune=True)
+
+    def capture(fn):
+        graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            fn()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            fn()
+        torch.cuda.synchronize()
+        return graph
+
+    ge, gi = capture(bx.run), capture(bi.run)
+
+    def batched():
+        ge.replay()
+        gi.replay()
+
+    def loop():
+        for v, R, sl, sh in zip(vols, rots, slices, shapes):
+            tt.extract_central_slices_rfft_3d(v, sh, R)
+            tt.insert_central_slices_rfft_3d(sl, sh, R)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    ms_b, ms_l = timed(batched), timed(loop)
+    # parity of the batched outputs with the one-call results
+    worst = 0.0
+    for k in range(len(vols)):
+        a = tt.extract_central_slices_rfft_3d(vols[k], shapes[k], rots[k])
+        worst = max(worst, float((a - bx.outputs[k]).abs().max()) / max(float(a.abs().max()), 1e-30))
+        va, wa = tt.insert_central_slices_rfft_3d(slices[k], shapes[k], rots[k])
+        vb, wb = bi.outputs[k]
+        worst = max(worst, float(torch.linalg.vector_norm(va - vb) / torch.linalg.vector_norm(va)),
+                    float(torch.linalg.vector_norm(wa - wb) / torch.linalg.vector_norm(wa)))
+    n_rot = int(counts.sum())
+    line = {
+        "metric": "slices/s batched extract + insert over 32 volumes of mixed box size (graph replay)",
+        "value": 2 * n_rot / (ms_b / 1e3), "unit": "slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_b, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64",
+        "data": "synthetic (seeded blob volumes, uniform SO(3) rotations, N(0,1) c64 slices)",
+        "config": {"workload": f"32 volumes, boxes {sorted(set(int(v) for v in sizes))}, {n_rot} rotations, "
+                               "extract + insert", "parallelism": "single GPU"},
+        "one_call_loop_ms": ms_l, "speedup_vs_loop": ms_l / ms_b,
+        "tiles": {"extract": sorted(set(bx.tile_shapes)), "insert": sorted(set(bi.tile_shapes))},
+        "parity_vs_one_call": {"worst_relative": worst, "bar": "<= 1e-5"},
+    }
+    bx.close()
+    bi.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -545,6 +635,9 @@ def main():
         return
     if args.workload == "insert3d":
         run_insert3d(args, device)
+        return
+    if args.workload == "batch":
+        run_batch(args, device)
         return
     from teamtomo_amd import _lib, set_gather_path
 
