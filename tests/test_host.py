@@ -1,6 +1,8 @@
 """Host-side logic of the drop-in layer that runs without a GPU: argument
 validation and errors mirror the reference, and nothing falls back to the CPU."""
 
+import os
+
 import pytest
 import torch
 
@@ -162,3 +164,33 @@ def test_fourier_shift_refuses_cpu_and_checks_shapes():
     assert _result_dtype(torch.zeros(1, dtype=c64), torch.zeros(1, dtype=torch.float64)) == c128
     assert _result_dtype(torch.zeros(1, dtype=c128), torch.zeros(1)) == c128
     assert _result_dtype(torch.zeros(1, dtype=torch.float32), torch.zeros(1)) == c64
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N>1 exits with a message (one GPU per box; sharding is
+    designed, not launched), before touching a GPU."""
+    import json
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
+
+
+def test_bench_algorithmic_bytes():
+    """The roofline's algorithmic bytes at 256^3 x 4096: 8 B per slice pixel
+    + 36 B per rotation + 8 B per volume voxel, for each direction."""
+    import importlib.util
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    ext, adj = mod.algorithmic_bytes()
+    P, V = 256 * 129, 256 * 256 * 129
+    assert ext == adj == 4096 * (8 * P + 36) + 8 * V
+    assert abs(ext / 1e9 - 1.150) < 1e-3
