@@ -13,15 +13,17 @@ roofline and the FP64 oracle timed on the host CPU on a bounded sample.  The
 plain-torch comparator on the same GPU is timed with --torch.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--torch] [--no-cpu] [--copy-probe MIB]
-                      [--workload slices3d|lines2d|insert3d|batch]
+                      [--workload slices3d|lines2d|insert3d|batch|project]
 
 ``--workload lines2d`` measures the 2-D analogue instead (central lines of a
 (1024, 513) complex64 2-D rfft for 65536 rotations, forward + image
 gradient) and ``--workload insert3d`` the insertion (4096 N(0,1) complex64
 slices into (256, 256, 129), and 61 tilts of 512^2 into (512, 512, 257)) and
 ``--workload batch`` 32 volumes of mixed size as two graph-captured batched
-launches against a loop of one-call extract + insert; each prints its own
-JSON line.  The default is the headline above.
+launches against a loop of one-call extract + insert, and
+``--workload project`` project_3d_to_2d / backproject_2d_to_3d end to end at
+128^3 with padding 2 and 512 poses; each prints its own JSON line.  The
+default is the headline above.
 """
 
 from __future__ import annotations
@@ -55,10 +57,12 @@ def parse():
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--gather", choices=["auto", "pixel", "brick"], default="auto",
                     help="extraction gather kernel (default: auto)")
-    ap.add_argument("--workload", choices=["slices3d", "lines2d", "insert3d", "batch"], default="slices3d",
+    ap.add_argument("--workload", choices=["slices3d", "lines2d", "insert3d", "batch", "project"],
+                    default="slices3d",
                     help="slices3d: the headline (default); lines2d: the 2-D central-line analogue; "
                          "insert3d: central-slice insertion (4096 slices into 256^3, 61 tilts into 512^3); "
-                         "batch: 32 volumes of mixed size, batched extract + insert against one-call loops")
+                         "batch: 32 volumes of mixed size, batched extract + insert against one-call loops; "
+                         "project: project_3d_to_2d / backproject_2d_to_3d end to end (128^3, pad 2, 512 poses)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
@@ -613,6 +617,69 @@ def run_batch(args, device):
     print(json.dumps(line), flush=True)
 
 
+def torch_project(volume, R, pad):
+    """The reference's projection (project.py:44-93) composed from torch
+    operations, with the plain-torch gather above in chunks of 64 poses."""
+    import torch.nn.functional as Fn
+
+    d = volume.shape[-1]
+    vol = Fn.pad(volume, [pad] * 6)
+    n = vol.shape[-1]
+    f = torch.fft.fftshift(torch.fft.fftfreq(n, device=vol.device))
+    r = torch.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2)
+    vol = vol * torch.special.sinc(r) ** 2
+    dft = torch.fft.fftshift(vol, dim=(-3, -2, -1))
+    dft = torch.fft.fftshift(torch.fft.rfftn(dft, dim=(-3, -2, -1)), dim=(-3, -2))
+    sl = torch.cat([torch_extract(dft, (n, n, n), R[k:k + 64]) for k in range(0, R.shape[0], 64)])
+    sl = torch.fft.ifftshift(sl, dim=(-2,))
+    proj = torch.fft.ifftshift(torch.fft.irfftn(sl, dim=(-2, -1)), dim=(-2, -1))
+    return proj[..., pad:pad + d, pad:pad + d]
+
+
+def run_project(args, device):
+    """project_3d_to_2d and backproject_2d_to_3d end to end at config 3's
+    size (128^3 blob volume, pad_factor 2 -> 256^3, 512 poses): the fused
+    pre/post kernels, torch's FFTs and the brick gather / scatter; the
+    projection also against its torch composition on the same GPU."""
+    import teamtomo_amd as tt
+    from teamtomo_amd.synthetic import blob_volume, uniform_rotations
+
+    vol = blob_volume(128, 200, seed=2, device=device)
+    R = uniform_rotations(512, seed=0, device=device)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    with torch.no_grad():
+        proj = tt.project_3d_to_2d(vol, R, pad_factor=2.0)
+        ms_p = timed(lambda: tt.project_3d_to_2d(vol, R, pad_factor=2.0))
+        ms_b = timed(lambda: tt.backproject_2d_to_3d(proj, R, pad_factor=2.0))
+        ref = torch_project(vol, R, 64)
+        ms_t = timed(lambda: torch_project(vol, R, 64))
+    rel = float(torch.linalg.vector_norm(proj - ref) / torch.linalg.vector_norm(ref))
+    line = {
+        "metric": "poses/s project_3d_to_2d at 128^3 (pad 2 -> 256^3), 512 poses",
+        "value": 512 / (ms_p / 1e3), "unit": "poses/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_p, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/c64",
+        "data": "synthetic (seeded 200-blob 128^3 volume, uniform SO(3) rotations)",
+        "config": {"workload": "project_3d_to_2d 128^3 pad_factor 2, 512 poses; backproject_2d_to_3d of the result",
+                   "parallelism": "single GPU"},
+        "backproject_ms": ms_b,
+        "torch_same_gpu": {"project_ms": ms_t, "speedup": ms_t / ms_p, "rel_l2_vs_ours": rel,
+                           "note": "torch composition with the plain advanced-indexing gather"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -638,6 +705,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args, device)
+        return
+    if args.workload == "project":
+        run_project(args, device)
         return
     from teamtomo_amd import _lib, set_gather_path
 
