@@ -67,6 +67,21 @@ int tt_abi_version(void);
 int tt_set_gather_path(int path);
 
 /*
+ * Rotation preparation: out (n, 3, 3) float32 = the (n, 3, 3) float32 or
+ * float64 matrices cast to float32, entries with |R| < rot_tolerance set to
+ * zero (rot_tolerance < 0: no clamp; the comparison in float32), and both
+ * matrix axes reversed when flip != 0 (xyz -> zyx).  This is the form every
+ * rot_zyx argument below takes.  Replaces the preparation steps of
+ * torch-fourier-slice/.../slice_extraction/_extract_central_slices_rfft_3d.py:147-154
+ * and .../slice_insertion/_insert_central_slices_rfft_3d.py:16,66-67, without
+ * the reference's in-place clamp of the caller's tensor.
+ */
+int tt_prepare_rotations_f32(const float *rot, int64_t n, int flip, double rot_tolerance, float *out,
+                             void *stream);
+int tt_prepare_rotations_f64(const double *rot, int64_t n, int flip, double rot_tolerance, float *out,
+                             void *stream);
+
+/*
  * Central-slice extraction (the gather).
  * Replaces extract_central_slices_rfft_3d
  * (torch-fourier-slice/.../slice_extraction/_extract_central_slices_rfft_3d.py:84-222)

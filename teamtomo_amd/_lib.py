@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -64,6 +64,8 @@ _SIGNATURES = {
     "tt_affine2d_f32": [_p, _i, _i, _i, _i, _p, _l, _i, _i, _i, _i, _i, _p, _p],
     "tt_crop3d_f32": [_p, _i, _i, _i, _p, _l, _i, _p, _p, _p],
     "tt_crop2d_f32": [_p, _i, _i, _i, _p, _l, _i, _p, _p, _p],
+    "tt_prepare_rotations_f32": [_p, _l, _i, _d, _p, _p],
+    "tt_prepare_rotations_f64": [_p, _l, _i, _d, _p, _p],
     "tt_fourier_shift_c64": [_p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p],
     "tt_fourier_shift_c128": [_p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p],
     "tt_fourier_shift_bwd_c64": [_p, _p, _l, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _p],

@@ -347,7 +347,7 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 10; }
+int tt_abi_version(void) { return 11; }
 
 static int g_gather_path = TT_GATHER_AUTO;
 

@@ -65,6 +65,17 @@ def _prepare_rotations(rotation_matrices, device, zyx_matrices, rot_tolerance):
     slice_extraction/_extract_central_slices_rfft_3d.py:147-154;
     slice_insertion/_insert_central_slices_rfft_3d.py:16,66-67.
     """
+    Rin = rotation_matrices
+    if (Rin.device == device and device.type == "cuda" and Rin.dtype in (torch.float32, torch.float64)
+            and Rin.ndim >= 2 and tuple(Rin.shape[-2:]) == (3, 3)
+            and not (torch.is_grad_enabled() and Rin.requires_grad)):
+        # one kernel for the cast, the clamp and the flip (no autograd needed)
+        src = Rin.contiguous()
+        out = torch.empty(Rin.shape, dtype=torch.float32, device=device)
+        fn = "tt_prepare_rotations_f32" if src.dtype == torch.float32 else "tt_prepare_rotations_f64"
+        _lib.call(fn, _lib.ptr(src), src.numel() // 9, int(not zyx_matrices),
+                  -1.0 if rot_tolerance is None else float(rot_tolerance), _lib.ptr(out), _lib.stream_of(out))
+        return out
     R = rotation_matrices.to(device=device, dtype=torch.float32)
     if rot_tolerance is not None:
         R = torch.where(torch.abs(R) < rot_tolerance, torch.zeros((), dtype=R.dtype, device=device), R)

@@ -127,6 +127,35 @@ def test_empty_batches_give_zero_outputs(device):
     assert tt.sample_image_3d(torch.randn(4, 4, 4, device=device), torch.zeros(0, 3, device=device)).shape == (0,)
 
 
+def test_prepare_rotations_kernel_matches_torch_steps(device):
+    """tt_prepare_rotations_*: float32 cast, |R| < tol -> 0 (in float32) and
+    the xyz -> zyx flip in one pass, equal to the reference's torch steps
+    (TFS extract :147-154) for float32 and float64 input, any leading dims."""
+    from teamtomo_amd.fourier_slice import _prepare_rotations
+
+    g = torch.Generator().manual_seed(31)
+    R = torch.randn(4, 5, 3, 3, generator=g, dtype=torch.float64)
+    R[0, 0, 0, 1] = 5e-9  # below the tolerance
+    R[0, 1, 2, 2] = -2e-8  # above it
+    R[1, 0, 1, 0] = 1e-8 * (1 - 1e-9)  # rounds to float32(1e-8): not below it in float32
+    for dt in (torch.float32, torch.float64):
+        for zyx in (False, True):
+            for tol in (None, 1e-8):
+                Rd = R.to(dt).to(device)
+                got = _prepare_rotations(Rd, device, zyx, tol)
+                want = R.to(dt).to(torch.float32)
+                if tol is not None:
+                    want = torch.where(torch.abs(want) < tol, torch.zeros(()), want)
+                if not zyx:
+                    want = torch.flip(want, dims=(-2, -1))
+                assert got.dtype == torch.float32 and got.shape == R.shape
+                assert torch.equal(got.cpu(), want), (dt, zyx, tol)
+    # a rotation that requires grad takes the differentiable torch steps
+    Rg = R.to(torch.float32).to(device).requires_grad_(True)
+    _prepare_rotations(Rg, device, False, 1e-8).sum().backward()
+    assert Rg.grad is not None
+
+
 # ------------------------------------------------------------------ insertion
 def test_insert_config1_vs_oracle_and_golden(device):
     G = golden("extract_insert_64")
