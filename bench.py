@@ -250,10 +250,10 @@ def _cpu_model():
 
 def torch_cpu_baseline(min_s=5.0):
     """The plain-torch restatement (torch_extract / torch_adjoint above) on the
-    host CPU with torch's intra-op threads: config 1 in full (64^3 N(0,1)
-    volume, 100 rotations) and the 256^3 workload on rotation chunks of 16
+    host CPU with torch's intra-op threads: a 64^3 N(0,1) volume with 100
+    rotations in full and the 256^3 workload on rotation chunks of 16
     until `min_s` seconds have passed, both reported per slice; plus the FP64
-    C oracle on config 1."""
+    C oracle on the 64^3 case."""
     from oracle import c_oracle as C
     from oracle import fourier_slice as O
     from teamtomo_amd.synthetic import blob_volume, uniform_rotations, volume_rfft
@@ -270,14 +270,14 @@ def torch_cpu_baseline(min_s=5.0):
     torch_extract(V1, (n1,) * 3, R1)
     torch_adjoint(g1, torch.zeros_like(V1), (n1,) * 3, R1)
     dt = time.perf_counter() - t0
-    out["config1_64cubed_100rot"] = {"value": 100 / dt, "unit": "slices/s", "seconds": dt}
+    out["vol64_100rot"] = {"value": 100 / dt, "unit": "slices/s", "seconds": dt}
     Rn = O.prepare_rotations(R1.numpy(), False, 1e-8)
     Vn = V1.numpy().astype(np.complex128)
     t0 = time.perf_counter()
     C.extract(Vn, (n1,) * 3, Rn)
     C.extract_adjoint(g1.numpy(), Vn.shape, (n1,) * 3, Rn)
     dt = time.perf_counter() - t0
-    out["config1_c_oracle"] = {"value": 100 / dt, "unit": "slices/s", "seconds": dt, "threads": C.threads()}
+    out["vol64_100rot_c_oracle"] = {"value": 100 / dt, "unit": "slices/s", "seconds": dt, "threads": C.threads()}
     V2 = volume_rfft(blob_volume(N, 200, seed=2, device=cpu)).contiguous()
     R2 = uniform_rotations(N_ROT, seed=0, device=cpu)
     gen = torch.Generator().manual_seed(4)
@@ -290,7 +290,7 @@ def torch_cpu_baseline(min_s=5.0):
         torch_adjoint(gS, torch.zeros_like(V2), (N, N, N), R2[done:done + k])
         dt += time.perf_counter() - t0
         done += k
-    out["config2_sample"] = {"value": done / dt, "unit": "slices/s", "rotations": done, "seconds": dt}
+    out["vol256_4096rot_sample"] = {"value": done / dt, "unit": "slices/s", "rotations": done, "seconds": dt}
     return out
 
 
@@ -637,8 +637,8 @@ def torch_project(volume, R, pad):
 
 
 def run_project(args, device):
-    """project_3d_to_2d and backproject_2d_to_3d end to end at config 3's
-    size (128^3 blob volume, pad_factor 2 -> 256^3, 512 poses): the fused
+    """project_3d_to_2d and backproject_2d_to_3d end to end at 128^3
+    (blob volume, pad_factor 2 -> 256^3, 512 poses): the fused
     pre/post kernels, torch's FFTs and the brick gather / scatter; the
     projection also against its torch composition on the same GPU."""
     import teamtomo_amd as tt

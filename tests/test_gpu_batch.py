@@ -123,7 +123,7 @@ def test_batched_extract_and_insert_match_one_call_and_replay(device):
         gi.replay()
 
     t_loop, t_batch = _time(loop), _time(graphs)
-    print(f"config-4 extract+insert: {len(vols)} sequential one-call pairs {t_loop:.2f} ms, "
+    print(f"32 mixed-size volumes, extract+insert: {len(vols)} sequential one-call pairs {t_loop:.2f} ms, "
           f"batched graph replay {t_batch:.2f} ms ({t_loop / t_batch:.2f}x)")
     bx.close()
     bi.close()

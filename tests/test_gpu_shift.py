@@ -184,7 +184,7 @@ def test_shift_gradient_complex128_vs_finite_differences(device):
 
 
 def test_shift_pass_rate(device):
-    """Config-5-sized batch (61 tilts of 512^2, rfft, fftshifted): one pass of
+    """A 61-tilt batch of 512^2 rffts (fftshifted): one pass of
     the kernel against the reference's composition in torch (fftfreq grid,
     angle sum, cos/sin, complex, fftshift, product) on the same GPU."""
     n, h, w = 61, 512, 512

@@ -29,13 +29,13 @@ def _np(t):
 
 
 # ----------------------------------------------------------------- extraction
-def test_extract_config1_vs_oracle_and_golden(device):
+def test_extract_64cubed_100rot_vs_oracle_and_golden(device):
     G = golden("extract_insert_64")
     V = _t(G["volume_rfft"], device)
     R = _t(G["rotations"], device)
     S = _np(tt.extract_central_slices_rfft_3d(V, (64, 64, 64), R))
     ref = O.extract_central_slices(G["volume_rfft"], (64, 64, 64), O.prepare_rotations(G["rotations"], False, 1e-8))
-    print("extract config1 rel_l2 vs FP64:", rel_l2(S, ref), "max", max_abs(S, ref))
+    print("extract 64^3 x 100 rel_l2 vs FP64:", rel_l2(S, ref), "max", max_abs(S, ref))
     assert rel_l2(S, ref) < EXT_TOL
     assert rel_l2(S, G["slices"]) < EXT_TOL
 
@@ -157,11 +157,11 @@ def test_prepare_rotations_kernel_matches_torch_steps(device):
 
 
 # ------------------------------------------------------------------ insertion
-def test_insert_config1_vs_oracle_and_golden(device):
+def test_insert_64cubed_100rot_vs_oracle_and_golden(device):
     G = golden("extract_insert_64")
     Vi, Wi = tt.insert_central_slices_rfft_3d(_t(G["slices"], device), (64, 64, 64), _t(G["rotations"], device))
     rV, rW = O.insert_central_slices(G["slices"], (64, 64, 64), O.prepare_rotations(G["rotations"], False, None))
-    print("insert config1 rel_l2 vs FP64:", rel_l2(_np(Vi), rV), rel_l2(_np(Wi), rW))
+    print("insert 64^3 x 100 rel_l2 vs FP64:", rel_l2(_np(Vi), rV), rel_l2(_np(Wi), rW))
     assert Vi.dtype == torch.complex64 and Wi.dtype == torch.float32
     assert rel_l2(_np(Vi), rV) < INS_TOL and rel_l2(_np(Wi), rW) < INS_TOL
     assert rel_l2(_np(Vi), G["ins_volume"]) < INS_TOL and rel_l2(_np(Wi), G["ins_weights"]) < INS_TOL
@@ -233,7 +233,7 @@ def test_insert_backward_vs_golden(device):
     assert rel_l2(_np(R.grad), G["ins_grad_rotations"]) < GRAD_TOL
 
 
-def test_extract_adjoint_vs_oracle_config1(device):
+def test_extract_adjoint_vs_oracle_64cubed_100rot(device):
     G = golden("extract_insert_64")
     rng = np.random.default_rng(5)
     T = (rng.standard_normal(G["slices"].shape) + 1j * rng.standard_normal(G["slices"].shape)).astype(np.complex64)
@@ -242,7 +242,7 @@ def test_extract_adjoint_vs_oracle_config1(device):
     S.backward(_t(T, device))
     Rz = O.prepare_rotations(G["rotations"], False, 1e-8)
     ref = O.extract_adjoint(T, (64, 64, 33), (64, 64, 64), Rz)
-    print("adjoint config1 rel_l2 vs FP64:", rel_l2(_np(V.grad), ref))
+    print("adjoint 64^3 x 100 rel_l2 vs FP64:", rel_l2(_np(V.grad), ref))
     assert rel_l2(_np(V.grad), ref) < INS_TOL
 
 

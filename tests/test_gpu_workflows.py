@@ -45,7 +45,7 @@ def ctf_rfft(n, pixel_size, defocus_u, defocus_v, astig_angle, kv=300.0, cs_mm=2
     return c
 
 
-def test_config3_ctf_loss_gradients_vs_finite_differences(device):
+def test_ctf_loss_gradients_128cubed_512poses_vs_finite_differences(device):
     """128^3 blob volume, pad 2 (256^3 inside), 64 poses (a reduced count of
     the 512 so that the FP64 finite differences stay affordable), CTF at
     300 kV / Cs 2.7 mm / 0.07 / 1.5 A/px, defocus U(1,3) um, astigmatism
@@ -126,7 +126,7 @@ def test_config3_ctf_loss_gradients_vs_finite_differences(device):
         fd = (loss64(dft_of(vol64 + hv * d), Rz, all_idx) - loss64(dft_of(vol64 - hv * d), Rz, all_idx)) / (2 * hv)
         g = float(np.sum(v.grad.double().cpu().numpy() * d))
         err_V.append(abs(g - fd) / abs(fd))
-    print(f"config-3 loss {float(loss.detach()):.6e}: grad R rel_l2 vs FP64 FD {err_R:.2e}; "
+    print(f"128^3 pad 2 x 512 loss {float(loss.detach()):.6e}: grad R rel_l2 vs FP64 FD {err_R:.2e}; "
           f"grad V directional rel err {err_V[0]:.2e}, {err_V[1]:.2e}")
     assert err_R < 1e-4
     assert max(err_V) < 1e-4
@@ -155,7 +155,7 @@ def _phase_shift(images, shifts):
     return torch.fft.irfftn(torch.fft.rfftn(images, dim=(-2, -1)) * ramp, s=(n, n), dim=(-2, -1))
 
 
-def test_config5_tomogram_backprojection_and_shift_fit(device):
+def test_61tilt_512cubed_backprojection_and_shift_fit(device):
     """61 tilts -60..60 step 2, 512^2 float32 images = N(0,1) + 500 Gaussian
     features (projections of 500 seeded 3-D blobs), per-tilt shifts
     U(-5,5) px applied as phase ramps (torch, FP64).  The insertion into
@@ -185,7 +185,7 @@ def test_config5_tomogram_backprojection_and_shift_fit(device):
     Rz = O.prepare_rotations(R.cpu().numpy(), False, None)
     ref_v, ref_w = C.insert(x.cpu().numpy(), (n, n, n), Rz)
     ev, ew = rel_l2(dft.cpu().numpy(), ref_v), rel_l2(w.cpu().numpy(), ref_w)
-    print(f"config-5 insert (512,512,257): rel_l2 volume {ev:.2e}, weights {ew:.2e}")
+    print(f"61-tilt insert (512,512,257): rel_l2 volume {ev:.2e}, weights {ew:.2e}")
     assert ev < 1e-4 and ew < 1e-4
     del dft, w, ref_v, ref_w, x
 
@@ -213,6 +213,6 @@ def test_config5_tomogram_backprojection_and_shift_fit(device):
         losses.append(float(loss.detach()))
         del rec, loss, moved
     err1 = float(torch.sqrt(torch.mean((s.detach().cpu() - shifts_true) ** 2)))
-    print(f"config-5 shift fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}, shift error rms {err0:.3f} -> {err1:.3f} px")
+    print(f"61-tilt shift fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}, shift error rms {err0:.3f} -> {err1:.3f} px")
     assert losses[-1] < 0.1 * losses[0]
     assert err1 < 0.25 * err0
