@@ -185,6 +185,45 @@ def torch_adjoint(gS, gV, image_shape, R):
         out.index_add_(0, idx.reshape(-1), gr * w.reshape(-1, 1))
 
 
+def torch_insert(S, vol, wts, R):
+    """Plain-torch insertion (slice_insertion/_insert_central_slices_rfft_3d.py:
+    9-106, :289-406): conj where x' < 0, x = 0 / y < 0 pixels skipped, floor/ceil
+    corners, index_add_ of w * value and w, and the Friedel mirror of every
+    x = 0 corner but DC."""
+    d, h, wr = vol.shape
+    w = 2 * (wr - 1)
+    R = torch.flip(R.to(torch.float32), dims=(-2, -1))
+    fy = torch.fft.fftshift(torch.fft.fftfreq(h, device=S.device))[:, None]
+    fx = torch.fft.rfftfreq(w, device=S.device)[None, :]
+    q = [R[:, k, 1, None, None] * fy + R[:, k, 2, None, None] * fx for k in range(3)]
+    conj = q[2] < 0
+    sgn = 1.0 - 2.0 * conj.to(torch.float32)
+    c = torch.stack([(sgn * q[0] + 0.5) / (1 - 1 / d) * (d - 1), (sgn * q[1] + 0.5) / (1 - 1 / h) * (h - 1),
+                     sgn * q[2] * 2 * (wr - 1)], dim=-1)
+    keep = ~((fx == 0) & (fy < 0)).expand(S.shape[-2:])
+    val = torch.where(conj, torch.conj(S), S)
+    dims = torch.tensor([d, h, wr], device=S.device, dtype=torch.float32)
+    ok = keep[None] & torch.all((c >= 0) & (c <= dims - 1), dim=-1)
+    c, val = c[ok], val[ok]
+    lo, hi = torch.floor(c), torch.ceil(c)
+    t = c - lo
+    lo, hi = lo.long(), hi.long()
+    vr = torch.view_as_real(vol).view(-1, 2)
+    wf = wts.view(-1)
+    vv = torch.view_as_real(val.resolve_conj())
+    for k in range(8):
+        dz, dy, dx = k >> 2, (k >> 1) & 1, k & 1
+        iz, iy, ix = (hi if dz else lo)[:, 0], (hi if dy else lo)[:, 1], (hi if dx else lo)[:, 2]
+        wk = (t[:, 0] if dz else 1 - t[:, 0]) * (t[:, 1] if dy else 1 - t[:, 1]) * (t[:, 2] if dx else 1 - t[:, 2])
+        idx = (iz * h + iy) * wr + ix
+        vr.index_add_(0, idx, vv * wk[:, None])
+        wf.index_add_(0, idx, wk)
+        m = (ix == 0) & ~((iz == d // 2) & (iy == h // 2))  # the x = 0 Friedel mirror
+        midx = (((d - iz[m]) % d) * h + (h - iy[m]) % h) * wr
+        vr.index_add_(0, midx, torch.stack([vv[m, 0], -vv[m, 1]], -1) * wk[m, None])
+        wf.index_add_(0, midx, wk[m])
+
+
 def time_torch_comparator(V, R, gS, steps=1, chunk=256):
     """Plain-torch extract + adjoint on the same GPU, chunked over rotations."""
 
@@ -482,6 +521,26 @@ def run_insert3d(args, device):
     Ry = np.zeros((t.size, 3, 3), dtype=np.float32)
     Ry[:, 0, 0], Ry[:, 0, 2], Ry[:, 1, 1], Ry[:, 2, 0], Ry[:, 2, 2] = np.cos(t), np.sin(t), 1.0, -np.sin(t), np.cos(t)
     c5, _ = case(512, torch.from_numpy(Ry).to(device), 5)
+    torch_line = None
+    if args.torch:
+        def torch_step():
+            vt = torch.zeros_like(vol2)
+            wt = torch.zeros_like(w2)
+            for r0 in range(0, N_ROT, 256):
+                torch_insert(S2[r0:r0 + 256], vt, wt, Rz2[r0:r0 + 256].flip(dims=(-2, -1)))
+            return vt, wt
+
+        vt, wt = torch_step()
+        run2()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        torch_step()
+        torch.cuda.synchronize()
+        tms = (time.perf_counter() - t0) * 1e3
+        rel_v = float(torch.linalg.vector_norm(vt - vol2) / torch.linalg.vector_norm(vt))
+        rel_w = float(torch.linalg.vector_norm(wt - w2) / torch.linalg.vector_norm(wt))
+        torch_line = {"value": N_ROT / (tms / 1e3), "unit": "slices/s", "ms_per_step": tms, "rotation_chunk": 256,
+                      "speedup": tms / c2["ms"], "rel_l2_vs_ours": {"volume": rel_v, "weights": rel_w}}
     line = {
         "metric": "slices/s central-slice insertion at 256^3 c64 (4096 slices); % HBM roofline",
         "value": c2["slices_per_s"], "unit": "slices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -493,6 +552,8 @@ def run_insert3d(args, device):
                      "frac": c2["GBs"] / HBM_PEAK_GBS, "traffic": None},
         "kernels": {"insert_256": c2, "insert_512_tilts": dict(c5, frac=c5["GBs"] / HBM_PEAK_GBS)},
     }
+    if torch_line is not None:
+        line["torch_same_gpu"] = torch_line
     if not args.no_cpu:
         from oracle import c_oracle as C
         from oracle import fourier_slice as O
