@@ -151,6 +151,12 @@ def fourier_shift_dft_3d(dft: torch.Tensor, image_shape: tuple[int, int, int], s
 def _shift_image(image, shifts, ndim):
     dims = tuple(range(-ndim, 0))
     shape = tuple(image.shape[-ndim:])
+    if image.numel() == 0:  # hipFFT refuses empty batches; the result is empty
+        _lib.require_gpu(image, shifts)
+        batch = torch.broadcast_shapes(tuple(image.shape[:-ndim]), tuple(shifts.shape[:-1] if ndim > 1 else shifts.shape))
+        out_shape = tuple(batch) + shape[:-1] + (2 * (shape[-1] // 2),)
+        return torch.empty(out_shape, dtype=torch.float32 if image.dtype != torch.float64 else torch.float64,
+                           device=image.device)
     dft = torch.fft.rfftn(image, dim=dims)
     dft = _fourier_shift(dft, shape, shifts, True, False, ndim, inplace=True)
     # irfftn without s, as the reference calls it: an odd last axis comes back

@@ -48,6 +48,11 @@ def _shift(patches, shifts, ndim, p, return_rfft, decenter):
     """The subpixel shift (fourier_shift_image_nd, or the rfft with
     fourier_shift_dft_nd when return_rfft), as the reference composes it."""
     dims = tuple(range(-ndim, 0))
+    if patches.numel() == 0:  # hipFFT refuses empty batches; the result is empty
+        lead = tuple(patches.shape[:-ndim]) + (p,) * (ndim - 1)
+        if return_rfft:
+            return torch.empty(lead + (p // 2 + 1,), dtype=torch.complex64, device=patches.device)
+        return torch.empty(lead + (2 * (p // 2),), dtype=torch.float32, device=patches.device)
     dft = torch.fft.rfftn(patches, dim=dims)
     if return_rfft and decenter:
         shifts = shifts + p / 2

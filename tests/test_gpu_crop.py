@@ -56,3 +56,13 @@ def test_crop_3d_many_patches_vs_oracle(device):
         tt.subpixel_crop_3d(_t(vol, device).double(), _t(pos, device), 16)
     with pytest.raises(ValueError):
         tt.subpixel_crop_2d(torch.zeros((2, 8, 8), device=device), torch.zeros((4, 3, 2), device=device), 4)
+
+
+def test_crop_without_positions(device):
+    """No positions: empty patches (real or rfft) rather than an FFT error."""
+    vol = torch.rand(16, 16, 16, device=device)
+    assert tt.subpixel_crop_3d(vol, torch.zeros(0, 3, device=device), 8).shape == (0, 8, 8, 8)
+    out = tt.subpixel_crop_3d(vol, torch.zeros(0, 3, device=device), 8, return_rfft=True)
+    assert out.shape == (0, 8, 8, 5) and out.dtype == torch.complex64
+    imgs = torch.rand(2, 16, 16, device=device)
+    assert tt.subpixel_crop_2d(imgs, torch.zeros(0, 2, 2, device=device), 8).shape == (0, 2, 8, 8)

@@ -219,3 +219,16 @@ def test_shift_pass_rate(device):
     print(f"\nfourier_shift_dft_2d 61x512x257 c64: {t_ours * 1e6:.1f} us ({gbs:.0f} GB/s), "
           f"torch composition {t_ref * 1e6:.1f} us ({t_ref / t_ours:.1f}x)")
     assert t_ours < t_ref
+
+
+def test_empty_inputs(device):
+    """Empty batches give empty results of the right shape and dtype."""
+    dft = torch.zeros((0, 8, 5), dtype=torch.complex64, device=device)
+    out = tt.fourier_shift_dft_2d(dft, (8, 8), torch.zeros((0, 2), device=device), True, False)
+    assert out.shape == (0, 8, 5) and out.dtype == torch.complex64
+    out = tt.fourier_shift_image_3d(torch.zeros((0, 4, 4, 4), device=device), torch.zeros((0, 3), device=device))
+    assert out.shape == (0, 4, 4, 4)
+    assert tt.sample_image_1d(torch.rand(10, device=device), torch.zeros(0, device=device)).shape == (0,)
+    img = torch.zeros(10, device=device)
+    img, w = tt.insert_into_image_1d(torch.zeros(0, device=device), torch.zeros(0, device=device), img)
+    assert not bool(img.abs().amax()) and not bool(w.abs().amax())
