@@ -378,6 +378,8 @@ def project_3d_to_2d(
         volume_rfft=dft, image_shape=volume_shape, rotation_matrices=rotation_matrices,
         fftfreq_max=fftfreq_max, zyx_matrices=zyx_matrices,
     )
+    if projections.numel() == 0:  # no poses: hipFFT refuses empty batches
+        return torch.empty(projections.shape[:-2] + (d, d), dtype=volume.dtype, device=volume.device)
     projections = torch.fft.ifftshift(projections, dim=(-2,))
     projections = torch.fft.irfftn(projections, dim=(-2, -1))
     projections = torch.fft.ifftshift(projections, dim=(-2, -1))
@@ -423,9 +425,13 @@ def backproject_2d_to_3d(
         images = F.pad(images, pad=[p] * 4)
     h, w = images.shape[-2:]
     volume_shape = (w, w, w)
-    images = torch.fft.fftshift(images, dim=(-2, -1))
-    images = torch.fft.rfftn(images, dim=(-2, -1))
-    images = torch.fft.fftshift(images, dim=(-2,))
+    if images.numel() == 0:  # hipFFT refuses empty batches: an empty rfft stack
+        cdt = torch.complex128 if images.dtype == torch.float64 else torch.complex64
+        images = torch.empty(images.shape[:-1] + (w // 2 + 1,), dtype=cdt, device=images.device)
+    else:
+        images = torch.fft.fftshift(images, dim=(-2, -1))
+        images = torch.fft.rfftn(images, dim=(-2, -1))
+        images = torch.fft.fftshift(images, dim=(-2,))
     dft, weights = insert_central_slices_rfft_3d(
         image_rfft=images, volume_shape=volume_shape, rotation_matrices=rotation_matrices,
         fftfreq_max=fftfreq_max, zyx_matrices=zyx_matrices,
@@ -574,6 +580,9 @@ def project_2d_to_1d(
         image_rfft=dft, image_shape=image_shape, rotation_matrices=rotation_matrices,
         fftfreq_max=fftfreq_max, yx_matrices=yx_matrices,
     )
+    if lines.numel() == 0:  # no rotations: hipFFT refuses empty batches
+        n = image_shape[-1] - (2 * p if pad_factor > 1.0 else 0)
+        return torch.empty(lines.shape[:-1] + (n,), dtype=image.dtype, device=image.device)
     projections = torch.fft.irfftn(lines, dim=(-1,))
     projections = torch.fft.ifftshift(projections, dim=(-1,))
     if pad_factor > 1.0:

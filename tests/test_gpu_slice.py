@@ -156,6 +156,17 @@ def test_prepare_rotations_kernel_matches_torch_steps(device):
     assert Rg.grad is not None
 
 
+def test_projections_of_empty_batches(device):
+    """No poses / no images: empty projections and a zero backprojection
+    (hipFFT refuses empty batches, so these skip the FFT)."""
+    vol = torch.rand(16, 16, 16, device=device)
+    none = torch.zeros(0, 3, 3, device=device)
+    assert tt.project_3d_to_2d(vol, none).shape == (0, 16, 16)
+    rec = tt.backproject_2d_to_3d(torch.zeros(0, 16, 16, device=device), none)
+    assert rec.shape == (16, 16, 16) and not bool(rec.abs().amax())
+    assert tt.project_2d_to_1d(torch.rand(16, 16, device=device), torch.zeros(0, 2, 2, device=device)).shape == (0, 16)
+
+
 # ------------------------------------------------------------------ insertion
 def test_insert_64cubed_100rot_vs_oracle_and_golden(device):
     G = golden("extract_insert_64")
