@@ -146,6 +146,20 @@ int tt_insert_c128(const void *slices, int vol_d, int vol_h, int vol_w,
                    void *volume, double *weights, int flags, void *stream);
 
 /*
+ * Multichannel insertion: replaces insert_central_slices_rfft_3d_multichannel
+ * (torch-fourier-slice/.../slice_insertion/_insert_central_slices_rfft_3d.py:109-210).
+ *   slices   (n_rot, channels, vol_h, vol_w/2+1) complex64, read in place
+ *   volume   (channels, vol_d, vol_h, vol_w/2+1) complex64
+ *   weights  (vol_d, vol_h, vol_w/2+1) float32, shared by the channels (as in
+ *            the reference, one weight per inserted point, not per channel)
+ * Accumulates, or writes with TT_FLAG_OVERWRITE.  Brick engine only
+ * (TT_FLAG_PIXEL_ATOMIC and geometries beyond its limits are refused).
+ */
+int tt_insert_mc_c64(const void *slices, int channels, int vol_d, int vol_h, int vol_w,
+                     const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                     void *volume, float *weights, int flags, void *stream);
+
+/*
  * Backward of the insertion: the gradient w.r.t. the slices (the adjoint
  * gather, written into grad_slices) and/or w.r.t. the rotations (accumulated
  * into grad_rot).  grad_weights may be NULL (treated as 0); `slices` is read
@@ -173,6 +187,23 @@ int tt_sample3d_f32(const float *image, int channels, int d, int h, int w, int i
                     const float *coords, int64_t n, int mode, float *out, void *stream);
 int tt_sample3d_f64(const double *image, int channels, int d, int h, int w, int is_complex,
                     const double *coords, int64_t n, int mode, double *out, void *stream);
+
+/*
+ * Backward of the generic sampler: replaces the autograd of sample_image_3d,
+ * i.e. grid_sample's backward with border padding and align_corners=True and
+ * the zeroing of outside samples (.../image_interpolation_3d.py:64-89).
+ *   grad_out    (n, channels) gradient of the samples (complex interleaved)
+ *   grad_image  (channels, d, h, w), ACCUMULATED (the caller zeroes it), or NULL
+ *   grad_coords (n, 3) written, or NULL; Re(conj(g) ds/dc) summed over the
+ *               channels, 0 along an axis where c <= 0 or c >= N-1, 0 for
+ *               nearest sampling and for samples outside the image
+ */
+int tt_sample3d_bwd_f32(const float *image, int channels, int d, int h, int w, int is_complex,
+                        const float *coords, int64_t n, int mode, const float *grad_out, float *grad_image,
+                        float *grad_coords, void *stream);
+int tt_sample3d_bwd_f64(const double *image, int channels, int d, int h, int w, int is_complex,
+                        const double *coords, int64_t n, int mode, const double *grad_out, double *grad_image,
+                        double *grad_coords, void *stream);
 
 /*
  * Generic splat, accumulating in place: replaces insert_into_image_3d

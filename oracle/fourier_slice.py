@@ -30,7 +30,11 @@ insert_adjoint / insert_rotation_grad -- dot-product test and finite
 differences; sample_image_3d / insert_into_image_3d -- textbook trilinear
 (affine reproduction), known answers from the reference tests, golden
 fixtures; project_3d_to_2d / backproject_2d_to_3d -- rotation-centre and
-central-slice-theorem tests, golden fixtures.
+central-slice-theorem tests, golden fixtures; the multichannel operators --
+the golden fixtures ``multichannel_16`` and equality with the single-channel
+functions applied per channel; sample_image_3d_grad -- golden torch-autograd
+gradients (``sample_autograd``), the adjoint identity of the image gradient
+and FP64 central differences of the coordinate gradient.
 """
 
 from __future__ import annotations
@@ -49,7 +53,11 @@ __all__ = [
     "insert_adjoint",
     "insert_rotation_grad",
     "sample_image_3d",
+    "sample_image_3d_grad",
     "insert_into_image_3d",
+    "extract_central_slices_multichannel",
+    "insert_central_slices_multichannel",
+    "project_3d_to_2d_multichannel",
     "project_3d_to_2d",
     "backproject_2d_to_3d",
     "volume_rfft",
@@ -505,6 +513,87 @@ def sample_image_3d(image, coordinates, interpolation="trilinear"):
     return out if multich else out[..., 0]
 
 
+def _grid_sample_source(c, shape, precision):
+    """grid_sample's unclipped source coordinate of array coordinate c
+    (TII/grid_sample_utils.py:24 normalises g = c / (0.5 N - 0.5) - 1,
+    align_corners=True undoes it as u = ((g + 1) / 2) (N - 1)) in the
+    sampler's precision; the round trip can move an integer coordinate by an
+    ulp, which decides its cell and one-sided derivative."""
+    f = np.dtype(precision).type
+    cc = np.asarray(c, dtype=precision)
+    N = np.asarray(shape, dtype=precision)
+    sc = (f(0.5) * N - f(0.5)).astype(precision)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = ((cc / sc).astype(precision) - f(1.0)).astype(precision)
+        u = (((g + f(1.0)).astype(precision) / f(2.0)).astype(precision) * (N - f(1.0))).astype(precision)
+    return np.where(N > 1, u, 0).astype(np.float64)
+
+
+def sample_image_3d_grad(image, coordinates, grad_samples, interpolation="trilinear", precision=np.float32):
+    """Gradients of sample_image_3d with respect to the image and the coordinates.
+
+    TII/image_interpolation_3d.py:64-89: grid_sample (border padding,
+    align_corners=True) then ``samples[~inside] *= 0``, so that
+    * samples outside [0, N-1] pass no gradient at all;
+    * trilinear: with u the source coordinate (``_grid_sample_source``, in
+      the sampler's ``precision``) clipped to [0, N-1], the image gradient is
+      the transposed interpolation, ``gI[corner] += w_corner(u) * g``; the
+      coordinate gradient along axis d is
+      ``Re(conj(g) * sum_corner dw/du_d * I[corner])``, with dw/du_d the
+      derivative of the floor / floor+1 weights, and 0 where the border clip
+      holds the coordinate (``u_d <= 0`` or ``u_d >= N-1``: grid_sample's
+      clip sets the gradient to 0 there);
+    * nearest: the whole gradient goes to the sampled voxel and the
+      coordinates get none.
+    Multichannel images sum the coordinate gradient over channels; complex
+    images follow torch's convention (grad = dL/dRe + i dL/dIm).
+    Returns (grad_image with the image's shape, grad_coords (..., 3)).
+    """
+    img = np.asarray(image)
+    multich = img.ndim == 4
+    if not multich:
+        img = img[None]
+    dt = np.complex128 if np.iscomplexobj(img) else np.float64
+    img = img.astype(dt)
+    coords = np.asarray(coordinates, dtype=np.float64)
+    lead = coords.shape[:-1]
+    c = coords.reshape(-1, 3)
+    g = np.asarray(grad_samples).astype(dt).reshape(c.shape[0], img.shape[0])
+    shape = img.shape[1:]
+    ok = _inside(c, shape)
+    gI = np.zeros_like(img)
+    gc = np.zeros_like(c)
+    if interpolation == "nearest":
+        idx = _grid_sample_nearest_index(np.where(ok[:, None], c, 0.0), shape)
+        for ch in range(img.shape[0]):
+            np.add.at(gI[ch], (idx[ok, 0], idx[ok, 1], idx[ok, 2]), g[ok, ch])
+    else:
+        u = _grid_sample_source(np.where(ok[:, None], c, 0.0), shape, precision)
+        hi = np.asarray(shape, dtype=np.float64) - 1
+        clipped = (u <= 0) | (u >= hi)
+        cc = np.clip(u, 0.0, hi)
+        i0 = np.floor(cc).astype(np.int64)
+        t = cc - i0
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    iz, iy, ix = i0[:, 0] + dz, i0[:, 1] + dy, i0[:, 2] + dx
+                    valid = ok & (iz < shape[0]) & (iy < shape[1]) & (ix < shape[2])
+                    wz = t[:, 0] if dz else 1 - t[:, 0]
+                    wy = t[:, 1] if dy else 1 - t[:, 1]
+                    wx = t[:, 2] if dx else 1 - t[:, 2]
+                    sz, sy, sx = (1.0 if dz else -1.0), (1.0 if dy else -1.0), (1.0 if dx else -1.0)
+                    dw = np.stack([sz * wy * wx, wz * sy * wx, wz * wy * sx], axis=-1)
+                    jz, jy, jx = np.where(valid, iz, 0), np.where(valid, iy, 0), np.where(valid, ix, 0)
+                    for ch in range(img.shape[0]):
+                        v = img[ch][jz, jy, jx]
+                        np.add.at(gI[ch], (jz[valid], jy[valid], jx[valid]), (wz * wy * wx * g[:, ch])[valid])
+                        gc += np.where(valid[:, None], np.real(np.conj(g[:, ch]) * v)[:, None] * dw, 0.0)
+        gc = np.where(clipped, 0.0, gc)
+    gc = np.where(ok[:, None], gc, 0.0)
+    return (gI if multich else gI[0]), gc.reshape(*lead, 3)
+
+
 def _grid_sample_nearest_index(c, shape):
     """Nearest index as grid_sample computes it, in float32.
 
@@ -560,6 +649,35 @@ def insert_into_image_3d(values, coordinates, image, weights=None, interpolation
                 np.add.at(img4[ch], (iz, iy, ix), w * v[:, ch])
             np.add.at(weights, (iz, iy, ix), w)
     return img, weights
+
+
+def extract_central_slices_multichannel(volume, image_shape, R_zyx, fftfreq_max=None, x_tolerance=1e-8):
+    """TFS/slice_extraction/_extract_central_slices_rfft_3d.py:225-367: every
+    channel of a (c, d, h, w) volume sampled at the same coordinates, as the
+    multichannel grid_sample does; returns (n, c, H_i, Wr_i)."""
+    vol = np.asarray(volume)
+    return np.stack([extract_central_slices(vol[k], image_shape, R_zyx, fftfreq_max, x_tolerance)
+                     for k in range(vol.shape[0])], axis=-3)
+
+
+def insert_central_slices_multichannel(slices, volume_shape, R_zyx, fftfreq_max=None):
+    """TFS/slice_insertion/_insert_central_slices_rfft_3d.py:109-210: (n, c, h, wr)
+    slices into (c, d, h, wr) volumes at the same coordinates; one weights
+    volume shared by the channels (the weights do not depend on the values)."""
+    d = np.asarray(slices)
+    vols, wts = [], None
+    for k in range(d.shape[-3]):
+        v, wts = insert_central_slices(d[..., k, :, :], volume_shape, R_zyx, fftfreq_max)
+        vols.append(v)
+    return np.stack(vols, axis=0), wts
+
+
+def project_3d_to_2d_multichannel(volume, R, pad_factor=2.0, fftfreq_max=None, zyx_matrices=False):
+    """TFS/project.py:96-177: project_3d_to_2d of every channel with the same
+    rotations; (c, d, d, d) -> (..., c, d, d)."""
+    vol = np.asarray(volume)
+    return np.stack([project_3d_to_2d(vol[k], R, pad_factor, fftfreq_max, zyx_matrices)
+                     for k in range(vol.shape[0])], axis=-3)
 
 
 def _sinc2_grid(shape):

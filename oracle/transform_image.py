@@ -16,14 +16,17 @@ samplers (``fourier_slice.sample_image_3d``, ``slice_2d.sample_image_2d``).
 Pinning (tests/test_oracle_transform.py): the golden fixtures written from the
 reference (``transform_image``), the identity matrix (returns the image), an
 integer translation (an exact shift with zero fill) and a 90-degree rotation
-about the centre (an exact permutation).
+about the centre (an exact permutation).  ``affine_transform_image_3d_grad``
+(the gradients of the 3-D warp through the matmul and the sampler): the
+golden torch-autograd gradients (``sample_autograd``) and FP64 central
+differences of the matrices.
 """
 
 from __future__ import annotations
 
 import numpy as np
 
-from .fourier_slice import sample_image_3d
+from .fourier_slice import sample_image_3d, sample_image_3d_grad
 from .slice_2d import sample_image_2d
 
 
@@ -43,6 +46,40 @@ def _warp(image, matrices, out_shape, flip, sampler, interpolation):
 def affine_transform_image_3d(image, matrices, interpolation, output_shape=None, zyx_matrices=False):
     out_shape = tuple(output_shape) if output_shape else np.asarray(image).shape[-3:]
     return _warp(image, matrices, out_shape, not zyx_matrices, sample_image_3d, interpolation)
+
+
+def affine_transform_image_3d_grad(image, matrices, grad_out, interpolation, output_shape=None,
+                                   zyx_matrices=False, exact_matrices=False):
+    """Gradients of affine_transform_image_3d (transforms_3d.py:22-40) with
+    respect to the image and the (..., 4, 4) matrices: the sampler's
+    gradients at coords = M hom(z, y, x), then dL/dM[i, j] = sum over the
+    grid of dL/dc_i hom_j, summed over the grid dimensions the matrices
+    broadcast against, and flipped back to xyz unless zyx_matrices.  The
+    bottom row gets no gradient (the homogeneous coordinate is dropped).
+    ``exact_matrices`` keeps FP64 matrices (for finite differences) instead of
+    the reference's float32 ones."""
+    out_shape = tuple(output_shape) if output_shape else np.asarray(image).shape[-3:]
+    M0 = np.asarray(matrices, dtype=np.float64 if exact_matrices else np.float32).astype(np.float64)
+    M = M0.copy()
+    if not zyx_matrices:
+        M[..., :3, :3] = M[..., :3, :3][..., ::-1, ::-1]
+        M[..., :3, 3] = M[..., :3, 3][..., ::-1]
+    grid = np.moveaxis(np.indices(out_shape, dtype=np.float64), 0, -1)
+    hom = np.concatenate([grid, np.ones(grid.shape[:-1] + (1,))], axis=-1)[..., None]
+    coords = (M @ hom)[..., :3, 0]
+    gI, gc = sample_image_3d_grad(image, coords, grad_out, interpolation)
+    gM = np.zeros(np.broadcast_shapes(M.shape[:-2], out_shape) + (4, 4))
+    gM[..., :3, :] = gc[..., :, None] * hom[..., 0][..., None, :]
+    # sum over the broadcast dimensions down to the matrices' own shape
+    lead = gM.ndim - 2 - (M.ndim - 2)
+    gM = gM.sum(axis=tuple(range(lead))) if lead else gM
+    for ax, n in enumerate(M.shape[:-2]):
+        if n == 1 and gM.shape[ax] != 1:
+            gM = gM.sum(axis=ax, keepdims=True)
+    if not zyx_matrices:
+        gM[..., :3, :3] = gM[..., :3, :3][..., ::-1, ::-1]
+        gM[..., :3, 3] = gM[..., :3, 3][..., ::-1]
+    return gI, gM
 
 
 def affine_transform_image_2d(image, matrices, interpolation, output_shape=None, yx_matrices=False):

@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -48,10 +48,13 @@ _SIGNATURES = {
     "tt_extract_bwd_c64": [_p, _p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p, _i, _p],
     "tt_insert_c64": [_p, _i, _i, _i, _p, _l, _d, _p, _p, _i, _p],
     "tt_insert_c128": [_p, _i, _i, _i, _p, _l, _d, _p, _p, _i, _p],
+    "tt_insert_mc_c64": [_p, _i, _i, _i, _i, _p, _l, _d, _p, _p, _i, _p],
     "tt_insert_bwd_c64": [_p, _p, _p, _i, _i, _i, _p, _l, _d, _p, _p, _p],
     "tt_insert_bwd_c128": [_p, _p, _p, _i, _i, _i, _p, _l, _d, _p, _p, _p],
     "tt_sample3d_f32": [_p, _i, _i, _i, _i, _i, _p, _l, _i, _p, _p],
     "tt_sample3d_f64": [_p, _i, _i, _i, _i, _i, _p, _l, _i, _p, _p],
+    "tt_sample3d_bwd_f32": [_p, _i, _i, _i, _i, _i, _p, _l, _i, _p, _p, _p, _p],
+    "tt_sample3d_bwd_f64": [_p, _i, _i, _i, _i, _i, _p, _l, _i, _p, _p, _p, _p],
     "tt_splat3d_f32": [_p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p],
     "tt_splat3d_f64": [_p, _p, _l, _i, _i, _i, _i, _i, _i, _p, _p, _i, _p],
     "tt_extract2d_c64": [_p, _i, _i, _i, _i, _p, _l, _d, _p, _p],

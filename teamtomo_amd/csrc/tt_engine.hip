@@ -74,8 +74,25 @@ struct PlaneRec {
 // degenerate plane (every brick is a candidate).
 typedef float4 PlaneTest;
 
-__device__ __forceinline__ void plane_rec_one(const SliceGeom &g, const float *__restrict__ rot, int r,
-                                              PlaneRec *__restrict__ out, PlaneTest *__restrict__ test) {
+// Bits of a bound on the pixels of one slice that contribute to one voxel:
+// their coordinates lie in the open cube of side 2 around it, a plane section
+// of diameter at most D = 2 sqrt(3); the pixel lattice has column step |B|
+// and row spacing h = |A x B| / |B| in the plane, so at most
+// (D / h + 1)(D / |B| + 1) pixels fit, and never more than the slice's P.
+__device__ __forceinline__ int slice_count_bits(const SliceGeom &g, double nn, double bb) {
+  const double D = 3.4641016151377544;
+  double cnt = (double)g.P;
+  if (nn > 0.0 && bb > 0.0) {
+    const double lb = sqrt(bb);
+    cnt = fmin(cnt, (D * lb / nn + 1.0) * (D / lb + 1.0));
+  }
+  int e = 0;
+  frexp(cnt, &e);  // cnt < 2^e
+  return e;
+}
+
+__device__ __forceinline__ int plane_rec_one(const SliceGeom &g, const float *__restrict__ rot, int r,
+                                             PlaneRec *__restrict__ out, PlaneTest *__restrict__ test) {
   const Rot rr = load_rot(rot + (int64_t)r * 9, g);
   PlaneRec p;
   double A[3], B[3];
@@ -110,13 +127,21 @@ __device__ __forceinline__ void plane_rec_one(const SliceGeom &g, const float *_
     t = make_float4((float)(n0 / nn), (float)(n1 / nn), (float)(n2 / nn), 1.0f);
   }
   test[r] = t;
+  return slice_count_bits(g, nn, bb);
+}
+
+// The wave's largest per-slice count bits into *bits (if given).
+__device__ __forceinline__ void max_count_bits(int b, unsigned *bits) {
+  for (int off = 32; off > 0; off >>= 1) b = max(b, __shfl_xor(b, off, 64));
+  if (bits && (threadIdx.x & 63) == 0 && b > 0) atomicMax(bits, (unsigned)b);
 }
 
 __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float *__restrict__ rot,
                                                         int n_rot, PlaneRec *__restrict__ out,
-                                                        PlaneTest *__restrict__ test) {
+                                                        PlaneTest *__restrict__ test, unsigned *bits) {
   const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r < n_rot) plane_rec_one(g, rot, r, out, test);
+  const int b = r < n_rot ? plane_rec_one(g, rot, r, out, test) : 0;
+  max_count_bits(b, bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -227,16 +252,45 @@ struct __align__(16) ItemRec {
 constexpr int kTake = 32;              // rotations per item pass (two halves each)
 constexpr float kRowMargin = 1e-4f;    // widening of the row phase's tile box (voxels)
 
-// Fixed-point scale of the scatter: values are multiplied by
-// 2^(30 - headroom - e), with 2^e > the sampled max|v|, so a value up to
-// 2^headroom times the sampled max still converts to a 32-bit integer; a
-// larger (or non-finite) value raises *flag and the call is redone with the
-// exact maximum (headroom 0).
+// Fixed-point scale of the scatter, one per brick: values are multiplied by
+// 2^(30 - headroom - e), with 2^e > the largest |v| the brick can receive,
+// so a value up to 2^headroom times that maximum still converts to a 32-bit
+// integer; a larger (or non-finite) value raises *flag and the call is redone
+// with the exact maximum of the whole input (headroom 0).
+//
+// The maximum a brick can receive is taken per frequency band: a rotation
+// keeps |f|, so the pixels that reach a brick have radii |f| within the range
+// the brick's coordinate box spans, and only the bands of that range count.
+// When those bands' maxima agree within a factor kFlatRatio (a flat spectrum,
+// or a brick far from DC), each contribution is rounded to a 32-bit quantum
+// of 2^(e + headroom - 30), at most ~1e-6 of the smallest band maximum of the
+// brick.  Otherwise (a brick spanning a steep part of the spectrum, such as
+// the ones next to DC, whose voxels at shell 15 would be quantised against
+// the DC value) the FP32 product w v is converted exactly to a 64-bit integer
+// at scale 2^(62 - sumbits - headroom - e), sumbits bounding the bits of a
+// voxel's sum, a quantum of ~1e-11 of the maximum.  Either way small
+// high-frequency values keep their FP32 relative precision (per-shell
+// relative error ~1e-7 for a spectrum falling 1e4 from DC to Nyquist, where
+// one 32-bit scale per call gave 1e-3).
+constexpr int kBands = 64;            // bands of |f| in cycles/pixel over [0, kBandTop)
+constexpr float kFlatRatio = 4.0f;     // band maxima within this factor: 32-bit contributions
+constexpr float kBandTop = 0.75f;     // above the largest slice radius sqrt(1/2)
+
+__device__ __forceinline__ int band_of(float rho) {
+  return min(kBands - 1, max(0, (int)(rho * ((float)kBands / kBandTop))));
+}
+
 struct ScaleArgs {
-  const unsigned *maxbits;  // float bits of max|v| (sampled or exact), kMaxSlots slots
+  const unsigned *maxbits;  // float bits of the exact max|v| of the input, kMaxSlots slots
+  const unsigned *bandmax;  // float bits of the sampled max|v| per band (kBands words), or null
   unsigned *flag;           // set when a value exceeds the limit
-  int headroom;             // bits of headroom above the sampled maximum
+  int headroom;             // bits of headroom above the sampled maxima
   int guard;                // 1: run only if *flag is set (the exact redo)
+  // bits reserved for the sum of one voxel's 64-bit contributions: rotbits =
+  // ceil(log2(rotations)) plus *slicebits, the largest per-slice bound of the
+  // plane records (slice_count_bits)
+  int rotbits;
+  const unsigned *slicebits;
 };
 
 // The block maxima are combined with atomicMax into kMaxSlots words kSlotStride
@@ -244,15 +298,59 @@ struct ScaleArgs {
 // address); the maximum is the largest slot.
 constexpr int kMaxSlots = 32, kSlotStride = 32;
 constexpr int kSetWords = kMaxSlots * kSlotStride;  // words of one set of slots
+// scale words of a scatter call: sampled slots (unused), exact slots, the
+// violation flag (on a line of its own), then the sampled band maxima
+constexpr int kFlagWord = 2 * kSetWords, kBandWord = 2 * kSetWords + 32;
+constexpr int kSliceWord = kBandWord + kBands;  // the largest per-slice count bits
+constexpr int kScaleWords = kSliceWord + 1;
 
-__device__ __forceinline__ void scale_of(const ScaleArgs &sa, float &scale, float &limit) {
-  unsigned mb = 0u;
+// Radius range (cycles/pixel) of the coordinate box [lo, hi] (array units):
+// f_d = (c_d - b_d) / a_d with b = (bz, by, 0) and the array scales a.
+__device__ __forceinline__ void box_radii(const SliceGeom &g, const float lo[3], const float hi[3], float &rmin,
+                                          float &rmax) {
+  const float b[3] = {(float)g.bz, (float)g.by, 0.0f};
+  const float ia[3] = {(float)(1.0 / g.az), (float)(1.0 / g.ay), (float)(1.0 / g.ax)};
+  float n2 = 0.0f, f2 = 0.0f;
 #pragma unroll
-  for (int k = 0; k < kMaxSlots; ++k) mb = max(mb, sa.maxbits[k * kSlotStride]);
+  for (int d = 0; d < 3; ++d) {
+    const float u = (lo[d] - b[d]) * ia[d], v = (hi[d] - b[d]) * ia[d];
+    const float near = (u > 0.0f) ? u : ((v < 0.0f) ? -v : 0.0f);
+    const float far = fmaxf(fabsf(u), fabsf(v));
+    n2 += near * near;
+    f2 += far * far;
+  }
+  rmin = sqrtf(n2);
+  rmax = sqrtf(f2);
+}
+
+// The brick's scale, limit and contribution width (wide: 64-bit): from the
+// sampled band maxima of the radii its cells span (one band of margin each
+// side), or from the exact maximum of the input (always wide).
+__device__ __forceinline__ void scale_of(const ScaleArgs &sa, const SliceGeom &g, const float lo[3],
+                                         const float hi[3], float &scale, float &limit, bool &wide) {
+  unsigned mb = 0u;
+  wide = true;
+  if (sa.bandmax) {
+    float rmin, rmax;
+    box_radii(g, lo, hi, rmin, rmax);
+    const int k0 = max(band_of(rmin * 0.9999f) - 1, 0), k1 = min(band_of(rmax * 1.0001f) + 1, kBands - 1);
+    unsigned mn = 0xffffffffu;
+    for (int k = k0; k <= k1; ++k) {
+      const unsigned b = sa.bandmax[k];
+      mb = max(mb, b);
+      if (b) mn = min(mn, b);
+    }
+    wide = mb != 0u && __uint_as_float(mb) > kFlatRatio * __uint_as_float(mn);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k) mb = max(mb, sa.maxbits[k * kSlotStride]);
+  }
   const float m = __uint_as_float(mb);
   int e = 0;
   if (m > 0.0f && m < 3.0e38f) frexpf(m, &e);
-  scale = ldexpf(1.0f, 30 - sa.headroom - e);
+  e = max(e, -60);  // keeps the scales finite; values this small only lose relative precision
+  const int sumbits = sa.rotbits + (int)*sa.slicebits;
+  scale = ldexpf(1.0f, wide ? 62 - sumbits - sa.headroom - e : 30 - sa.headroom - e);
   limit = ldexpf(1.0f, e + sa.headroom);
 }
 
@@ -270,6 +368,22 @@ __device__ __forceinline__ int cvt_floor_i32(float x) {
 // Adds round(w * v) (half up) as a 64-bit fixed-point value (|w v| < 2^31).
 __device__ __forceinline__ void lds_add_fixed(u64 *p, float w, float v) {
   atomicAdd(p, (u64)(long long)cvt_floor_i32(__fmaf_rn(w, v, 0.5f)));
+}
+
+// round(p) (half to even) as a 64-bit integer, |p| < 2^62.  hi = trunc(p 2^-31)
+// and lo = p - hi 2^31 are exact in FP32: below 2^31 hi is 0 and lo = p;
+// above, p's last bit weighs at least 2^8, so lo is a multiple of 2^8 below
+// 2^31 (at most 23 significant bits).
+__device__ __forceinline__ long long to_fixed64(float p) {
+  const float hi = truncf(p * 4.656612873077393e-10f);  // 2^-31
+  const float lo = __fmaf_rn(hi, -2147483648.0f, p);
+  return (long long)((u64)(long long)(int)hi << 31) + (long long)(int)rintf(lo);
+}
+
+// Adds the FP32 product w v (v already multiplied by the power-of-two scale,
+// which commutes with its rounding) as an exact 64-bit fixed-point value.
+__device__ __forceinline__ void lds_add_wide(u64 *p, float w, float v) {
+  atomicAdd(p, (u64)to_fixed64(__fmul_rn(w, v)));
 }
 
 // The exact canonical FP64 cell of pixel (i, j) under rotation r and Friedel
@@ -334,14 +448,15 @@ __device__ __forceinline__ BrickBox brick_box(const SliceGeom &g, const BrickGri
 // is last) are converted and stored (the insertion's x = 0 plane into
 // plane/wplane for the Friedel mirror pass); the upper shell (l = B on an
 // axis where the brick is not last) belongs to the next brick and goes to
-// the halo buffer as fixed-point partial sums (unless write_shell is false:
-// a brick no slice touched, whose shell the halo pass then skips).
+// the halo buffer as FP32 partial sums, converted with this brick's scale
+// (unless write_shell is false: a brick no slice touched, whose shell the
+// halo pass then skips).
 template <int BZ, int BY, int BX, int MODE>
 __device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astride, int v,
                                             const BrickBox &bb, const SliceGeom &g, int brick,
                                             float inv_v, int accumulate, f2v *__restrict__ out,
                                             float *__restrict__ wout, f2v *__restrict__ plane,
-                                            float *__restrict__ wplane, u64 *__restrict__ halo,
+                                            float *__restrict__ wplane, float *__restrict__ halo,
                                             bool write_shell = true) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int SX = C::SX, SY = C::SY, NA = C::NACC, NSH = C::NSH;
@@ -351,10 +466,10 @@ __device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astri
                      (lx == bb.ext[2] + 1 && !bb.last[2]);
   if (shell) {
     if (!write_shell) return;
-    u64 *hb = halo + (int64_t)brick * NA * NSH;
+    float *hb = halo + (int64_t)brick * NA * NSH;
     const int si = shell_index<BZ, BY, BX>(lz, ly, lx);
 #pragma unroll
-    for (int k = 0; k < NA; ++k) hb[k * NSH + si] = a[k * astride];
+    for (int k = 0; k < NA; ++k) hb[k * NSH + si] = (float)(long long)a[k * astride] * (k < 2 ? inv_v : 1.0f / kWScale);
     return;
   }
   const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
@@ -368,7 +483,7 @@ __device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astri
       return;
     }
     const int64_t e = gz * sz + gy * sy + gx;
-    wout[e] = accumulate ? wout[e] + w : w;
+    if (wout) wout[e] = accumulate ? wout[e] + w : w;  // no weights: a channel after the first
     out[e] = accumulate ? out[e] + val : val;
   } else {
     const int64_t e = gz * sz + gy * sy + gx;
@@ -380,7 +495,7 @@ template <int BZ, int BY, int BX, int MODE>
 __device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, const BrickGrid &bg,
                            int brick, float vscale, int accumulate, f2v *__restrict__ out,
                            float *__restrict__ wout, f2v *__restrict__ plane,
-                           float *__restrict__ wplane, u64 *__restrict__ halo, int tid, int nthreads,
+                           float *__restrict__ wplane, float *__restrict__ halo, int tid, int nthreads,
                            bool write_shell) {
   constexpr int NT = Cfg<BZ, BY, BX, MODE>::NT;
   const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
@@ -400,7 +515,7 @@ __device__ __forceinline__ void brick_body(
     const f2v *__restrict__ src, const SliceGeom &g, const BrickGrid &bg, const PlaneRec *__restrict__ pre,
     const PlaneTest *__restrict__ ptest, const Unit &un, int n_rot, f2v *__restrict__ out,
     float *__restrict__ wout, f2v *__restrict__ plane, float *__restrict__ wplane,
-    u64 *__restrict__ segbuf, u64 *__restrict__ halo, const ScaleArgs &sa, int accumulate,
+    u64 *__restrict__ segbuf, float *__restrict__ halo, const ScaleArgs &sa, int accumulate,
     int *__restrict__ touched) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
@@ -437,7 +552,12 @@ __device__ __forceinline__ void brick_body(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   float vscale = 1.0f, vlimit = 3.0e38f;
-  if (SCATTER) scale_of(sa, vscale, vlimit);
+  bool wide = false;  // 64-bit contributions (uniform over the block)
+  if (SCATTER) {
+    const float blo[3] = {(float)lo0, (float)lo1, (float)lo2};
+    const float bhi[3] = {(float)(ch0 + 1), (float)(ch1 + 1), (float)(ch2 + 1)};
+    scale_of(sa, g, blo, bhi, vscale, vlimit, wide);
+  }
   bool bad = false;  // a value beyond the fixed-point limit was seen
 
   if (!SCATTER) {
@@ -744,8 +864,10 @@ __device__ __forceinline__ void brick_body(
           const int iref = (int)(ir.ref & 0xffffu), jref = (int)(ir.ref >> 16);
           const int rr = (int)(ir.rnt & 0x3fffffffu);
           const bool neg = (ir.rnt >> 30) & 1;
-          // flat index of the chunk's first pixel (i Wr + j0 < H Wr < 2^31)
-          const int64_t base = (int64_t)((u64)(unsigned)rr * (u64)(unsigned)g.P +
+          // flat index of the chunk's first pixel (i Wr + j0 < H Wr < 2^31);
+          // scatter input rotation stride: g.ostride (P, or C P for one
+          // channel of a multichannel stack)
+          const int64_t base = (int64_t)((u64)(unsigned)rr * (u64)(SCATTER ? g.ostride : (int64_t)g.P) +
                                          (u64)(unsigned)(i * g.Wr + j0));
           // scatter: the chunk's values are loaded first, so that the load
           // overlaps the coordinate arithmetic (pixels q < cnt are in the row)
@@ -846,20 +968,37 @@ __device__ __forceinline__ void brick_body(
               v.y *= vscale;
               const float tz = t3[q][0], ty = t3[q][1], tx = t3[q][2];
               const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
+              if (wide) {
 #pragma unroll
-              for (int dz = 0; dz < 2; ++dz)
+                for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
-                for (int dy = 0; dy < 2; ++dy) {
-                  const float wzy = wz[dz] * wy[dy];
+                  for (int dy = 0; dy < 2; ++dy) {
+                    const float wzy = wz[dz] * wy[dy];
 #pragma unroll
-                  for (int dx = 0; dx < 2; ++dx) {
-                    const float w = wzy * wx[dx];
-                    const int o = e[q] + dz * SY * SX + dy * SX + dx;
-                    lds_add_fixed(&s_tile[o], w, v.x);
-                    lds_add_fixed(&s_tile[NT + o], w, v.y);
-                    if (MODE == kInsert) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                    for (int dx = 0; dx < 2; ++dx) {
+                      const float w = wzy * wx[dx];
+                      const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                      lds_add_wide(&s_tile[o], w, v.x);
+                      lds_add_wide(&s_tile[NT + o], w, v.y);
+                      if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                    }
                   }
-                }
+              } else {
+#pragma unroll
+                for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                  for (int dy = 0; dy < 2; ++dy) {
+                    const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                      const float w = wzy * wx[dx];
+                      const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                      lds_add_fixed(&s_tile[o], w, v.x);
+                      lds_add_fixed(&s_tile[NT + o], w, v.y);
+                      if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                    }
+                  }
+              }
             }
           }
         }
@@ -894,7 +1033,7 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
     const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_rot,
     f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
-    float *__restrict__ wplane, u64 *__restrict__ segbuf, u64 *__restrict__ halo, ScaleArgs sa,
+    float *__restrict__ wplane, u64 *__restrict__ segbuf, float *__restrict__ halo, ScaleArgs sa,
     int accumulate, int *__restrict__ touched) {
   const Unit un = units[blockIdx.x];
   brick_body<BZ, BY, BX, MODE, MASK, NCH>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
@@ -908,7 +1047,7 @@ template <int BZ, int BY, int BX, int MODE>
 __device__ __forceinline__ void split_body(
     const SliceGeom &g, const BrickGrid &bg, const SplitRec &sp, const u64 *__restrict__ segbuf,
     const ScaleArgs &sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo, int v,
+    f2v *__restrict__ plane, float *__restrict__ wplane, float *__restrict__ halo, int v,
     int *__restrict__ touched) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NT = C::NT, NA = C::NACC;
@@ -923,9 +1062,15 @@ __device__ __forceinline__ void split_body(
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] += b[a * NT + v];
   }
-  float vscale, vlimit;
-  scale_of(sa, vscale, vlimit);
   const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, sp.brick);
+  float vscale, vlimit;
+  {
+    const float blo[3] = {(float)bb.lo[0], (float)bb.lo[1], (float)bb.lo[2]};
+    const float bhi[3] = {(float)(bb.lo[0] + bb.ext[0] + 1), (float)(bb.lo[1] + bb.ext[1] + 1),
+                          (float)(bb.lo[2] + bb.ext[2] + 1)};
+    bool wide;
+    scale_of(sa, g, blo, bhi, vscale, vlimit, wide);
+  }
   flush_voxel<BZ, BY, BX, MODE>(acc, 1, v, bb, g, sp.brick, 1.0f / vscale, accumulate, out, wout, plane,
                                 wplane, halo);
 }
@@ -934,7 +1079,7 @@ template <int BZ, int BY, int BX, int MODE>
 __global__ __launch_bounds__(256) void split_combine_kernel(
     SliceGeom g, BrickGrid bg, const SplitRec *__restrict__ splits, const u64 *__restrict__ segbuf,
     ScaleArgs sa, int accumulate, f2v *__restrict__ out, float *__restrict__ wout,
-    f2v *__restrict__ plane, float *__restrict__ wplane, u64 *__restrict__ halo, int *__restrict__ touched) {
+    f2v *__restrict__ plane, float *__restrict__ wplane, float *__restrict__ halo, int *__restrict__ touched) {
   const SplitRec sp = splits[blockIdx.x];
   split_body<BZ, BY, BX, MODE>(g, bg, sp, segbuf, sa, accumulate, out, wout, plane, wplane, halo,
                                (int)(blockIdx.y * 256 + threadIdx.x), touched);
@@ -947,7 +1092,7 @@ __global__ __launch_bounds__(256) void split_combine_kernel(
 // skipped, and so is a voxel that receives none.
 template <int BZ, int BY, int BX, int MODE>
 __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &bg,
-                                          const u64 *__restrict__ halo, const ScaleArgs &sa,
+                                          const float *__restrict__ halo, const ScaleArgs &sa,
                                           f2v *__restrict__ out, float *__restrict__ wout,
                                           f2v *__restrict__ plane, float *__restrict__ wplane, int brick,
                                           int u, const int *__restrict__ touched) {
@@ -958,9 +1103,6 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
   const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
   const int B[3] = {BZ, BY, BX};
   const int bstride[3] = {bg.nby * bg.nbx, bg.nbx, 1};
-  float vscale, vlimit;
-  scale_of(sa, vscale, vlimit);
-  const float inv_v = 1.0f / vscale;
   const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
   {
     // lower shell in the same order as the upper one: Z face, Y face, X face
@@ -988,7 +1130,9 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
       if (l[d] == 0 && bb.lo[d] > 0) amask |= 1 << d;
     }
     if (!owned || amask == 0) return;
-    u64 acc[3] = {0ull, 0ull, 0ull};
+    // FP32 partial sums (each converted with its own brick's scale), added in
+    // the fixed order of the loop below: the result does not depend on timing
+    float acc[3] = {0.0f, 0.0f, 0.0f};
     int nread = 0;
     // the nonempty subsets of amask, unrolled so that every load is in flight at once
 #pragma unroll
@@ -1004,15 +1148,15 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
       if (touched && !touched[nb]) continue;
       ++nread;
       const int si = shell_index<BZ, BY, BX>(lp[0], lp[1], lp[2]);
-      const u64 *hb = halo + (int64_t)nb * NA * NSH;
+      const float *hb = halo + (int64_t)nb * NA * NSH;
 #pragma unroll
       for (int a = 0; a < NA; ++a) acc[a] += hb[a * NSH + si];
     }
     if (touched && nread == 0) return;
     const int gz = bb.lo[0] + l[0], gy = bb.lo[1] + l[1], gx = bb.lo[2] + l[2];
-    const f2v val = {(float)(long long)acc[0] * inv_v, (float)(long long)acc[1] * inv_v};
+    const f2v val = {acc[0], acc[1]};
     if (MODE == kInsert) {
-      const float w = (float)(long long)acc[2] * (1.0f / kWScale);
+      const float w = acc[2];
       if (gx == 0) {
         const int64_t pe = (int64_t)gz * g.vH + gy;
         plane[pe] += val;
@@ -1020,7 +1164,7 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
         return;
       }
       const int64_t e = gz * sz + gy * sy + gx;
-      wout[e] += w;
+      if (wout) wout[e] += w;
       out[e] += val;
     } else {
       const int64_t e = gz * sz + gy * sy + gx;
@@ -1031,7 +1175,7 @@ __device__ __forceinline__ void halo_body(const SliceGeom &g, const BrickGrid &b
 
 template <int BZ, int BY, int BX, int MODE>
 __global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGrid bg,
-                                                           const u64 *__restrict__ halo, ScaleArgs sa,
+                                                           const float *__restrict__ halo, ScaleArgs sa,
                                                            f2v *__restrict__ out,
                                                            float *__restrict__ wout,
                                                            f2v *__restrict__ plane,
@@ -1080,6 +1224,39 @@ __global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restric
                                                             unsigned *__restrict__ res) {
   maxabs_body(x, n, stride_chunks, guard, res, (int)blockIdx.x, (int)gridDim.x);
 }
+
+// Sampled per-band max(|re|, |im|) of a slice stack: every rot_stride-th
+// slice, each pixel's value into the band of its radius |f| (block maxima in
+// LDS, then one atomic per band and block).
+__device__ __forceinline__ void band_max_body(const f2v *__restrict__ x, const SliceGeom &g, int64_t n_rot,
+                                              int64_t rot_stride, unsigned *__restrict__ bands, int block,
+                                              int nblocks) {
+  __shared__ unsigned s_b[kBands];
+  for (int k = threadIdx.x; k < kBands; k += blockDim.x) s_b[k] = 0u;
+  __syncthreads();
+  const int64_t nsl = (n_rot + rot_stride - 1) / rot_stride;
+  const int64_t total = nsl * g.P;
+  const float ih = (float)g.inv_H, iw = (float)g.inv_W;
+  for (int64_t t = (int64_t)block * blockDim.x + threadIdx.x; t < total; t += (int64_t)nblocks * blockDim.x) {
+    const int64_t sl = t / g.P;
+    const int p = (int)(t - sl * g.P);
+    const int i = p / g.Wr, j = p - i * g.Wr;
+    const f2v v = x[sl * rot_stride * g.ostride + p];
+    const float fy = (float)(i - g.half_H) * ih, fx = (float)j * iw;
+    atomicMax(&s_b[band_of(sqrtf(fy * fy + fx * fx))], __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kBands; k += blockDim.x)
+    if (s_b[k]) atomicMax(bands + k, s_b[k]);
+}
+
+__global__ __launch_bounds__(256) void band_max_kernel(const f2v *__restrict__ x, SliceGeom g, int64_t n_rot,
+                                                       int64_t rot_stride, unsigned *__restrict__ bands) {
+  band_max_body(x, g, n_rot, rot_stride, bands, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// Slices sampled for the band maxima: about 64 spread over the stack.
+__host__ __device__ inline int64_t band_stride(int64_t n_rot) { return n_rot > 64 ? n_rot / 64 : 1; }
 
 __global__ void set_word_kernel(unsigned *p, unsigned v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -1224,10 +1401,10 @@ __device__ __forceinline__ void mirror_one(const f2v *__restrict__ plane, const 
   const int64_t e = ((int64_t)z * H + y) * Wr;
   if (accumulate) {
     val += out[e];
-    w += wout[e];
+    if (wout) w += wout[e];
   }
   out[e] = val;
-  wout[e] = w;
+  if (wout) wout[e] = w;
 }
 
 __global__ __launch_bounds__(256) void mirror_kernel(const f2v *__restrict__ plane,
@@ -1448,6 +1625,13 @@ int check_engine(const SliceGeom &g, int64_t n_rot) {
   return TT_OK;
 }
 
+// ceil(log2(n)) for n >= 1 (0 for n <= 1).
+__host__ __device__ inline int ceil_log2(int64_t n) {
+  int b = 0;
+  while (b < 62 && ((int64_t)1 << b) < n) ++b;
+  return b;
+}
+
 // Scatter driver shared by the adjoint and the insertion.
 template <int BZ, int BY, int BX, int MODE>
 int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_rot, f2v *out,
@@ -1458,9 +1642,9 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   const BrickGrid bg = grid_of(g, BZ, BY, BX);
   const int64_t n_bricks = (int64_t)bg.nbz * bg.nby * bg.nbx;
   const size_t np = (size_t)g.vD * g.vH;
-  // scratch: words (max slots sampled, max slots exact, flag) | plane | wplane |
+  // scratch: words (max slots, flag, band maxima) | plane | wplane |
   // PlaneRec | PlaneTest | rotation-segment slots | halo shells
-  size_t off = ((2 * kSetWords + 1) * sizeof(unsigned) + 255) & ~(size_t)255;
+  size_t off = (kScaleWords * sizeof(unsigned) + 255) & ~(size_t)255;
   const size_t off_plane = off;
   if (MODE == kInsert) off += np * sizeof(f2v);
   const size_t off_wplane = off;
@@ -1477,7 +1661,7 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   off += (size_t)pe.n_slots * C::NACC * C::NT * sizeof(u64);
   off = (off + 255) & ~(size_t)255;
   const size_t off_halo = off;
-  off += (size_t)n_bricks * C::NACC * C::NSH * sizeof(u64);
+  off += (size_t)n_bricks * C::NACC * C::NSH * sizeof(float);
   off = (off + 255) & ~(size_t)255;
   const size_t off_touched = off;  // per brick: reached by a slice (written by every brick)
   off += (size_t)n_bricks * sizeof(int);
@@ -1490,33 +1674,43 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   PlaneRec *pre = (PlaneRec *)(base + off_pre);
   PlaneTest *ptest = (PlaneTest *)(base + off_test);
   u64 *segbuf = (u64 *)(base + off_seg);
-  u64 *halo = (u64 *)(base + off_halo);
+  float *halo = (float *)(base + off_halo);
   int *touched = (int *)(base + off_touched);
   if (int st = fill_zero<unsigned>(words, (int64_t)(zero_end / sizeof(unsigned)), s)) return st;
-  const int64_t n = n_rot * (int64_t)g.P;
-  // accumulate: exact max (the outputs cannot be rewritten); otherwise a
-  // sampled max with 8 bits of headroom and an exact redo if it was exceeded
+  // values the exact maximum reads: the whole (strided) stack, a bound for
+  // every channel of a multichannel one
+  const int64_t n = (n_rot - 1) * g.ostride + g.P;
+  // accumulate: the exact maximum of the input for every brick (the outputs
+  // cannot be rewritten); otherwise per-brick scales from sampled band maxima
+  // with 8 bits of headroom, and an exact redo if a value exceeded them
   const bool sampled = !accumulate;
   const int mgrid = (int)std::min<int64_t>(std::max<int64_t>(n / 4096, 1), 2048);
   if (n_rot > 0) {
-    // sampled: one 8192-value chunk in 64, about four values per lane
-    const int64_t nsamp = (n / (64 * 8192) + 1) * 8192;
-    const int sgrid = (int)std::min<int64_t>((nsamp + 1023) / 1024, 2048);
-    hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(sampled ? sgrid : mgrid), dim3(256), 0, s, src, n,
-                       (int64_t)(sampled ? 64 : 1), (const unsigned *)nullptr, words);
+    if (sampled) {
+      const int64_t bs = band_stride(n_rot);
+      const int64_t nv = (n_rot + bs - 1) / bs * g.P;
+      hipLaunchKernelGGL(band_max_kernel, dim3((unsigned)std::min<int64_t>((nv + 2047) / 2048, 1024)), dim3(256), 0,
+                         s, src, g, n_rot, bs, words + kBandWord);
+    } else {
+      hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
+                         (const unsigned *)nullptr, words + kSetWords);
+    }
     hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g,
-                       rot, (int)n_rot, pre, ptest);
-    if (int st = check_launch("maxabs_chunks / plane_rec")) return st;
+                       rot, (int)n_rot, pre, ptest, words + kSliceWord);
+    if (int st = check_launch("band_max / maxabs_chunks / plane_rec")) return st;
   }
   for (int pass = 0; pass < (sampled ? 2 : 1); ++pass) {
     ScaleArgs sa;
-    sa.maxbits = words + (pass ? kSetWords : 0);
-    sa.flag = words + 2 * kSetWords;
+    sa.maxbits = words + kSetWords;
+    sa.bandmax = (sampled && pass == 0) ? words + kBandWord : nullptr;
+    sa.flag = words + kFlagWord;
     sa.headroom = (sampled && pass == 0) ? 8 : 0;
-    sa.guard = pass;
+    sa.guard = sampled ? pass : 0;
+    sa.rotbits = ceil_log2(n_rot);
+    sa.slicebits = words + kSliceWord;
     if (pass == 1 && n_rot > 0) {
       hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
-                         (const unsigned *)(words + 2 * kSetWords), words + kSetWords);
+                         (const unsigned *)(words + kFlagWord), words + kSetWords);
       if (int st = check_launch("maxabs_chunks_kernel")) return st;
     }
     if (pe.n_units > 0) {
@@ -1532,13 +1726,14 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
     }
     if (pe.n_splits > 0) {
       hipLaunchKernelGGL((split_combine_kernel<BZ, BY, BX, MODE>),
-                         dim3((unsigned)pe.n_splits, (unsigned)((C::NT + 255) / 256)), dim3(256), 0, s, g, bg, (const SplitRec *)pe.splits, (const u64 *)segbuf, sa, accumulate,
+                         dim3((unsigned)pe.n_splits, (unsigned)((C::NT + 255) / 256)), dim3(256), 0, s, g, bg,
+                         (const SplitRec *)pe.splits, (const u64 *)segbuf, sa, accumulate,
                          out, wout, plane, wplane, halo, touched);
       if (int st = check_launch("split_combine_kernel")) return st;
     }
     hipLaunchKernelGGL((halo_combine_kernel<BZ, BY, BX, MODE>),
                        dim3((unsigned)n_bricks, (unsigned)((C::NSH + 255) / 256)), dim3(256), 0, s,
-                       g, bg, (const u64 *)halo, sa, out, wout, plane, wplane, (const int *)touched);
+                       g, bg, (const float *)halo, sa, out, wout, plane, wplane, (const int *)touched);
     if (int st = check_launch("halo_combine_kernel")) return st;
   }
   if (MODE == kInsert) {
@@ -1569,7 +1764,7 @@ struct BatchItemDev {
   f2v *plane;
   float *wplane;
   u64 *segbuf;
-  u64 *halo;
+  float *halo;
   int n_rot;
   int n_bricks;
   int64_t n_src;  // values in src (insertion)
@@ -1581,10 +1776,13 @@ struct BatchUnit {
 
 __device__ __forceinline__ ScaleArgs batch_scale(const BatchItemDev &it, int pass) {
   ScaleArgs sa;
-  sa.maxbits = it.words + (pass ? kSetWords : 0);
-  sa.flag = it.words + 2 * kSetWords;
+  sa.maxbits = it.words + kSetWords;
+  sa.bandmax = pass ? nullptr : it.words + kBandWord;
+  sa.flag = it.words + kFlagWord;
   sa.headroom = pass ? 0 : 8;
   sa.guard = pass;
+  sa.rotbits = ceil_log2(it.n_rot);
+  sa.slicebits = it.words + kSliceWord;
   return sa;
 }
 
@@ -1625,15 +1823,18 @@ __global__ __launch_bounds__(256) void plane_rec_batch_kernel(const BatchItemDev
   const int2 b = blocks[blockIdx.x];
   const BatchItemDev &it = items[b.x];
   const int r = b.y * 256 + threadIdx.x;
-  if (r < it.n_rot) plane_rec_one(it.g, it.rot, r, it.pre, it.ptest);
+  const int bits = r < it.n_rot ? plane_rec_one(it.g, it.rot, r, it.pre, it.ptest) : 0;
+  max_count_bits(bits, it.words ? it.words + kSliceWord : nullptr);
 }
 
 __global__ __launch_bounds__(256) void maxabs_batch_kernel(const BatchItemDev *__restrict__ items,
                                                            const int3 *__restrict__ blocks, int pass) {
   const int3 b = blocks[blockIdx.x];
   const BatchItemDev &it = items[b.x];
-  maxabs_body(it.src, it.n_src, pass ? 1 : 64, pass ? (const unsigned *)(it.words + 2 * kSetWords) : nullptr,
-              it.words + (pass ? kSetWords : 0), b.y, b.z);
+  if (pass == 0)
+    band_max_body(it.src, it.g, it.n_rot, band_stride(it.n_rot), it.words + kBandWord, b.y, b.z);
+  else
+    maxabs_body(it.src, it.n_src, 1, (const unsigned *)(it.words + kFlagWord), it.words + kSetWords, b.y, b.z);
 }
 
 template <int BZ, int BY, int BX, int MODE>
@@ -1728,7 +1929,7 @@ void batch_sizes(const SliceGeom &g, int n_slots, size_t &seg_bytes, size_t &hal
   const BrickGrid bg = grid_of(g, BZ, BY, BX);
   n_bricks = bg.nbz * bg.nby * bg.nbx;
   seg_bytes = MODE == kGather ? 0 : (size_t)n_slots * C::NACC * C::NT * sizeof(u64);
-  halo_bytes = MODE == kGather ? 0 : (size_t)n_bricks * C::NACC * C::NSH * sizeof(u64);
+  halo_bytes = MODE == kGather ? 0 : (size_t)n_bricks * C::NACC * C::NSH * sizeof(float);
 }
 
 int batch_item_sizes(int kind, int cfg, const SliceGeom &g, int n_slots, size_t &seg, size_t &halo,
@@ -1769,7 +1970,7 @@ int batch_build(BatchPlan &bp) {
   std::vector<size_t> words_off(n), plane_off(n), wplane_off(n);
   for (int i = 0; i < n; ++i) {
     words_off[i] = zoff;
-    zoff += ((2 * kSetWords + 1) * sizeof(unsigned) + 255) & ~(size_t)255;
+    zoff += (kScaleWords * sizeof(unsigned) + 255) & ~(size_t)255;
     const size_t np = (size_t)geo[i].vD * geo[i].vH;
     plane_off[i] = zoff;
     if (!ext) zoff += (np * sizeof(f2v) + 255) & ~(size_t)255;
@@ -1816,7 +2017,7 @@ int batch_build(BatchPlan &bp) {
     d.plane = ext ? nullptr : (f2v *)(z + plane_off[i]);
     d.wplane = ext ? nullptr : (float *)(z + wplane_off[i]);
     d.segbuf = (u64 *)(sc + seg_off[i]);
-    d.halo = (u64 *)(sc + halo_off[i]);
+    d.halo = (float *)(sc + halo_off[i]);
     d.n_rot = (int)it.n_rot;
     d.n_bricks = nbr[i];
     d.n_src = it.n_rot * (int64_t)geo[i].P;
@@ -1872,8 +2073,9 @@ int batch_build(BatchPlan &bp) {
       for (int64_t b = 0; b < nb; ++b) zb.push_back(make_int2(i, (int)b));
     } else {
       const int64_t nv = nr * (int64_t)geo[i].P;
-      const int64_t nsamp = (nv / (64 * 8192) + 1) * 8192;
-      const int s0 = (int)std::min<int64_t>((nsamp + 1023) / 1024, 2048);
+      const int64_t bs = band_stride(nr);
+      const int64_t nv0 = (nr + bs - 1) / bs * geo[i].P;
+      const int s0 = (int)std::min<int64_t>((nv0 + 2047) / 2048, 1024);
       const int s1 = (int)std::min<int64_t>(std::max<int64_t>(nv / 4096, 1), 2048);
       for (int b = 0; b < s0; ++b) m0.push_back(make_int3(i, b, s0));
       for (int b = 0; b < s1; ++b) m1.push_back(make_int3(i, b, s1));
@@ -2024,21 +2226,21 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
   PlaneRec *pre = (PlaneRec *)sc.p;
   PlaneTest *ptest = (PlaneTest *)((char *)sc.p + off_test);
   hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
-                     (int)n_rot, pre, ptest);
+                     (int)n_rot, pre, ptest, (unsigned *)nullptr);
   if (int st = check_launch("plane_rec_kernel")) return st;
   const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
-  ScaleArgs sa = {nullptr, nullptr, 0, 0};
+  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
   constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
   if (g.use_mask)
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)pe.n_units), dim3(T),
                        0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot, out,
                        (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (u64 *)nullptr, sa, 0, (int *)nullptr);
+                       (float *)nullptr, sa, 0, (int *)nullptr);
   else
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)pe.n_units),
                        dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
-                       (u64 *)nullptr, (u64 *)nullptr, sa, 0, (int *)nullptr);
+                       (u64 *)nullptr, (float *)nullptr, sa, 0, (int *)nullptr);
   return check_launch("brick_kernel");
 }
 
@@ -2063,21 +2265,21 @@ int engine_extract2(const SliceGeom &g, const f2v *vol, const float *rot, int64_
   PlaneRec *pre = (PlaneRec *)sc.p;
   PlaneTest *ptest = (PlaneTest *)((char *)sc.p + off_test);
   hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
-                     (int)n_rot, pre, ptest);
+                     (int)n_rot, pre, ptest, (unsigned *)nullptr);
   if (int st = check_launch("plane_rec_kernel")) return st;
   const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
-  ScaleArgs sa = {nullptr, nullptr, 0, 0};
+  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
   constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
   if (g.use_mask)
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true, 2>), dim3((unsigned)pe.n_units), dim3(T),
                        0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (u64 *)nullptr, sa, 0, (int *)nullptr);
+                       (float *)nullptr, sa, 0, (int *)nullptr);
   else
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false, 2>), dim3((unsigned)pe.n_units), dim3(T),
                        0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (u64 *)nullptr, sa, 0, (int *)nullptr);
+                       (float *)nullptr, sa, 0, (int *)nullptr);
   return check_launch("brick_kernel");
 }
 

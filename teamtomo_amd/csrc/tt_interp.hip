@@ -93,6 +93,102 @@ __global__ __launch_bounds__(kBlock) void sample3d_kernel(const T *__restrict__ 
                             out + p * C * E);
 }
 
+// grid_sample's source coordinate of array coordinate c on an axis of n
+// samples, in the sampler's precision: the normalisation of
+// grid_sample_utils.py:24, g = c / (0.5 n - 0.5) - 1, undone as
+// u = ((g + 1) / 2) (n - 1) (align_corners=True), unclipped.  The round trip
+// can move an integer coordinate by an ulp, which decides the cell (and so
+// the one-sided derivative) there.
+__device__ __forceinline__ float gs_source(float c, int n) {
+  if (n <= 1) return 0.0f;
+  float sc = __fsub_rn(__fmul_rn(0.5f, (float)n), 0.5f);
+  float g = __fsub_rn(__fdiv_rn(c, sc), 1.0f);
+  return __fmul_rn(__fdiv_rn(__fadd_rn(g, 1.0f), 2.0f), (float)(n - 1));
+}
+
+__device__ __forceinline__ double gs_source(double c, int n) {
+  if (n <= 1) return 0.0;
+  double sc = __dsub_rn(__dmul_rn(0.5, (double)n), 0.5);
+  double g = __dsub_rn(__ddiv_rn(c, sc), 1.0);
+  return __dmul_rn(__ddiv_rn(__dadd_rn(g, 1.0), 2.0), (double)(n - 1));
+}
+
+// Backward of sample3d_point for one sample: the image gradient (trilinear
+// weights of the floor / floor+1 corners, or the nearest voxel, added with
+// global atomics into a zeroed gimg) and the coordinate gradient,
+// Re(conj(g) ds/dc), from the analytic derivative of the weights.  As in
+// grid_sample's backward with border padding and align_corners
+// (image_interpolation_3d.py:64-75), the weights are those of the source
+// coordinate u (gs_source) clipped to [0, N-1], the derivative along an axis
+// is 0 where the clip holds it (u <= 0 or u >= N-1), nearest sampling has no
+// coordinate gradient, and samples outside the image (zeroed in the forward,
+// :84-89) get no gradient at all.
+template <typename T, bool CPLX>
+__device__ __forceinline__ void sample3d_point_bwd(const T *__restrict__ img, int C, int d, int h, int w, T cz,
+                                                   T cy, T cx, int mode, const T *__restrict__ g,
+                                                   T *__restrict__ gimg, T (&gc)[3]) {
+  const int E = CPLX ? 2 : 1;
+  const int64_t plane = (int64_t)h * w, vol = (int64_t)d * plane;
+  gc[0] = gc[1] = gc[2] = 0;
+  const bool ok = cz >= 0 && cz <= (T)(d - 1) && cy >= 0 && cy <= (T)(h - 1) && cx >= 0 && cx <= (T)(w - 1);
+  if (!ok) return;
+  if (mode == TT_INTERP_NEAREST) {
+    if (!gimg) return;
+    int z = (int)nearest_index_f(cz, d), y = (int)nearest_index_f(cy, h), x = (int)nearest_index_f(cx, w);
+    int64_t e = (int64_t)z * plane + (int64_t)y * w + x;
+    for (int ch = 0; ch < C; ++ch)
+      for (int k = 0; k < E; ++k) atomicAdd(&gimg[(ch * vol + e) * E + k], g[ch * E + k]);
+    return;
+  }
+  const T uz = gs_source(cz, d), uy = gs_source(cy, h), ux = gs_source(cx, w);
+  const bool clz = uz <= 0 || uz >= (T)(d - 1), cly = uy <= 0 || uy >= (T)(h - 1),
+             clx = ux <= 0 || ux >= (T)(w - 1);
+  const T vz = fmin(fmax(uz, (T)0), (T)(d - 1)), vy = fmin(fmax(uy, (T)0), (T)(h - 1)),
+          vx = fmin(fmax(ux, (T)0), (T)(w - 1));
+  int z0 = (int)floor(vz), y0 = (int)floor(vy), x0 = (int)floor(vx);
+  T tz = vz - (T)z0, ty = vy - (T)y0, tx = vx - (T)x0;
+  for (int q = 0; q < 8; ++q) {
+    int dz = q >> 2, dy = (q >> 1) & 1, dx = q & 1;
+    int z = z0 + dz, y = y0 + dy, x = x0 + dx;
+    if (z >= d || y >= h || x >= w) continue;  // weight and derivatives are 0 there
+    const T wz = dz ? tz : (T)1 - tz, wy = dy ? ty : (T)1 - ty, wx = dx ? tx : (T)1 - tx;
+    const T sz = dz ? (T)1 : (T)-1, sy = dy ? (T)1 : (T)-1, sx = dx ? (T)1 : (T)-1;
+    const T wt = wz * wy * wx;
+    const T dwz = sz * wy * wx, dwy = wz * sy * wx, dwx = wz * wy * sx;
+    for (int ch = 0; ch < C; ++ch) {
+      int64_t e = ch * vol + (int64_t)z * plane + (int64_t)y * w + x;
+      for (int k = 0; k < E; ++k) {
+        const T gk = g[ch * E + k], v = img[e * E + k];
+        if (gimg) atomicAdd(&gimg[e * E + k], wt * gk);
+        gc[0] += gk * v * dwz;
+        gc[1] += gk * v * dwy;
+        gc[2] += gk * v * dwx;
+      }
+    }
+  }
+  if (clz) gc[0] = 0;
+  if (cly) gc[1] = 0;
+  if (clx) gc[2] = 0;
+}
+
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(kBlock) void sample3d_bwd_kernel(const T *__restrict__ img, int C, int d, int h,
+                                                              int w, const T *__restrict__ coords, int64_t n,
+                                                              int mode, const T *__restrict__ gout,
+                                                              T *__restrict__ gimg, T *__restrict__ gcoords) {
+  const int E = CPLX ? 2 : 1;
+  for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock) {
+    T gc[3];
+    sample3d_point_bwd<T, CPLX>(img, C, d, h, w, coords[3 * p], coords[3 * p + 1], coords[3 * p + 2], mode,
+                                gout + p * C * E, gimg, gc);
+    if (gcoords) {
+      gcoords[3 * p] = gc[0];
+      gcoords[3 * p + 1] = gc[1];
+      gcoords[3 * p + 2] = gc[2];
+    }
+  }
+}
+
 // Fused affine warp (affine_transform_image_3d): output voxel (z, y, x) of
 // leading batch index t samples the image at M (z, y, x, 1)^T, M the 3x4 zyx
 // block (row-major, float32, as the reference's float32 matmul of the
@@ -326,6 +422,25 @@ int sample_impl(const T *image, int channels, int d, int h, int w, int is_comple
 }
 
 template <typename T>
+int sample_bwd_impl(const T *image, int channels, int d, int h, int w, int is_complex, const T *coords,
+                    int64_t n, int mode, const T *grad_out, T *grad_image, T *grad_coords, void *stream) {
+  int st = check_image(channels, d, h, w);
+  if (st) return st;
+  if (mode != TT_INTERP_NEAREST && mode != TT_INTERP_TRILINEAR)
+    return fail_invalid("unknown interpolation mode");
+  if (n <= 0 || (!grad_image && !grad_coords)) return TT_OK;
+  if (!image || !coords || !grad_out) return fail_invalid("NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (is_complex)
+    hipLaunchKernelGGL((sample3d_bwd_kernel<T, true>), dim3(grid_for(n)), dim3(kBlock), 0, s, image,
+                       channels, d, h, w, coords, n, mode, grad_out, grad_image, grad_coords);
+  else
+    hipLaunchKernelGGL((sample3d_bwd_kernel<T, false>), dim3(grid_for(n)), dim3(kBlock), 0, s, image,
+                       channels, d, h, w, coords, n, mode, grad_out, grad_image, grad_coords);
+  return check_launch("sample3d_bwd_kernel");
+}
+
+template <typename T>
 int splat_impl(const T *values, const float *coords, int64_t n, int channels, int is_complex, int d,
                int h, int w, int mode, T *image, void *weights, int weights_f64, void *stream) {
   int st = check_image(channels, d, h, w);
@@ -456,6 +571,20 @@ int tt_sample3d_f32(const float *image, int channels, int d, int h, int w, int i
 int tt_sample3d_f64(const double *image, int channels, int d, int h, int w, int is_complex,
                     const double *coords, int64_t n, int mode, double *out, void *stream) {
   return sample_impl<double>(image, channels, d, h, w, is_complex, coords, n, mode, out, stream);
+}
+
+int tt_sample3d_bwd_f32(const float *image, int channels, int d, int h, int w, int is_complex,
+                        const float *coords, int64_t n, int mode, const float *grad_out, float *grad_image,
+                        float *grad_coords, void *stream) {
+  return sample_bwd_impl<float>(image, channels, d, h, w, is_complex, coords, n, mode, grad_out, grad_image,
+                                grad_coords, stream);
+}
+
+int tt_sample3d_bwd_f64(const double *image, int channels, int d, int h, int w, int is_complex,
+                        const double *coords, int64_t n, int mode, const double *grad_out, double *grad_image,
+                        double *grad_coords, void *stream) {
+  return sample_bwd_impl<double>(image, channels, d, h, w, is_complex, coords, n, mode, grad_out, grad_image,
+                                 grad_coords, stream);
 }
 
 int tt_splat3d_f32(const float *values, const float *coords, int64_t n, int channels,

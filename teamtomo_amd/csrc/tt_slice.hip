@@ -347,7 +347,7 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 11; }
+int tt_abi_version(void) { return 12; }
 
 static int g_gather_path = TT_GATHER_AUTO;
 
@@ -490,6 +490,38 @@ int tt_insert_c64(const void *slices, int vol_d, int vol_h, int vol_w, const flo
                   int flags, void *stream) {
   return insert_impl<float>(slices, vol_d, vol_h, vol_w, rot_zyx, n_rot, fftfreq_max, volume,
                             weights, flags, stream);
+}
+
+// Multichannel insertion: every channel through the brick engine at the same
+// coordinates, reading its channel of the (n, C, H, Wr) stack in place; the
+// weights (which do not depend on the values) are accumulated with channel 0
+// only.
+int tt_insert_mc_c64(const void *slices, int channels, int vol_d, int vol_h, int vol_w, const float *rot_zyx,
+                     int64_t n_rot, double fftfreq_max, void *volume, float *weights, int flags, void *stream) {
+  const int vol_wr = vol_w / 2 + 1;
+  int st = check_common(vol_d, vol_h, vol_wr, vol_d, vol_h, vol_w, n_rot, rot_zyx);
+  if (st) return st;
+  if (channels < 1) return fail_invalid("channels must be >= 1");
+  if (!volume || !weights) return fail_invalid("volume / weights pointer is NULL");
+  if (n_rot > 0 && !slices) return fail_invalid("slices pointer is NULL");
+  if (flags & TT_FLAG_PIXEL_ATOMIC) return fail_invalid("tt_insert_mc_c64 runs the brick engine only");
+  SliceGeom g = make_geom(vol_d, vol_h, vol_w, vol_d, vol_h, vol_wr, fftfreq_max, 1e-8);
+  if (!engine_supports(g, n_rot)) return fail_invalid("geometry beyond the brick engine's limits");
+  hipStream_t s = (hipStream_t)stream;
+  const bool overwrite = flags & TT_FLAG_OVERWRITE;
+  const int64_t nv = (int64_t)vol_d * vol_h * vol_wr;
+  if (n_rot == 0) {
+    if (!overwrite) return TT_OK;
+    if ((st = zero_fill<float>((float *)volume, 2 * nv * channels, s))) return st;
+    return zero_fill<float>(weights, nv, s);
+  }
+  g.ostride = (int64_t)channels * g.P;
+  for (int c = 0; c < channels; ++c) {
+    st = engine_insert(g, (const f2v *)slices + (int64_t)c * g.P, rot_zyx, n_rot, (f2v *)volume + c * nv,
+                       c == 0 ? weights : nullptr, !overwrite, s);
+    if (st) return st;
+  }
+  return TT_OK;
 }
 
 int tt_insert_c128(const void *slices, int vol_d, int vol_h, int vol_w, const float *rot_zyx,

@@ -302,16 +302,79 @@ def insert_central_slices_rfft_3d_multichannel(
     """Multichannel insertion: (..., c, h, w//2+1) -> ((c, d, h, w//2+1), weights).
 
     Reference: slice_insertion/_insert_central_slices_rfft_3d.py:109-210; the
-    weights are shared by the channels.
+    weights are shared by the channels.  complex64: one call
+    (tt_insert_mc_c64) reads every channel in place and accumulates the
+    weights once; complex128 or the pixel-atomic path: a loop over channels.
     """
-    vols, wts = [], None
-    for c in range(image_rfft.shape[-3]):
-        v, w = insert_central_slices_rfft_3d(
-            image_rfft[..., c, :, :], volume_shape, rotation_matrices, fftfreq_max, zyx_matrices
+    _lib.require_gpu(image_rfft)
+    if image_rfft.dtype not in (torch.complex64, torch.complex128):
+        raise TypeError(f"insert_central_slices_rfft_3d_multichannel: complex input required, got {image_rfft.dtype}")
+    if image_rfft.dtype == torch.complex128 or _config.pixel_atomic_enabled():
+        vols, wts = [], None
+        for c in range(image_rfft.shape[-3]):
+            v, w = insert_central_slices_rfft_3d(
+                image_rfft[..., c, :, :], volume_shape, rotation_matrices, fftfreq_max, zyx_matrices
+            )
+            vols.append(v)
+            wts = w
+        return torch.stack(vols, dim=0), wts
+    D, H, W = (int(s) for s in volume_shape)
+    Wr = _rfft_width(W)
+    C = image_rfft.shape[-3]
+    if tuple(image_rfft.shape[-2:]) != (H, Wr):
+        raise IndexError(
+            f"The shape of the mask [{H}, {Wr}] does not match the shape of the indexed tensor "
+            f"{list(image_rfft.shape)}"
         )
-        vols.append(v)
-        wts = w
-    return torch.stack(vols, dim=0), wts
+    R = _prepare_rotations(rotation_matrices, image_rfft.device, zyx_matrices, None)
+    if tuple(image_rfft.shape[:-3]) != tuple(R.shape[:-2]):
+        raise ValueError("One coordinate triplet is required for each value in data.")
+    slices = image_rfft.reshape(-1, C, H, Wr).contiguous()
+    R = R.reshape(-1, 3, 3).contiguous()
+    return _InsertMCFn.apply(slices, R, (D, H, W), fftfreq_max)
+
+
+class _InsertMCFn(torch.autograd.Function):
+    """(c, d, h, wr) volumes and shared weights from (n, c, h, wr) slices;
+    backward: the adjoint gather of every channel, and the rotation gradient
+    summed over the channels (the weights' part counted once)."""
+
+    @staticmethod
+    def forward(ctx, slices, R, volume_shape, fftfreq_max):
+        D, H, W = (int(s) for s in volume_shape)
+        Wr = _rfft_width(W)
+        n, C = slices.shape[0], slices.shape[1]
+        vol = torch.empty((C, D, H, Wr), dtype=slices.dtype, device=slices.device)
+        wts = torch.empty((D, H, Wr), dtype=torch.float32, device=slices.device)
+        _lib.call("tt_insert_mc_c64", _lib.ptr(slices), C, D, H, W, _lib.ptr(R), n, _fmax_arg(fftfreq_max),
+                  _lib.ptr(vol), _lib.ptr(wts), _scatter_flags(), _lib.stream_of(slices))
+        ctx.save_for_backward(slices, R)
+        ctx.geom = (D, H, W, fftfreq_max)
+        return vol, wts
+
+    @staticmethod
+    def backward(ctx, grad_volume, grad_weights):
+        slices, R = ctx.saved_tensors
+        D, H, W, fftfreq_max = ctx.geom
+        need_s, need_r = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_s or need_r):
+            return None, None, None, None
+        C = slices.shape[1]
+        gS = torch.empty_like(slices) if need_s else None
+        gR = torch.zeros((R.shape[0], 3, 3), dtype=torch.float64, device=R.device) if need_r else None
+        gW = grad_weights.to(torch.float32).contiguous() if grad_weights is not None else None
+        for c in range(C):
+            gV = (grad_volume[c].to(slices.dtype).contiguous() if grad_volume is not None
+                  else torch.zeros((D, H, _rfft_width(W)), dtype=slices.dtype, device=slices.device))
+            sc = slices[:, c].contiguous() if need_r else None
+            gs = torch.empty((slices.shape[0], H, _rfft_width(W)), dtype=slices.dtype,
+                             device=slices.device) if need_s else None
+            _lib.call("tt_insert_bwd_c64", _lib.ptr(gV), _lib.ptr(gW if c == 0 else None), _lib.ptr(sc),
+                      D, H, W, _lib.ptr(R), R.shape[0], _fmax_arg(fftfreq_max), _lib.ptr(gs), _lib.ptr(gR),
+                      _lib.stream_of(slices))
+            if need_s:
+                gS[:, c] = gs
+        return gS, (gR.to(R.dtype) if gR is not None else None), None, None
 
 
 def _sinc2_grid(shape, device):
@@ -379,7 +442,11 @@ def project_3d_to_2d(
         fftfreq_max=fftfreq_max, zyx_matrices=zyx_matrices,
     )
     if projections.numel() == 0:  # no poses: hipFFT refuses empty batches
-        return torch.empty(projections.shape[:-2] + (d, d), dtype=volume.dtype, device=volume.device)
+        # the shape irfftn (without s) and the crop give: the last axis is
+        # 2 (Wr - 1), one less than the padded box when that is odd
+        hp, wr = projections.shape[-2:]
+        shape = (hp - 2 * p, 2 * (wr - 1) - 2 * p)
+        return torch.empty(projections.shape[:-2] + shape, dtype=volume.dtype, device=volume.device)
     projections = torch.fft.ifftshift(projections, dim=(-2,))
     projections = torch.fft.irfftn(projections, dim=(-2, -1))
     projections = torch.fft.ifftshift(projections, dim=(-2, -1))
@@ -395,12 +462,40 @@ def project_3d_to_2d_multichannel(
     fftfreq_max: float | None = None,
     zyx_matrices: bool = False,
 ) -> torch.Tensor:
-    """Multichannel projection (c, d, d, d) -> (..., c, d, d) (TFS/project.py:96-177)."""
-    outs = [
-        project_3d_to_2d(volume[c], rotation_matrices, pad_factor, fftfreq_max, zyx_matrices)
-        for c in range(volume.shape[0])
-    ]
-    return torch.stack(outs, dim=-3)
+    """Multichannel projection (c, d, d, d) -> (..., c, d, d) (TFS/project.py:96-177).
+
+    The fused pad / sinc^2 / fftshift prologue per channel, one batched rfftn,
+    then one multichannel extraction (the brick gather serves two channels
+    per launch), as the reference makes one multichannel extraction call.
+    """
+    d, h, w = volume.shape[-3:]
+    if len({d, h, w}) != 1:
+        raise ValueError("all dimensions of volume must be equal.")
+    if pad_factor < 1.0:
+        raise ValueError("pad_factor must be >= 1.0")
+    p = 0
+    if pad_factor > 1.0:
+        p = int((volume.shape[-1] * (pad_factor - 1.0)) // 2)
+    volume_shape = (d + 2 * p,) * 3
+    dft = torch.stack([_PadSinc2Shift.apply(volume[c], p) for c in range(volume.shape[0])], dim=0)
+    dft = torch.fft.rfftn(dft, dim=(-3, -2, -1))
+    dft = torch.fft.fftshift(dft, dim=(-3, -2))
+    projections = extract_central_slices_rfft_3d_multichannel(
+        volume_rfft=dft, image_shape=volume_shape, rotation_matrices=rotation_matrices,
+        fftfreq_max=fftfreq_max, zyx_matrices=zyx_matrices,
+    )
+    if projections.numel() == 0:  # no poses: hipFFT refuses empty batches
+        # the shape irfftn (without s) and the crop give: the last axis is
+        # 2 (Wr - 1), one less than the padded box when that is odd
+        hp, wr = projections.shape[-2:]
+        shape = (hp - 2 * p, 2 * (wr - 1) - 2 * p)
+        return torch.empty(projections.shape[:-2] + shape, dtype=volume.dtype, device=volume.device)
+    projections = torch.fft.ifftshift(projections, dim=(-2,))
+    projections = torch.fft.irfftn(projections, dim=(-2, -1))
+    projections = torch.fft.ifftshift(projections, dim=(-2, -1))
+    if pad_factor > 1.0:
+        projections = F.pad(projections, pad=[-p] * 4)
+    return projections
 
 
 def backproject_2d_to_3d(
@@ -581,7 +676,7 @@ def project_2d_to_1d(
         fftfreq_max=fftfreq_max, yx_matrices=yx_matrices,
     )
     if lines.numel() == 0:  # no rotations: hipFFT refuses empty batches
-        n = image_shape[-1] - (2 * p if pad_factor > 1.0 else 0)
+        n = 2 * (lines.shape[-1] - 1) - 2 * p  # irfftn's length (without s), then the crop
         return torch.empty(lines.shape[:-1] + (n,), dtype=image.dtype, device=image.device)
     projections = torch.fft.irfftn(lines, dim=(-1,))
     projections = torch.fft.ifftshift(projections, dim=(-1,))

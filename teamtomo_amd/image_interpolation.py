@@ -4,9 +4,11 @@ Reference: packages/primitives/torch-image-interpolation/src/
 torch_image_interpolation/image_interpolation_3d.py, image_interpolation_2d.py
 and image_interpolation_1d.py.
 Sampling and splatting run as gfx950 HIP kernels through the C ABI
-(include/teamtomo_amd.h).  These generic samplers are forward-only: a call
-whose image or values require grad raises, rather than returning a result
-without a gradient.
+(include/teamtomo_amd.h).  ``sample_image_3d`` is differentiable with respect
+to the image and the coordinates, as the reference's grid_sample is; its
+backward is a HIP kernel (tt_sample3d_bwd_*).  The other samplers and the
+splats are forward-only: a call whose inputs require grad raises, rather than
+returning a result without a gradient.
 """
 
 from __future__ import annotations
@@ -56,6 +58,42 @@ def _no_grad(*tensors) -> None:
         )
 
 
+class _Sample3dFn(torch.autograd.Function):
+    """samples (n, c) of a (c, d, h, w) image at (n, 3) coordinates; backward:
+    the image gradient (transposed trilinear weights) and the coordinate
+    gradient (analytic weight derivatives), one kernel."""
+
+    @staticmethod
+    def forward(ctx, img, crd, mode):
+        C, d, h, w = img.shape
+        n = crd.shape[0]
+        real_t = _REAL_OF[img.dtype]
+        out = torch.empty((n, C), dtype=img.dtype, device=img.device)
+        fn = "tt_sample3d_f32" if real_t == torch.float32 else "tt_sample3d_f64"
+        _lib.call(fn, _lib.ptr(img), C, d, h, w, int(img.is_complex()), _lib.ptr(crd), n, mode, _lib.ptr(out),
+                  _lib.stream_of(img))
+        ctx.save_for_backward(img, crd)
+        ctx.mode = mode
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        img, crd = ctx.saved_tensors
+        need_i, need_c = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_i or need_c):
+            return None, None, None
+        C, d, h, w = img.shape
+        n = crd.shape[0]
+        real_t = _REAL_OF[img.dtype]
+        g = grad_out.to(img.dtype).contiguous()
+        gi = torch.zeros_like(img) if need_i else None
+        gc = torch.empty_like(crd) if need_c else None
+        fn = "tt_sample3d_bwd_f32" if real_t == torch.float32 else "tt_sample3d_bwd_f64"
+        _lib.call(fn, _lib.ptr(img), C, d, h, w, int(img.is_complex()), _lib.ptr(crd), n, ctx.mode, _lib.ptr(g),
+                  _lib.ptr(gi), _lib.ptr(gc), _lib.stream_of(img))
+        return gi, gc, None
+
+
 def sample_image_3d(
     image: torch.Tensor,
     coordinates: torch.Tensor,
@@ -64,12 +102,15 @@ def sample_image_3d(
     """Sample a (d, h, w) or (c, d, h, w) image at (..., 3) zyx array coordinates.
 
     Reference: image_interpolation_3d.py:11-101.  Returns (...) or (..., c);
-    points outside [0, N-1] in any dimension give 0.
+    points outside [0, N-1] in any dimension give 0.  Differentiable with
+    respect to ``image`` and ``coordinates`` (grid_sample's gradients with
+    border padding: 0 along an axis where the coordinate sits on or beyond
+    the border, 0 for samples outside the image, no coordinate gradient for
+    nearest sampling).
     """
     if image.ndim not in (3, 4):
         raise ValueError(f"image should have shape (d, h, w) or (c, d, h, w), got {image.shape}")
     _lib.require_gpu(image)
-    _no_grad(image, coordinates)
     if image.dtype not in _REAL_OF:
         raise TypeError(f"sample_image_3d: unsupported image dtype {image.dtype}")
     real_t = _REAL_OF[image.dtype]
@@ -81,17 +122,10 @@ def sample_image_3d(
     mode = _mode(interpolation)
     multichannel = image.ndim == 4
     img = image if multichannel else image.unsqueeze(0)
-    C, d, h, w = img.shape
     lead = tuple(coordinates.shape[:-1])
     crd = coordinates.to(device=image.device, dtype=real_t).reshape(-1, 3).contiguous()
-    n = crd.shape[0]
-    is_c = image.is_complex()
-    img = img.contiguous()
-    out = torch.empty((n, C), dtype=image.dtype, device=image.device)
-    fn = "tt_sample3d_f32" if real_t == torch.float32 else "tt_sample3d_f64"
-    _lib.call(fn, _lib.ptr(img), C, d, h, w, int(is_c), _lib.ptr(crd), n, mode, _lib.ptr(out),
-              _lib.stream_of(image))
-    out = out.reshape(*lead, C)
+    out = _Sample3dFn.apply(img.contiguous(), crd, mode)
+    out = out.reshape(*lead, img.shape[0])
     return out if multichannel else out[..., 0]
 
 

@@ -5,8 +5,11 @@ transforms_3d.py:10-40 and transforms_2d.py:11-178.  The reference builds the
 dense coordinate grid, multiplies it by the matrices and samples it with
 sample_image_3d / sample_image_2d; here one HIP kernel computes each output
 voxel's coordinate M (z, y, x, 1)^T and samples it (``tt_affine3d_f32``,
-``tt_affine2d_f32``), so no grid is materialised.  Forward only: a call whose
-image requires grad raises.
+``tt_affine2d_f32``), so no grid is materialised.  The 3-D warp is
+differentiable with respect to the image and the matrices: when either needs
+a gradient it takes the reference's composition (coordinate grid, matmul,
+then the differentiable sample_image_3d).  The 2-D warps are forward only: a
+call whose image requires grad raises.
 """
 
 from __future__ import annotations
@@ -17,7 +20,7 @@ from typing import Literal, Optional
 import torch
 
 from . import _lib
-from .image_interpolation import _mode, _no_grad
+from .image_interpolation import _mode, _no_grad, sample_image_3d
 
 __all__ = [
     "affine_transform_image_3d",
@@ -27,9 +30,10 @@ __all__ = [
 ]
 
 
-def _check_image(image, ndim, name):
+def _check_image(image, ndim, name, grad_ok=False):
     _lib.require_gpu(image)
-    _no_grad(image)
+    if not grad_ok:
+        _no_grad(image)
     if image.dtype not in (torch.float32, torch.complex64):
         # the reference samples with a float32 coordinate grid, which grid_sample
         # refuses for float64 / complex128 images
@@ -87,9 +91,11 @@ def affine_transform_image_3d(
     reference's matmul: a (4, 4) matrix gives (d, h, w), a (b, 1, 1, 1, 4, 4)
     stack gives (b, d, h, w); multichannel images add a trailing c.
     """
-    _check_image(image, 3, "affine_transform_image_3d")
+    _check_image(image, 3, "affine_transform_image_3d", grad_ok=True)
     d, h, w = (int(v) for v in (output_shape if output_shape else image.shape[-3:]))
     mode = _mode(interpolation)
+    if torch.is_grad_enabled() and (image.requires_grad or (torch.is_tensor(matrices) and matrices.requires_grad)):
+        return _affine_3d_differentiable(image, matrices, interpolation, (d, h, w), zyx_matrices)
     blk, lead, n, (md, mh, mw) = _matrices(matrices, image.device, (d, h, w), not zyx_matrices)
     multichannel = image.ndim == 4
     img = (image if multichannel else image.unsqueeze(0)).contiguous()
@@ -99,6 +105,26 @@ def affine_transform_image_3d(
               d, h, w, mode, _lib.ptr(out), _lib.stream_of(image))
     out = out.reshape(*lead, d, h, w, C)
     return out if multichannel else out[..., 0]
+
+
+def _affine_3d_differentiable(image, matrices, interpolation, shape, zyx_matrices):
+    """The reference's composition (transforms_3d.py:22-40) in torch operations
+    around the differentiable sampler: gradients reach the image through the
+    sampler's backward kernel and the matrices through the matmul."""
+    M = torch.as_tensor(matrices, device=image.device)
+    if M.dtype != torch.float32:
+        raise RuntimeError(f"expected m1 and m2 to have the same dtype, but got: {M.dtype} != float")
+    if M.shape[-2:] != (4, 4):
+        raise ValueError(f"matrices must have shape (..., 4, 4), got {tuple(M.shape)}")
+    if not zyx_matrices:  # xyz -> zyx, out of place
+        A = torch.flip(M[..., :3, :3], dims=(-2, -1))
+        t = torch.flip(M[..., :3, 3:], dims=(-2,))
+        M = torch.cat([torch.cat([A, t], dim=-1), M[..., 3:, :]], dim=-2)
+    axes = [torch.arange(n, dtype=torch.float32, device=image.device) for n in shape]
+    grid = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1)  # (d, h, w, zyx)
+    grid = torch.cat([grid, torch.ones(shape + (1,), device=image.device)], dim=-1)[..., None]
+    grid = (M @ grid)[..., :3, 0]
+    return sample_image_3d(image, grid, interpolation)
 
 
 def affine_transform_image_2d(

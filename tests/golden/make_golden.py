@@ -2,8 +2,9 @@
 
 Run once, on a machine that has the reference sources, with
 ``python tests/golden/make_golden.py [REFERENCE_ROOT] [--2d]`` (default
-``/root/reference``; ``--2d`` / ``--transform`` write only the fixtures of the 2-D
-analogues / of the affine warps).  It imports the reference's pure-Python packages
+``/root/reference``; ``--2d`` / ``--transform`` / ``--mc-grad`` write only the
+fixtures of the 2-D analogues / of the affine warps / of the multichannel
+operators and the sampler's autograd).  It imports the reference's pure-Python packages
 (torch-fourier-slice, torch-image-interpolation, torch-grid-utils) on the CPU
 and stores inputs and outputs as small ``.npz`` fixtures next to this script.
 Nothing at test time reads the reference: the tests load the fixtures.
@@ -449,6 +450,86 @@ def main_transform(root):
     save("subpixel_crop", **out)
 
 
+def main_mc_grad(root):
+    """Multichannel extraction, insertion and projection
+    (slice_extraction/_extract_central_slices_rfft_3d.py:225-367,
+    slice_insertion/_insert_central_slices_rfft_3d.py:109-210,
+    project.py:96-177), and the torch-autograd gradients of sample_image_3d
+    (image_interpolation_3d.py:11-101) and affine_transform_image_3d
+    (transforms_3d.py:10-40) with respect to the image, the coordinates and
+    the matrices."""
+    import torch
+
+    torch.set_num_threads(8)
+    tfs, tii = _import_reference(root)
+    import torch_transform_image as tti  # noqa: E402
+
+    def save(name, **arrays):
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
+        print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+    # G17: multichannel operators, three channels (an odd count)
+    g = torch.Generator().manual_seed(41)
+    vol = torch.randn(3, 16, 16, 16, generator=g)
+    V = vol_rfft(vol)
+    R = uniform_rotations(5, 42)
+    out = {"volume_rfft": V.numpy(), "rotations": R.numpy()}
+    out["slices"] = tfs.extract_central_slices_rfft_3d_multichannel(V, (16, 16, 16), R.clone()).numpy()
+    out["slices_fmax"] = tfs.extract_central_slices_rfft_3d_multichannel(
+        V, (16, 16, 16), R.clone(), fftfreq_max=0.3).numpy()
+    S = torch.complex(torch.randn(5, 3, 16, 9, generator=g), torch.randn(5, 3, 16, 9, generator=g))
+    Vi, Wi = tfs.insert_central_slices_rfft_3d_multichannel(S, (16, 16, 16), R.clone())
+    out["ins_slices"], out["ins_volume"], out["ins_weights"] = S.numpy(), Vi.numpy(), Wi.numpy()
+    Vf, Wf = tfs.insert_central_slices_rfft_3d_multichannel(S, (16, 16, 16), R.clone(), fftfreq_max=0.3)
+    out["ins_volume_fmax"], out["ins_weights_fmax"] = Vf.numpy(), Wf.numpy()
+    real = torch.randn(2, 12, 12, 12, generator=g)
+    Rp = uniform_rotations(4, 43)
+    out["proj_volume"], out["proj_rotations"] = real.numpy(), Rp.numpy()
+    out["projections"] = tfs.project_3d_to_2d_multichannel(real, Rp.clone()).numpy()
+    save("multichannel_16", **out)
+
+    # G18: autograd of sample_image_3d and affine_transform_image_3d
+    out = {}
+    g = torch.Generator().manual_seed(44)
+    shape = (7, 6, 8)
+    crd = torch.rand(60, 3, generator=g) * (torch.tensor(shape, dtype=torch.float32) + 1.0) - 1.0
+    crd[:6] = torch.tensor([[0.0, 2.5, 3.25], [6.0, 1.5, 2.0], [3.5, 0.0, 7.0], [2.25, 5.0, 4.5],
+                            [1.0, 2.0, 3.0], [6.0, 5.0, 7.0]])  # on faces, edges and voxels
+    out["coords"] = crd.numpy()
+    imgs = {"real": torch.randn(*shape, generator=g),
+            "complex": torch.complex(torch.randn(*shape, generator=g), torch.randn(*shape, generator=g)),
+            "multi": torch.randn(2, *shape, generator=g)}
+    for key, img in imgs.items():
+        out[f"img_{key}"] = img.numpy()
+        for mode in ("trilinear", "nearest"):
+            im = img.clone().requires_grad_(True)
+            c = crd.clone().requires_grad_(True)
+            smp = tii.sample_image_3d(im, c, interpolation=mode)
+            tgt = torch.randn(smp.shape, generator=g, dtype=torch.float32)
+            if smp.is_complex():
+                tgt = torch.complex(tgt, torch.randn(smp.shape, generator=g))
+                loss = torch.real(torch.sum(torch.conj(tgt) * smp))
+            else:
+                loss = torch.sum(tgt * smp)
+            loss.backward()
+            out[f"{key}_{mode}_target"] = tgt.numpy()
+            out[f"{key}_{mode}_samples"] = smp.detach().numpy()
+            out[f"{key}_{mode}_grad_image"] = im.grad.numpy()
+            out[f"{key}_{mode}_grad_coords"] = c.grad.numpy()
+    img = torch.randn(2, 9, 8, 7, generator=g)
+    M = affine_matrices_3d(3, 45, (9, 8, 7)).reshape(3, 1, 1, 1, 4, 4)
+    im = img.clone().requires_grad_(True)
+    Mg = M.clone().requires_grad_(True)
+    warped = tti.affine_transform_image_3d(im, Mg, "trilinear")
+    tgt = torch.randn(warped.shape, generator=g)
+    torch.sum(tgt * warped).backward()
+    out.update(affine_image=img.numpy(), affine_matrices=M.numpy(), affine_target=tgt.numpy(),
+               affine_warped=warped.detach().numpy(), affine_grad_image=im.grad.numpy(),
+               affine_grad_matrices=Mg.grad.numpy())
+    save("sample_autograd", **out)
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     root = args[0] if args else "/root/reference"
@@ -456,7 +537,10 @@ if __name__ == "__main__":
         main_2d(root)
     elif "--transform" in sys.argv:  # only the affine-warp fixtures
         main_transform(root)
+    elif "--mc-grad" in sys.argv:  # only the multichannel and sampler-autograd fixtures
+        main_mc_grad(root)
     else:
         main(root)
         main_2d(root)
         main_transform(root)
+        main_mc_grad(root)

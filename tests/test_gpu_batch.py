@@ -1,7 +1,10 @@
 """Batched extraction and insertion over volumes of uneven box size and
 rotation count: one persistent work-queue launch per tile configuration,
 captured in a graph, against the one-call entry points (the reference
-semantics are those of a loop of one-call extract/insert calls)."""
+semantics are those of a loop of one-call extract/insert calls,
+slice_extraction/_extract_central_slices_rfft_3d.py:84 and
+slice_insertion/_insert_central_slices_rfft_3d.py:9) and, for one item of
+every box size, against the FP64 oracle."""
 
 import math
 import time
@@ -12,6 +15,8 @@ import torch
 
 import teamtomo_amd as tt
 from conftest import rel_l2
+from oracle import c_oracle as C
+from oracle import fourier_slice as O
 from teamtomo_amd.synthetic import blob_volume, uniform_rotations, volume_rfft
 
 pytestmark = pytest.mark.gpu
@@ -80,7 +85,12 @@ def test_batched_extract_and_insert_match_one_call_and_replay(device):
     slices = [torch.complex(torch.randn(o.shape, generator=g, device=device),
                             torch.randn(o.shape, generator=g, device=device)).contiguous() for o in ref]
     shapes = [(v.shape[0],) * 3 for v in vols]
-    refi = [tt.insert_central_slices_rfft_3d(s, sh, R) for s, sh, R in zip(slices, shapes, rots)]
+    # the batch accumulates in fixed point: the one-call reference in the same mode
+    tt.set_deterministic(True)
+    try:
+        refi = [tt.insert_central_slices_rfft_3d(s, sh, R) for s, sh, R in zip(slices, shapes, rots)]
+    finally:
+        tt.set_deterministic(False)
     bi = tt.SliceBatch("insert", slices, rots, shapes=shapes, autotune=True)
     outi = [(v.clone(), w.clone()) for v, w in bi.run()]
     tiles_i = {int(s): bi.tile_shapes[k] for k, s in enumerate(sizes)}
@@ -94,6 +104,20 @@ def test_batched_extract_and_insert_match_one_call_and_replay(device):
         worst = max(worst, e)
     print(f"insert: worst rel_l2 batched vs one-call {worst:.2e}")
     assert worst < 1e-5
+
+    # ---- the FP64 oracle, for the item with the fewest rotations of every box size
+    for n in sorted(set(int(x) for x in sizes)):
+        k = min((k for k in range(len(vols)) if int(sizes[k]) == n), key=lambda k: int(counts[k]))
+        Rz = O.prepare_rotations(rots[k].cpu().numpy(), False, 1e-8)
+        Vn = vols[k].cpu().numpy()
+        e_x = rel_l2(out[k].cpu().numpy(), C.extract(Vn, (n, n, n), Rz))
+        Ri = O.prepare_rotations(rots[k].cpu().numpy(), False, None)
+        rV, rW = C.insert(slices[k].cpu().numpy(), (n, n, n), Ri)
+        e_v = rel_l2(outi[k][0].cpu().numpy(), rV)
+        e_w = rel_l2(outi[k][1].cpu().numpy(), rW)
+        print(f"box {n}, item {k} ({int(counts[k])} rotations, tiles {bx.tile_shapes[k]} / {bi.tile_shapes[k]}): "
+              f"vs FP64 oracle extract {e_x:.2e} insert {e_v:.2e} weights {e_w:.2e}", flush=True)
+        assert e_x < 1e-5 and e_v < 1e-4 and e_w < 1e-4
 
     # ---- graph capture: every replay equals the eager result
     ge, gi = _graph(bx.run), _graph(bi.run)

@@ -165,6 +165,19 @@ def test_projections_of_empty_batches(device):
     rec = tt.backproject_2d_to_3d(torch.zeros(0, 16, 16, device=device), none)
     assert rec.shape == (16, 16, 16) and not bool(rec.abs().amax())
     assert tt.project_2d_to_1d(torch.rand(16, 16, device=device), torch.zeros(0, 2, 2, device=device)).shape == (0, 16)
+    # the empty result has the shape a non-empty batch would have, odd padded boxes included
+    # (irfftn without s gives 2 (Wr - 1) = N - 1 for an odd N)
+    one3, one2 = torch.eye(3, device=device)[None], torch.eye(2, device=device)[None]
+    for d in (16, 33):
+        for pad in (1.0, 2.0):
+            v = torch.rand(d, d, d, device=device)
+            full = tt.project_3d_to_2d(v, one3, pad_factor=pad).shape[1:]
+            assert tt.project_3d_to_2d(v, none, pad_factor=pad).shape == (0,) + full, (d, pad, full)
+            mc = torch.rand(2, d, d, d, device=device)
+            assert tt.project_3d_to_2d_multichannel(mc, none, pad_factor=pad).shape == (0, 2) + full
+            im = torch.rand(d, d, device=device)
+            full1 = tt.project_2d_to_1d(im, one2, pad_factor=pad).shape[1:]
+            assert tt.project_2d_to_1d(im, torch.zeros(0, 2, 2, device=device), pad_factor=pad).shape == (0,) + full1
 
 
 # ------------------------------------------------------------------ insertion

@@ -46,14 +46,13 @@ def ctf_rfft(n, pixel_size, defocus_u, defocus_v, astig_angle, kv=300.0, cs_mm=2
 
 
 def test_ctf_loss_gradients_128cubed_512poses_vs_finite_differences(device):
-    """128^3 blob volume, pad 2 (256^3 inside), 64 poses (a reduced count of
-    the 512 so that the FP64 finite differences stay affordable), CTF at
+    """128^3 blob volume, pad 2 (256^3 inside), 512 poses, CTF at
     300 kV / Cs 2.7 mm / 0.07 / 1.5 A/px, defocus U(1,3) um, astigmatism
     U(0,0.1) um; loss = sum (ctf * proj - target)^2 with target = true
     projections + noise at SNR 0.1.  Gradients w.r.t. 8 poses' matrices (all
     9 entries) and w.r.t. the volume along 2 random directions against FP64
     central differences."""
-    n, n_pose = 128, 64
+    n, n_pose = 128, 512
     rng = np.random.default_rng(30)
     vol_true = blob_volume(n, 60, seed=31, device=device)
     vol = vol_true + 0.05 * torch.from_numpy(rng.standard_normal((n, n, n)).astype(np.float32)).to(device)
@@ -99,18 +98,37 @@ def test_ctf_loss_gradients_128cubed_512poses_vs_finite_differences(device):
     dft0 = dft_of(vol64)
     Rz = O.prepare_rotations(Rn, False, 1e-8).astype(np.float64)
     # rotation gradients: each pose's loss term depends on its own matrix only
-    poses = [0, 7, 13, 22, 31, 40, 51, 63]
-    # a small step: trilinear interpolation has kinks at cell faces, and
-    # coordinates up to 128 voxels move by 128 h
-    h = 1e-8
+    poses = [0, 57, 130, 222, 301, 399, 451, 511]
+    # Trilinear interpolation has kinks where a sample crosses a cell face (or
+    # the volume's bounds, or the Friedel half changes): a central difference
+    # across one is wrong.  The step starts small (coordinates up to 128
+    # voxels move by 128 h) and is cut until no pixel of the pose changes its
+    # cell, half or bounds between R - h E and R + h E, so the loss is a
+    # polynomial of the entry over the whole stencil.
+    def cells(Rz1):
+        fr = O._rotated_frequencies(Rz1, 2 * n, 2 * n)
+        conj = O._extract_conjugate_mask(fr, Rz1, 1e-8)
+        c = O._to_array_coords(np.where(conj[..., None], -fr, fr), (2 * n,) * 3)
+        ok = O._inside(c, (2 * n, 2 * n, n + 1))
+        return np.floor(c), conj, ok
+
     fd_R = np.zeros((len(poses), 3, 3))
+    steps = []
     for a, k in enumerate(poses):
         for i in range(3):
             for j in range(3):
-                Rp, Rm = Rz[k:k + 1].copy(), Rz[k:k + 1].copy()
-                Rp[0, i, j] += h
-                Rm[0, i, j] -= h
+                h = 1e-8
+                while True:
+                    Rp, Rm = Rz[k:k + 1].copy(), Rz[k:k + 1].copy()
+                    Rp[0, i, j] += h
+                    Rm[0, i, j] -= h
+                    cp, cm = cells(Rp), cells(Rm)
+                    if all(np.array_equal(u, v) for u, v in zip(cp, cm)) or h < 2e-10:
+                        break
+                    h /= 4
+                steps.append(h)
                 fd_R[a, i, j] = (loss64(dft0, Rp, [k], True) - loss64(dft0, Rm, [k], True)) / (2 * h)
+    print(f"  finite-difference steps: {sorted(set(steps))}")
     fd_R = O.rotation_grad_to_xyz(fd_R, Rn[poses], False, 1e-8)
     g_R = Rg.grad.detach().cpu().numpy()[poses]
     err_R = rel_l2(g_R, fd_R)
