@@ -98,6 +98,26 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr,
                    void *slices, void *stream);
 
 /*
+ * tt_extract_c64 without allocation: the same brick gather (the same slices,
+ * bit for bit) for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  It has no counterpart in the
+ * reference.  Its plane records and unit table live in a fixed file-scope
+ * device array per GPU (about 2.8 MB, held until the process ends) instead of
+ * the allocations tt_extract_c64 makes; the first call for a geometry plans
+ * its units on the host (host memory only) and copies them in, stream-ordered.
+ *   n_rot <= 16384, and the volume's brick units <= 65536 (256^3 needs
+ *   about 9000): larger calls are refused with TT_ERR_INVALID.
+ *   Every call on one GPU shares the array: calls that overlap on one GPU
+ *   (different streams, threads) race; calls on one stream are ordered.
+ * Geometries beyond the brick engine's limits are refused (no pixel path).
+ */
+int tt_extract_c64_noalloc(const void *volume, int vol_d, int vol_h, int vol_wr,
+                           int img_d, int img_h, int img_w,
+                           const float *rot_zyx, int64_t n_rot,
+                           double fftfreq_max, double x_tolerance,
+                           void *slices, void *stream);
+
+/*
  * Multichannel extraction: replaces extract_central_slices_rfft_3d_multichannel
  * (torch-fourier-slice/.../slice_extraction/_extract_central_slices_rfft_3d.py:225-367).
  *   volume  (channels, vol_d, vol_h, vol_wr) complex64

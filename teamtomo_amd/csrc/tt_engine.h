@@ -16,6 +16,12 @@ bool engine_supports(const SliceGeom &g, int64_t n_rot);
 int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
                    hipStream_t s);
 
+// As engine_extract, with no allocation: the plane records and unit table
+// live in a fixed file-scope device array (at most 16384 rotations and 65536
+// brick units per call; calls that overlap on one GPU race).
+int engine_extract_noalloc(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                           hipStream_t s);
+
 // Two channels of a multichannel volume (channel stride g.vstride) into a
 // multichannel slice stack (rotation stride g.ostride, channel stride g.P),
 // sharing one plan, plane-record pass and brick launch.

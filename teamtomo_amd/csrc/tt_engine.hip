@@ -1465,7 +1465,33 @@ constexpr int kTargetSlots = 256 * 2 * 2;
 // half-diagonal rb at distance d with probability ~ min(1, rb / d) and covers
 // ~ (BZ BY BX)^(2/3) of its pixels.  Bricks no slice sample can reach are
 // dropped (gather) or kept as empty units that write zeros (scatter).
-int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n_rot, PlanEntry &pe) {
+// Device copies of a plan's unit and split tables (hipMalloc, kept with the
+// cached plan).
+int upload_plan(PlanEntry &e) {
+  hipError_t err = hipSuccess;
+  if (e.n_units > 0 && !e.units) {
+    err = hipMalloc((void **)&e.units, e.h_units->size() * sizeof(Unit));
+    if (err == hipSuccess)
+      err = hipMemcpy(e.units, e.h_units->data(), e.h_units->size() * sizeof(Unit), hipMemcpyHostToDevice);
+  }
+  if (err == hipSuccess && e.n_splits > 0 && !e.splits) {
+    err = hipMalloc((void **)&e.splits, e.h_splits->size() * sizeof(SplitRec));
+    if (err == hipSuccess)
+      err = hipMemcpy(e.splits, e.h_splits->data(), e.h_splits->size() * sizeof(SplitRec), hipMemcpyHostToDevice);
+  }
+  if (err == hipSuccess) err = hipGetLastError();
+  if (err != hipSuccess) {
+    set_error(std::string("brick plan upload (") + std::to_string(e.n_units) + " units, " +
+              std::to_string(e.n_splits) + " splits): " + hipGetErrorString(err));
+    return TT_ERR_HIP;
+  }
+  return TT_OK;
+}
+
+// device_tables = false: the host tables only (no device allocation; the
+// caller copies them where it needs them).
+int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n_rot, PlanEntry &pe,
+             bool device_tables = true) {
   int dev = 0;
   if (hipError_t e0 = hipGetDevice(&dev)) {
     set_error(std::string("brick plan: hipGetDevice: ") + hipGetErrorString(e0));
@@ -1481,6 +1507,8 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
   std::lock_guard<std::mutex> lock(g_plan_mutex);
   auto it = g_plans.find(key);
   if (it != g_plans.end()) {
+    if (device_tables)
+      if (int st = upload_plan(it->second)) return st;
     pe = it->second;
     if (hipError_t e1 = hipGetLastError()) {
       set_error(std::string("brick plan (cached): ") + hipGetErrorString(e1));
@@ -1566,23 +1594,8 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
   e.h_est = std::make_shared<std::vector<double>>(us.size());
   for (size_t k = 0; k < us.size(); ++k) (*e.h_est)[k] = us[k].est;
   e.h_splits = std::make_shared<std::vector<SplitRec>>(splits);
-  hipError_t err = hipSuccess;
-  if (e.n_units > 0) {
-    err = hipMalloc((void **)&e.units, host.size() * sizeof(Unit));
-    if (err == hipSuccess)
-      err = hipMemcpy(e.units, host.data(), host.size() * sizeof(Unit), hipMemcpyHostToDevice);
-  }
-  if (err == hipSuccess && e.n_splits > 0) {
-    err = hipMalloc((void **)&e.splits, splits.size() * sizeof(SplitRec));
-    if (err == hipSuccess)
-      err = hipMemcpy(e.splits, splits.data(), splits.size() * sizeof(SplitRec), hipMemcpyHostToDevice);
-  }
-  if (err == hipSuccess) err = hipGetLastError();
-  if (err != hipSuccess) {
-    set_error(std::string("brick plan upload (") + std::to_string(host.size()) + " units, " +
-              std::to_string(splits.size()) + " splits): " + hipGetErrorString(err));
-    return TT_ERR_HIP;
-  }
+  if (device_tables)
+    if (int st = upload_plan(e)) return st;
   g_plans[key] = e;
   pe = e;
   return TT_OK;
@@ -2208,23 +2221,12 @@ bool engine_supports(const SliceGeom &g, int64_t n_rot) {
   return n_rot < ((int64_t)1 << 30) && g.H < 32768 && g.Wr < 65536;
 }
 
-int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
-                   hipStream_t s) {
-  if (int st = check_engine(g, n_rot)) return st;
-  if (n_rot == 0) return TT_OK;
-  const int64_t n_rows = n_rot * g.H;
-  const int64_t nblk = (n_rows + 255) / 256;
-  if (nblk >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
-  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out, 1);
-  if (int st = check_launch("gather_zero_kernel")) return st;
-  PlanEntry pe;
-  if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
-  if (pe.n_units == 0) return TT_OK;
-  Scratch sc(s);
-  const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
-  if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
-  PlaneRec *pre = (PlaneRec *)sc.p;
-  PlaneTest *ptest = (PlaneTest *)((char *)sc.p + off_test);
+namespace {
+
+// The gather's launches after the zero pass: plane records into pre/ptest,
+// then the brick kernel over the plan's units.
+int gather_launch(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                  PlaneRec *pre, PlaneTest *ptest, const Unit *units, int n_units, hipStream_t s) {
   hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
                      (int)n_rot, pre, ptest, (unsigned *)nullptr);
   if (int st = check_launch("plane_rec_kernel")) return st;
@@ -2232,16 +2234,108 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
   ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
   constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
   if (g.use_mask)
-    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)pe.n_units), dim3(T),
-                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot, out,
+    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)n_units), dim3(T),
+                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, units, (int)n_rot, out,
                        (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
                        (float *)nullptr, sa, 0, (int *)nullptr);
   else
-    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)pe.n_units),
-                       dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
+    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)n_units),
+                       dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, units,
                        (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
                        (u64 *)nullptr, (float *)nullptr, sa, 0, (int *)nullptr);
   return check_launch("brick_kernel");
+}
+
+int gather_zero(const SliceGeom &g, const float *rot, int64_t n_rot, f2v *out, hipStream_t s) {
+  const int64_t n_rows = n_rot * g.H;
+  const int64_t nblk = (n_rows + 255) / 256;
+  if (nblk >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
+  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out, 1);
+  return check_launch("gather_zero_kernel");
+}
+
+// Scratch of the allocation-free extraction (engine_extract_noalloc): the
+// unit table and the per-rotation plane records, in one file-scope device
+// array per GPU (kept until the process ends).  Host code addresses it only
+// through the pointer hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocMaxRot = 16384;
+constexpr int kNoAllocMaxUnits = 65536;
+constexpr size_t kNoAllocPreOff = ((size_t)kNoAllocMaxUnits * sizeof(Unit) + 255) & ~(size_t)255;
+constexpr size_t kNoAllocTestOff =
+    (kNoAllocPreOff + (size_t)kNoAllocMaxRot * sizeof(PlaneRec) + 255) & ~(size_t)255;
+constexpr size_t kNoAllocBytes = kNoAllocTestOff + (size_t)kNoAllocMaxRot * sizeof(PlaneTest);
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+  char *ptr = nullptr;                      // device address of noalloc_scratch on this GPU
+  const std::vector<Unit> *units = nullptr;  // the plan whose units the scratch holds
+};
+std::mutex g_noalloc_mutex;
+NoAllocDevice g_noalloc[kMaxDevices];
+
+}  // namespace
+
+int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                   hipStream_t s) {
+  if (int st = check_engine(g, n_rot)) return st;
+  if (n_rot == 0) return TT_OK;
+  if (int st = gather_zero(g, rot, n_rot, out, s)) return st;
+  PlanEntry pe;
+  if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
+  if (pe.n_units == 0) return TT_OK;
+  Scratch sc(s);
+  const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
+  if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
+  return gather_launch(g, vol, rot, n_rot, out, (PlaneRec *)sc.p, (PlaneTest *)((char *)sc.p + off_test),
+                       pe.units, pe.n_units, s);
+}
+
+int engine_extract_noalloc(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
+                           hipStream_t s) {
+  if (int st = check_engine(g, n_rot)) return st;
+  if (n_rot > kNoAllocMaxRot)
+    return fail_invalid("tt_extract_c64_noalloc: at most " + std::to_string(kNoAllocMaxRot) +
+                        " rotations per call (its fixed scratch), got " + std::to_string(n_rot));
+  if (n_rot == 0) return TT_OK;
+  int dev = 0;
+  if (hipError_t e = hipGetDevice(&dev)) {
+    set_error(std::string("tt_extract_c64_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+    return TT_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return fail_invalid("tt_extract_c64_noalloc: device index out of range");
+  PlanEntry pe;
+  if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe, false)) return st;
+  if (pe.n_units > kNoAllocMaxUnits)
+    return fail_invalid("tt_extract_c64_noalloc: the volume needs " + std::to_string(pe.n_units) +
+                        " brick units, more than its fixed scratch holds (" + std::to_string(kNoAllocMaxUnits) + ")");
+  char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    NoAllocDevice &nd = g_noalloc[dev];
+    if (!nd.ptr) {  // once per device, with that device current
+      void *p = nullptr;
+      if (hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch))) {
+        set_error(std::string("tt_extract_c64_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        return TT_ERR_HIP;
+      }
+      nd.ptr = (char *)p;
+    }
+    base = nd.ptr;
+    if (pe.n_units > 0 && nd.units != pe.h_units.get()) {
+      // stream-ordered: earlier calls on this stream finish with the old units first
+      if (hipError_t e = hipMemcpyAsync(base, pe.h_units->data(), pe.h_units->size() * sizeof(Unit),
+                                        hipMemcpyHostToDevice, s)) {
+        set_error(std::string("tt_extract_c64_noalloc: unit table copy: ") + hipGetErrorString(e));
+        return TT_ERR_HIP;
+      }
+      nd.units = pe.h_units.get();
+    }
+  }
+  if (int st = gather_zero(g, rot, n_rot, out, s)) return st;
+  if (pe.n_units == 0) return TT_OK;
+  return gather_launch(g, vol, rot, n_rot, out, (PlaneRec *)(base + kNoAllocPreOff),
+                       (PlaneTest *)(base + kNoAllocTestOff), (const Unit *)base, pe.n_units, s);
 }
 
 // Two channels of a multichannel volume (channel stride g.vstride) into a

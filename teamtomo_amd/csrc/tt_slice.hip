@@ -384,6 +384,19 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img
   return TT_OK;
 }
 
+// tt_extract_c64 without any allocation: the brick gather with its plane
+// records and unit table in a fixed device array (see the header).
+int tt_extract_c64_noalloc(const void *volume, int vol_d, int vol_h, int vol_wr, int img_d, int img_h,
+                           int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,
+                           double x_tolerance, void *slices, void *stream) {
+  int st = check_common(vol_d, vol_h, vol_wr, img_d, img_h, img_w, n_rot, rot_zyx);
+  if (st) return st;
+  if (n_rot == 0) return TT_OK;
+  if (!volume || !slices) return fail_invalid("volume / slices pointer is NULL");
+  SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, x_tolerance);
+  return engine_extract_noalloc(g, (const f2v *)volume, rot_zyx, n_rot, (f2v *)slices, (hipStream_t)stream);
+}
+
 int tt_extract_mc_c64(const void *volume, int channels, int vol_d, int vol_h, int vol_wr, int img_d,
                       int img_h, int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,
                       double x_tolerance, void *slices, void *stream) {
