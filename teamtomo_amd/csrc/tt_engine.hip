@@ -291,6 +291,10 @@ struct ScaleArgs {
   // plane records (slice_count_bits)
   int rotbits;
   const unsigned *slicebits;
+  // per brick, 1 if some slice plane can reach it, 0 if none does (then its
+  // units return at once and its voxels are written by uncrossed_fill_kernel),
+  // or null: every brick is processed
+  const int *crossed;
 };
 
 // The block maxima are combined with atomicMax into kMaxSlots words kSlotStride
@@ -328,22 +332,28 @@ __device__ __forceinline__ void box_radii(const SliceGeom &g, const float lo[3],
 // side), or from the exact maximum of the input (always wide).
 __device__ __forceinline__ void scale_of(const ScaleArgs &sa, const SliceGeom &g, const float lo[3],
                                          const float hi[3], float &scale, float &limit, bool &wide) {
+  // one word per lane (kBands = 64 = the wave), one load in flight, then
+  // wave reductions: the scale costs one memory latency per unit
+  static_assert(kBands == 64 && kMaxSlots <= 64, "one band (or slot) per lane");
+  const int lane = (int)(threadIdx.x & 63);
   unsigned mb = 0u;
   wide = true;
   if (sa.bandmax) {
     float rmin, rmax;
     box_radii(g, lo, hi, rmin, rmax);
     const int k0 = max(band_of(rmin * 0.9999f) - 1, 0), k1 = min(band_of(rmax * 1.0001f) + 1, kBands - 1);
-    unsigned mn = 0xffffffffu;
-    for (int k = k0; k <= k1; ++k) {
-      const unsigned b = sa.bandmax[k];
-      mb = max(mb, b);
-      if (b) mn = min(mn, b);
+    const unsigned b = sa.bandmax[lane];
+    const bool in = lane >= k0 && lane <= k1;
+    mb = in ? b : 0u;
+    unsigned mn = (in && b) ? b : 0xffffffffu;
+    for (int off = 32; off > 0; off >>= 1) {
+      mb = max(mb, (unsigned)__shfl_xor((int)mb, off, 64));
+      mn = min(mn, (unsigned)__shfl_xor((int)mn, off, 64));
     }
     wide = mb != 0u && __uint_as_float(mb) > kFlatRatio * __uint_as_float(mn);
   } else {
-#pragma unroll
-    for (int k = 0; k < kMaxSlots; ++k) mb = max(mb, sa.maxbits[k * kSlotStride]);
+    mb = lane < kMaxSlots ? sa.maxbits[lane * kSlotStride] : 0u;
+    for (int off = 32; off > 0; off >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, off, 64));
   }
   const float m = __uint_as_float(mb);
   int e = 0;
@@ -539,6 +549,7 @@ __device__ __forceinline__ void brick_body(
 
   if (SCATTER && sa.guard && *sa.flag == 0u) return;  // exact redo not needed
   const int brick = un.brick;
+  if (SCATTER && sa.crossed && !sa.crossed[brick]) return;  // no plane reaches it
   const int seg = un.seg >> 16, nseg_raw = un.seg & 0xffff;
   const int nseg = max(nseg_raw, 1);  // an empty brick (no slice reaches it) only writes zeros
   const int bz = brick / (bg.nby * bg.nbx);
@@ -1035,9 +1046,10 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
     float *__restrict__ wplane, u64 *__restrict__ segbuf, float *__restrict__ halo, ScaleArgs sa,
     int accumulate, int *__restrict__ touched) {
+  // one unit per block (a stride loop over units measured 4-25 % slower)
   const Unit un = units[blockIdx.x];
   brick_body<BZ, BY, BX, MODE, MASK, NCH>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
-                                     halo, sa, accumulate, touched);
+                                          halo, sa, accumulate, touched);
 }
 
 // Sums the rotation segments of a split brick (integer sums: the result does
@@ -1052,6 +1064,17 @@ __device__ __forceinline__ void split_body(
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NT = C::NT, NA = C::NACC;
   if (sa.guard && *sa.flag == 0u) return;
+  if (sa.crossed && !sa.crossed[sp.brick]) return;
+  // the scale first, with every lane of the wave (scale_of reduces over it)
+  const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, sp.brick);
+  float vscale, vlimit;
+  {
+    const float blo[3] = {(float)bb.lo[0], (float)bb.lo[1], (float)bb.lo[2]};
+    const float bhi[3] = {(float)(bb.lo[0] + bb.ext[0] + 1), (float)(bb.lo[1] + bb.ext[1] + 1),
+                          (float)(bb.lo[2] + bb.ext[2] + 1)};
+    bool wide;
+    scale_of(sa, g, blo, bhi, vscale, vlimit, wide);
+  }
   if (v >= NT) return;
   if (touched && v == 0) touched[sp.brick] = 1;  // split bricks are the busiest ones
   u64 acc[NA];
@@ -1061,15 +1084,6 @@ __device__ __forceinline__ void split_body(
     const u64 *b = segbuf + ((int64_t)sp.slot + sgi) * (NA * NT);
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] += b[a * NT + v];
-  }
-  const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, sp.brick);
-  float vscale, vlimit;
-  {
-    const float blo[3] = {(float)bb.lo[0], (float)bb.lo[1], (float)bb.lo[2]};
-    const float bhi[3] = {(float)(bb.lo[0] + bb.ext[0] + 1), (float)(bb.lo[1] + bb.ext[1] + 1),
-                          (float)(bb.lo[2] + bb.ext[2] + 1)};
-    bool wide;
-    scale_of(sa, g, blo, bhi, vscale, vlimit, wide);
   }
   flush_voxel<BZ, BY, BX, MODE>(acc, 1, v, bb, g, sp.brick, 1.0f / vscale, accumulate, out, wout, plane,
                                 wplane, halo);
@@ -1181,8 +1195,10 @@ __global__ __launch_bounds__(256) void halo_combine_kernel(SliceGeom g, BrickGri
                                                            f2v *__restrict__ plane,
                                                            float *__restrict__ wplane,
                                                            const int *__restrict__ touched) {
-  halo_body<BZ, BY, BX, MODE>(g, bg, halo, sa, out, wout, plane, wplane, (int)blockIdx.x,
-                              (int)(blockIdx.y * 256 + threadIdx.x), touched);
+  const int n_bricks = bg.nbz * bg.nby * bg.nbx;
+  for (int b = blockIdx.x; b < n_bricks; b += gridDim.x)
+    halo_body<BZ, BY, BX, MODE>(g, bg, halo, sa, out, wout, plane, wplane, b,
+                                (int)(blockIdx.y * 256 + threadIdx.x), touched);
 }
 
 // max(|re|, |im|) as float bits into *res, over chunks of 8192 consecutive
@@ -1226,8 +1242,10 @@ __global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restric
 }
 
 // Sampled per-band max(|re|, |im|) of a slice stack: every rot_stride-th
-// slice, each pixel's value into the band of its radius |f| (block maxima in
-// LDS, then one atomic per band and block).
+// slice, each pixel's value into the band of its radius |f|.  A thread takes
+// pixels (consecutive threads, consecutive pixels) and walks the sampled
+// slices for each, so a pixel's band is computed once; block maxima in LDS,
+// then one atomic per band and block.
 __device__ __forceinline__ void band_max_body(const f2v *__restrict__ x, const SliceGeom &g, int64_t n_rot,
                                               int64_t rot_stride, unsigned *__restrict__ bands, int block,
                                               int nblocks) {
@@ -1235,15 +1253,26 @@ __device__ __forceinline__ void band_max_body(const f2v *__restrict__ x, const S
   for (int k = threadIdx.x; k < kBands; k += blockDim.x) s_b[k] = 0u;
   __syncthreads();
   const int64_t nsl = (n_rot + rot_stride - 1) / rot_stride;
-  const int64_t total = nsl * g.P;
+  const int64_t step = rot_stride * g.ostride;
   const float ih = (float)g.inv_H, iw = (float)g.inv_W;
-  for (int64_t t = (int64_t)block * blockDim.x + threadIdx.x; t < total; t += (int64_t)nblocks * blockDim.x) {
-    const int64_t sl = t / g.P;
-    const int p = (int)(t - sl * g.P);
-    const int i = p / g.Wr, j = p - i * g.Wr;
-    const f2v v = x[sl * rot_stride * g.ostride + p];
+  for (int p = block * (int)blockDim.x + (int)threadIdx.x; p < g.P; p += nblocks * (int)blockDim.x) {
+    int i, j;
+    split_pixel(g, p, i, j);
     const float fy = (float)(i - g.half_H) * ih, fx = (float)j * iw;
-    atomicMax(&s_b[band_of(sqrtf(fy * fy + fx * fx))], __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
+    float m = 0.0f;
+    int64_t sl = 0;
+    for (; sl + 4 <= nsl; sl += 4) {  // four loads in flight
+      f2v v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = x[(sl + u) * step + p];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) m = fmaxf(m, fmaxf(fabsf(v[u].x), fabsf(v[u].y)));
+    }
+    for (; sl < nsl; ++sl) {
+      const f2v v = x[sl * step + p];
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    atomicMax(&s_b[band_of(sqrtf(fy * fy + fx * fx))], __float_as_uint(m));
   }
   __syncthreads();
   for (int k = threadIdx.x; k < kBands; k += blockDim.x)
@@ -1255,11 +1284,77 @@ __global__ __launch_bounds__(256) void band_max_kernel(const f2v *__restrict__ x
   band_max_body(x, g, n_rot, rot_stride, bands, (int)blockIdx.x, (int)gridDim.x);
 }
 
+// Blocks of a band-maximum pass: pixels of one slice over 256 threads.
+inline int band_blocks(const SliceGeom &g) { return std::max(1, std::min((g.P + 255) / 256, 256)); }
+
 // Slices sampled for the band maxima: about 64 spread over the stack.
 __host__ __device__ inline int64_t band_stride(int64_t n_rot) { return n_rot > 64 ? n_rot / 64 : 1; }
 
 __global__ void set_word_kernel(unsigned *p, unsigned v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// Bricks some slice plane can reach, for calls with few rotations (a tilt
+// series leaves whole wedges of the volume untouched): the brick kernel's
+// plane test with twice its margin, so that every brick whose units would
+// list a crossing rotation is marked.  Uncrossed bricks get crossed = 0 and
+// touched = 0 (no halo shell).
+template <int BZ, int BY, int BX>
+__global__ __launch_bounds__(256) void brick_cross_kernel(SliceGeom g, BrickGrid bg,
+                                                          const PlaneTest *__restrict__ ptest, int n_rot,
+                                                          int *__restrict__ crossed, int *__restrict__ touched) {
+  const int brick = blockIdx.x * 256 + threadIdx.x;
+  if (brick >= bg.nbz * bg.nby * bg.nbx) return;
+  const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
+  const double beta[3] = {g.bz, g.by, 0.0};
+  float cf[3], hf[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double L = (double)bb.lo[d] - kBoxMargin, U = (double)(bb.lo[d] + bb.ext[d] + 1) + kBoxMargin;
+    cf[d] = (float)(0.5 * (L + U) - beta[d]);
+    hf[d] = (float)(0.5 * (U - L));
+  }
+  const float tmarg = 2.0f * (1e-3f + 4e-6f * (fabsf(cf[0]) + fabsf(cf[1]) + fabsf(cf[2]) + hf[0] + hf[1] + hf[2])) +
+                      1e-3f;
+  int hit = 0;
+  for (int r = 0; r < n_rot && !hit; ++r) {
+    const float4 q = ptest[r];
+    const float dist = q.x * cf[0] + q.y * cf[1] + q.z * cf[2];
+    const float rad = fabsf(q.x) * hf[0] + fabsf(q.y) * hf[1] + fabsf(q.z) * hf[2];
+    hit = (q.w < 0.0f) | (fabsf(dist) <= rad + tmarg);
+  }
+  crossed[brick] = hit;
+  if (!hit) touched[brick] = 0;
+}
+
+// Zeros for the voxels of uncrossed bricks, written in every pass instead of
+// their units' empty tiles, before the halo pass adds the shells of crossed
+// neighbours; an insertion's x = 0 voxels go to its mirror plane.  A voxel v
+// on an axis of bricks of B cells belongs to brick min(v / B, nb - 1).
+template <int BZ, int BY, int BX>
+__global__ __launch_bounds__(256) void uncrossed_fill_kernel(SliceGeom g, BrickGrid bg,
+                                                             const int *__restrict__ crossed, ScaleArgs sa,
+                                                             f2v *__restrict__ out, float *__restrict__ wout,
+                                                             f2v *__restrict__ plane, float *__restrict__ wplane) {
+  if (sa.guard && *sa.flag == 0u) return;  // exact redo not needed
+  // one block row (z, y) at a time, its x run across the threads
+  const int nzy = g.vD * g.vH;
+  for (int zy = blockIdx.x; zy < nzy; zy += gridDim.x) {
+    const int z = zy / g.vH, y = zy - z * g.vH;
+    const int brow = (min(z / BZ, bg.nbz - 1) * bg.nby + min(y / BY, bg.nby - 1)) * bg.nbx;
+    const int64_t e0 = (int64_t)zy * g.vW;
+    for (int x = threadIdx.x; x < g.vW; x += 256) {
+      if (crossed[brow + min(x / BX, bg.nbx - 1)]) continue;
+      if (plane && x == 0) {
+        plane[zy] = f2v{0.0f, 0.0f};
+        wplane[zy] = 0.0f;
+      } else {
+        out[e0 + x] = f2v{0.0f, 0.0f};
+        if (wout) wout[e0 + x] = 0.0f;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1620,6 +1715,7 @@ struct Scratch {
 };
 
 constexpr int kGBZ = 8, kGBY = 16, kGBX = 16;  // gather brick (cells)
+constexpr int kSparseRot = 1024;  // at most this many rotations: bricks no plane reaches are skipped
 constexpr int kABZ = 16, kABY = 16, kABX = 8;  // adjoint brick (cells)
 constexpr int kIBZ = 16, kIBY = 8, kIBX = 8;   // insertion brick (cells)
 
@@ -1678,6 +1774,11 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   off = (off + 255) & ~(size_t)255;
   const size_t off_touched = off;  // per brick: reached by a slice (written by every brick)
   off += (size_t)n_bricks * sizeof(int);
+  off = (off + 255) & ~(size_t)255;
+  // few rotations (a tilt series): test every brick against the planes first
+  const bool sparse = n_rot > 0 && n_rot <= kSparseRot;
+  const size_t off_crossed = off;
+  if (sparse) off += (size_t)n_bricks * sizeof(int);
   Scratch sc(s);
   if (int st = sc.alloc(off)) return st;
   char *base = (char *)sc.p;
@@ -1689,6 +1790,7 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   u64 *segbuf = (u64 *)(base + off_seg);
   float *halo = (float *)(base + off_halo);
   int *touched = (int *)(base + off_touched);
+  int *crossed = sparse ? (int *)(base + off_crossed) : nullptr;
   if (int st = fill_zero<unsigned>(words, (int64_t)(zero_end / sizeof(unsigned)), s)) return st;
   // values the exact maximum reads: the whole (strided) stack, a bound for
   // every channel of a multichannel one
@@ -1700,10 +1802,8 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   const int mgrid = (int)std::min<int64_t>(std::max<int64_t>(n / 4096, 1), 2048);
   if (n_rot > 0) {
     if (sampled) {
-      const int64_t bs = band_stride(n_rot);
-      const int64_t nv = (n_rot + bs - 1) / bs * g.P;
-      hipLaunchKernelGGL(band_max_kernel, dim3((unsigned)std::min<int64_t>((nv + 2047) / 2048, 1024)), dim3(256), 0,
-                         s, src, g, n_rot, bs, words + kBandWord);
+      hipLaunchKernelGGL(band_max_kernel, dim3((unsigned)band_blocks(g)), dim3(256), 0, s, src, g, n_rot,
+                         band_stride(n_rot), words + kBandWord);
     } else {
       hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
                          (const unsigned *)nullptr, words + kSetWords);
@@ -1711,6 +1811,11 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
     hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g,
                        rot, (int)n_rot, pre, ptest, words + kSliceWord);
     if (int st = check_launch("band_max / maxabs_chunks / plane_rec")) return st;
+  }
+  if (sparse) {
+    hipLaunchKernelGGL((brick_cross_kernel<BZ, BY, BX>), dim3((unsigned)((n_bricks + 255) / 256)), dim3(256), 0, s,
+                       g, bg, (const PlaneTest *)ptest, (int)n_rot, crossed, touched);
+    if (int st = check_launch("brick_cross_kernel")) return st;
   }
   for (int pass = 0; pass < (sampled ? 2 : 1); ++pass) {
     ScaleArgs sa;
@@ -1721,18 +1826,26 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
     sa.guard = sampled ? pass : 0;
     sa.rotbits = ceil_log2(n_rot);
     sa.slicebits = words + kSliceWord;
+    sa.crossed = crossed;
     if (pass == 1 && n_rot > 0) {
-      hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
+      hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(std::min(mgrid, 512)), dim3(256), 0, s, src, n, (int64_t)1,
                          (const unsigned *)(words + kFlagWord), words + kSetWords);
       if (int st = check_launch("maxabs_chunks_kernel")) return st;
     }
+    if (sparse && !accumulate) {  // (accumulating: uncrossed voxels keep their values)
+      hipLaunchKernelGGL((uncrossed_fill_kernel<BZ, BY, BX>),
+                         dim3((unsigned)std::min<int64_t>((int64_t)g.vD * g.vH, pass ? 1024 : 65536)), dim3(256), 0, s, g,
+                         bg, (const int *)crossed, sa, out, wout, plane, wplane);
+      if (int st = check_launch("uncrossed_fill_kernel")) return st;
+    }
     if (pe.n_units > 0) {
+      const unsigned ugrid = (unsigned)pe.n_units;
       if (g.use_mask)
-        hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, true>), dim3((unsigned)pe.n_units), dim3(C::T),
+        hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, true>), dim3(ugrid), dim3(C::T),
                            0, s, src, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot,
                            out, wout, plane, wplane, segbuf, halo, sa, accumulate, touched);
       else
-        hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, false>), dim3((unsigned)pe.n_units), dim3(C::T),
+        hipLaunchKernelGGL((brick_kernel<BZ, BY, BX, MODE, false>), dim3(ugrid), dim3(C::T),
                            0, s, src, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units, (int)n_rot,
                            out, wout, plane, wplane, segbuf, halo, sa, accumulate, touched);
       if (int st = check_launch("brick_kernel")) return st;
@@ -1745,7 +1858,8 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
       if (int st = check_launch("split_combine_kernel")) return st;
     }
     hipLaunchKernelGGL((halo_combine_kernel<BZ, BY, BX, MODE>),
-                       dim3((unsigned)n_bricks, (unsigned)((C::NSH + 255) / 256)), dim3(256), 0, s,
+                       dim3((unsigned)(sa.guard ? std::min<int64_t>(n_bricks, 256) : n_bricks),
+                            (unsigned)((C::NSH + 255) / 256)), dim3(256), 0, s,
                        g, bg, (const float *)halo, sa, out, wout, plane, wplane, (const int *)touched);
     if (int st = check_launch("halo_combine_kernel")) return st;
   }
@@ -1796,6 +1910,7 @@ __device__ __forceinline__ ScaleArgs batch_scale(const BatchItemDev &it, int pas
   sa.guard = pass;
   sa.rotbits = ceil_log2(it.n_rot);
   sa.slicebits = it.words + kSliceWord;
+  sa.crossed = nullptr;
   return sa;
 }
 
@@ -2086,9 +2201,7 @@ int batch_build(BatchPlan &bp) {
       for (int64_t b = 0; b < nb; ++b) zb.push_back(make_int2(i, (int)b));
     } else {
       const int64_t nv = nr * (int64_t)geo[i].P;
-      const int64_t bs = band_stride(nr);
-      const int64_t nv0 = (nr + bs - 1) / bs * geo[i].P;
-      const int s0 = (int)std::min<int64_t>((nv0 + 2047) / 2048, 1024);
+      const int s0 = band_blocks(geo[i]);
       const int s1 = (int)std::min<int64_t>(std::max<int64_t>(nv / 4096, 1), 2048);
       for (int b = 0; b < s0; ++b) m0.push_back(make_int3(i, b, s0));
       for (int b = 0; b < s1; ++b) m1.push_back(make_int3(i, b, s1));
@@ -2231,7 +2344,7 @@ int gather_launch(const SliceGeom &g, const f2v *vol, const float *rot, int64_t 
                      (int)n_rot, pre, ptest, (unsigned *)nullptr);
   if (int st = check_launch("plane_rec_kernel")) return st;
   const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
-  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
+  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr, nullptr};
   constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
   if (g.use_mask)
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)n_units), dim3(T),
@@ -2362,7 +2475,7 @@ int engine_extract2(const SliceGeom &g, const f2v *vol, const float *rot, int64_
                      (int)n_rot, pre, ptest, (unsigned *)nullptr);
   if (int st = check_launch("plane_rec_kernel")) return st;
   const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
-  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
+  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr, nullptr};
   constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
   if (g.use_mask)
     hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true, 2>), dim3((unsigned)pe.n_units), dim3(T),

@@ -82,3 +82,40 @@ def test_adjoint_precision_per_shell(device, deterministic):
     print(f"deterministic={deterministic}: adjoint per-shell rel error max {e.max():.2e} "
           f"(shells 12-31 max {e[11:].max():.2e})")
     assert e.max() <= SHELL_TOL
+
+
+@pytest.mark.parametrize("n_rot", [200, 1100])
+def test_guarded_redo_after_an_unsampled_spike(device, n_rot):
+    """A value 1e6 times the rest sits in a slice the sampled band maxima skip
+    (every n_rot // 64-th slice is sampled): the first pass sees it exceed its
+    brick's limit and the call is redone with the exact maximum.  Insertion
+    and adjoint must still match the FP64 oracle, with few rotations (the
+    brick-crossing pre-pass) and many."""
+    from oracle import c_oracle as C
+
+    n = 48
+    R = uniform_rotations(n_rot, 50, device=device)
+    g = torch.Generator(device=device).manual_seed(51)
+    shape = (n_rot, n, n // 2 + 1)
+    S = torch.complex(torch.randn(shape, generator=g, device=device), torch.randn(shape, generator=g, device=device))
+    S[1, n // 2 + 3, 5] = 1e6 + 2e5j  # slice 1 is never sampled
+    V, W = tt.insert_central_slices_rfft_3d(S, (n,) * 3, R)
+    V2, W2 = tt.insert_central_slices_rfft_3d(S, (n,) * 3, R)
+    assert torch.equal(V, V2) and torch.equal(W, W2)
+    Ri = O.prepare_rotations(R.cpu().numpy(), False, None)
+    rV, rW = C.insert(S.cpu().numpy(), (n,) * 3, Ri)
+    e_v = float(np.linalg.norm(V.cpu().numpy() - rV) / np.linalg.norm(rV))
+    far = np.abs(rV) < 1e3  # the voxels away from the spike, on their own
+    e_far = float(np.linalg.norm((V.cpu().numpy() - rV)[far]) / np.linalg.norm(rV[far]))
+    vol = volume_rfft(blob_volume(n, 20, seed=52)).to(device).requires_grad_(True)
+    tt.extract_central_slices_rfft_3d(vol, (n,) * 3, R).backward(S)
+    Rz = O.prepare_rotations(R.cpu().numpy(), False, 1e-8)
+    ref = C.extract_adjoint(S.cpu().numpy(), vol.shape, (n,) * 3, Rz)
+    e_a = float(np.linalg.norm(vol.grad.cpu().numpy() - ref) / np.linalg.norm(ref))
+    far_a = np.abs(ref) < 1e3
+    e_afar = float(np.linalg.norm((vol.grad.cpu().numpy() - ref)[far_a]) / np.linalg.norm(ref[far_a]))
+    print(f"{n_rot} rotations with a spike: insert rel_l2 {e_v:.2e} (away from it {e_far:.2e}), "
+          f"adjoint {e_a:.2e} (away {e_afar:.2e})")
+    assert e_v < 1e-5 and e_a < 1e-5
+    # the redo scale is the global maximum: 2^-30 of 1.2e6, about 1e-3, per contribution
+    assert e_far < 1e-2 and e_afar < 1e-2
