@@ -9,7 +9,12 @@ _pixel_atomic = False
 
 
 def set_deterministic(enabled: bool) -> None:
-    """Select the fixed-order (bitwise reproducible) scatter for insertion and adjoints."""
+    """Require bitwise reproducible scatters (insertion and adjoints).
+
+    The brick scatter accumulates in fixed point and is reproducible either
+    way; with this set, the geometries only the pixel-atomic kernels handle
+    (complex128 insertion, sizes beyond the brick engine) raise instead of
+    running with float atomics."""
     global _deterministic
     _deterministic = bool(enabled)
 
@@ -34,10 +39,12 @@ _GATHER_PATHS = {"auto": 0, "pixel": 1, "brick": 2}
 
 
 def set_gather_path(path: str) -> None:
-    """Choose the extraction gather kernel: "auto" (default: the library's
-    choice, currently the pixel gather), "pixel" (one lane per pixel, corners
-    read through the caches) or "brick" (volume bricks staged in LDS, serving
-    every slice crossing them).  Both give bitwise identical samples."""
+    """Choose the extraction gather kernel: "auto" (default: the brick gather,
+    or the pixel gather for geometries beyond the brick engine's limits),
+    "pixel" (one lane per pixel, corners read through the caches) or "brick"
+    (volume bricks staged in LDS, serving every slice crossing them).  The two
+    agree to within 1e-6 of max|V| (the brick gather's FP32 frame coordinates
+    against the pixel gather's FP64 ones), not bit for bit."""
     if path not in _GATHER_PATHS:
         raise ValueError(f"gather path must be one of {sorted(_GATHER_PATHS)}, got {path!r}")
     from . import _lib

@@ -5,9 +5,12 @@ calls, one per volume (each with its own box size and rotation count), run as
 one persistent work-queue launch of the brick kernel per tile configuration
 (``tt_batch_*`` in include/teamtomo_amd.h).  The batch is planned and its
 scratch allocated once; ``run()`` only launches kernels on the current stream,
-so it can be captured in a ``torch.cuda.CUDAGraph`` and replayed.  The input
-and output tensors are bound at construction: update the inputs in place and
-run (or replay) again.
+so it can be captured in a ``torch.cuda.CUDAGraph`` and replayed.  The input,
+rotation and output tensors are bound at construction: update the inputs or
+the rotations in place and run (or replay) again.  Every run re-reads the
+caller's rotation tensors (the cast, near-zero clamp and zyx flip run as the
+first kernel of the run), so inputs and rotations must be contiguous tensors
+on the batch's device, rotations float32 or float64 of shape (n, 3, 3).
 
 The tile shape (brick of cells) of each box size is chosen by timing the
 candidates on the batch's own items of that size (``autotune=True``);
@@ -47,13 +50,21 @@ class SliceBatch:
         self._keep = []
         self.outputs = []
         items = (_lib.BatchItem * len(inputs))()
+        self._rot = []  # (caller's rotations, prepared buffer) per item, re-prepared by every run
+        self._flip = int(not zyx_matrices)
+        self._rot_tol = -1.0 if (kind != "extract" or rot_tolerance is None) else float(rot_tolerance)
         for k, (x, R) in enumerate(zip(inputs, rotations)):
             _lib.require_gpu(x)
             if x.dtype != torch.complex64:
                 raise TypeError(f"SliceBatch: complex64 input required, got {x.dtype}")
-            x = x.contiguous()
-            Rp = _prepare_rotations(R, x.device, zyx_matrices, rot_tolerance if kind == "extract" else None)
-            Rp = Rp.reshape(-1, 3, 3).contiguous()
+            if not x.is_contiguous():
+                raise ValueError(f"SliceBatch: item {k}: the input must be contiguous (runs read it in place)")
+            if (not torch.is_tensor(R) or R.device != x.device or R.dtype not in (torch.float32, torch.float64)
+                    or R.ndim != 3 or tuple(R.shape[-2:]) != (3, 3) or not R.is_contiguous()):
+                raise ValueError(f"SliceBatch: item {k}: rotations must be a contiguous (n, 3, 3) float32 or "
+                                 f"float64 tensor on {x.device} (runs re-read it in place)")
+            Rp = torch.empty(R.shape, dtype=torch.float32, device=x.device)
+            self._rot.append((R, Rp))
             it = items[k]
             if kind == "extract":
                 D, H, Wr = x.shape
@@ -83,6 +94,7 @@ class SliceBatch:
             it.x_tolerance = float(x_tolerance)
             self._keep += [x, Rp]
         self.device = inputs[0].device
+        self._prepare()
         handle = ctypes.c_void_p()
         kind_id = _lib.TT_BATCH_EXTRACT if kind == "extract" else _lib.TT_BATCH_INSERT
         _lib.call("tt_batch_create", kind_id, ctypes.cast(items, ctypes.c_void_p), len(inputs), int(bool(autotune)),
@@ -94,8 +106,21 @@ class SliceBatch:
             _lib.call("tt_batch_tile", self._handle, k, *[ctypes.byref(v) for v in b])
             self.tile_shapes.append(tuple(v.value for v in b))
 
+    def _prepare(self):
+        """Cast, clamp and flip every item's rotations into its bound buffer
+        (slice_extraction/_extract_central_slices_rfft_3d.py:147-154,
+        slice_insertion/_insert_central_slices_rfft_3d.py:16,66-67)."""
+        stream = _lib.stream_of(self._keep[0])
+        for R, Rp in self._rot:
+            if R.shape[0] == 0:
+                continue
+            fn = "tt_prepare_rotations_f32" if R.dtype == torch.float32 else "tt_prepare_rotations_f64"
+            _lib.call(fn, _lib.ptr(R), R.shape[0], self._flip, self._rot_tol, _lib.ptr(Rp), stream)
+
     def run(self):
-        """Launch the batch on the current stream (no allocation: graph-capturable)."""
+        """Launch the batch on the current stream (no allocation: graph-capturable),
+        reading the current contents of the bound inputs and rotations."""
+        self._prepare()
         _lib.call("tt_batch_run", self._handle, _lib.stream_of(self._keep[0]))
         return self.outputs
 

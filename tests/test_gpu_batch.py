@@ -178,3 +178,34 @@ def test_batch_items_without_rotations(device):
         assert rel_l2(v0.cpu().numpy(), rv.cpu().numpy()) < 1e-5 and rel_l2(w0.cpu().numpy(), rw.cpu().numpy()) < 1e-5
         assert not bool(v1.abs().amax()) and not bool(w1.abs().amax())
         bi.close()
+
+
+def test_replay_reads_rotations_updated_in_place(device):
+    """The batch re-reads its inputs and rotations on every run: after the
+    rotations are overwritten in place, a graph replay gives the one-call
+    results of the new rotations (extraction and insertion)."""
+    n = 64
+    vols = [volume_rfft(blob_volume(n, 20, seed=310 + k, device=device)).contiguous() for k in range(2)]
+    rots = [uniform_rotations(40, 311, device=device), uniform_rotations(70, 312, device=device)]
+    bx = tt.SliceBatch("extract", vols, rots, autotune=False)
+    g = torch.Generator(device=device).manual_seed(313)
+    slices = [torch.complex(torch.randn((r.shape[0], n, n // 2 + 1), generator=g, device=device),
+                            torch.randn((r.shape[0], n, n // 2 + 1), generator=g, device=device)) for r in rots]
+    bi = tt.SliceBatch("insert", slices, rots, shapes=[(n, n, n)] * 2, autotune=False)
+    ge, gi = _graph(bx.run), _graph(bi.run)
+    for rep in range(2):
+        for k, r in enumerate(rots):
+            r.copy_(uniform_rotations(r.shape[0], 400 + 10 * rep + k, device=device))  # in place
+        ge.replay()
+        gi.replay()
+        torch.cuda.synchronize()
+        for k in range(2):
+            ref = tt.extract_central_slices_rfft_3d(vols[k], (n, n, n), rots[k])
+            assert float((bx.outputs[k] - ref).abs().max()) <= 1e-6 * float(ref.abs().max()), f"replay {rep} item {k}"
+            rv, rw = tt.insert_central_slices_rfft_3d(slices[k], (n, n, n), rots[k])
+            v, w = bi.outputs[k]
+            assert rel_l2(v.cpu().numpy(), rv.cpu().numpy()) < 1e-5 and rel_l2(w.cpu().numpy(), rw.cpu().numpy()) < 1e-5
+    with pytest.raises(ValueError, match="contiguous"):
+        tt.SliceBatch("extract", [vols[0]], [rots[0].transpose(1, 2)], autotune=False)
+    bx.close()
+    bi.close()
