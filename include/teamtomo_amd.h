@@ -52,6 +52,11 @@ extern "C" {
 #define TT_GATHER_PIXEL 1 /* one lane per pixel, corners read through L2/L1      */
 #define TT_GATHER_BRICK 2 /* volume bricks staged in LDS, serving all slices     */
 
+/* Scatter kernels of the adjoint and the insertion (see tt_set_scatter_path) */
+#define TT_SCATTER_AUTO 0   /* binned for few pixels per brick, else the brick engine */
+#define TT_SCATTER_BRICK 1  /* brick engine: bricks find and walk their slice rows    */
+#define TT_SCATTER_BINNED 2 /* pixels sorted into per-brick bins, one pass per brick  */
+
 /* Message for the last error on the calling thread ("" if none). */
 const char *tt_last_error(void);
 
@@ -59,12 +64,26 @@ const char *tt_last_error(void);
 int tt_abi_version(void);
 
 /*
- * Selects the kernel tt_extract_c64 uses (TT_GATHER_AUTO by default).  Both
- * give the same samples; the choice only moves the cost between L2 traffic
+ * Selects the kernel tt_extract_c64 uses (TT_GATHER_AUTO by default).  The
+ * two agree to within ~1e-6 of max|V| (FP32 brick frames against FP64 pixel
+ * coordinates), not bit for bit; the choice moves the cost between L2 traffic
  * (pixel-major) and LDS staging (brick).  Has no counterpart in the
  * reference: it is a tuning and testing switch of this library.
  */
 int tt_set_gather_path(int path);
+
+/*
+ * Selects the scatter the complex64 adjoint and insertion entry points use
+ * (TT_SCATTER_AUTO by default: the binned scatter when the call has at most
+ * 320 slice pixels per 8x16x16 brick of the volume, such as a tilt series
+ * into a large volume, and its record buffer, 8 x 24 bytes per slice pixel,
+ * stays within 3 GiB; the brick engine otherwise).  TT_SCATTER_BINNED uses the binned scatter wherever it fits.
+ * Both accumulate in fixed point and are bitwise reproducible; they agree to
+ * ~1e-6 relative (different FP32 roundings of the coordinates and scales).
+ * Process-wide; a tuning and testing switch with no counterpart in the
+ * reference.
+ */
+int tt_set_scatter_path(int path);
 
 /*
  * Rotation preparation: out (n, 3, 3) float32 = the (n, 3, 3) float32 or

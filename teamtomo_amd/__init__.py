@@ -12,7 +12,7 @@ from __future__ import annotations
 import sys
 import types
 
-from ._config import deterministic_enabled, set_deterministic, set_gather_path, set_pixel_atomic
+from ._config import deterministic_enabled, set_deterministic, set_gather_path, set_pixel_atomic, set_scatter_path
 from .fourier_slice import (
     backproject_2d_to_3d,
     backproject_2d_to_3d_multichannel,
@@ -83,6 +83,7 @@ __all__ = [
     "deterministic_enabled",
     "set_pixel_atomic",
     "set_gather_path",
+    "set_scatter_path",
     "install_dropin",
     "SliceBatch",
 ]

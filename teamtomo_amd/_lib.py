@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libteamtom
 _lock = threading.Lock()
 _lib = None
 
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 TT_INTERP_NEAREST = 0
 TT_INTERP_TRILINEAR = 1
@@ -43,6 +43,7 @@ _d = ctypes.c_double
 # name -> argtypes (all return int status)
 _SIGNATURES = {
     "tt_set_gather_path": [_i],
+    "tt_set_scatter_path": [_i],
     "tt_extract_c64": [_p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],
     "tt_extract_c64_noalloc": [_p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],
     "tt_extract_mc_c64": [_p, _i, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],

@@ -37,4 +37,19 @@ int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t 
 int engine_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64_t n_rot, f2v *vol,
                   float *w, int accumulate, hipStream_t s);
 
+// The insertion's x=0-plane Friedel mirror pass: out[z, y, 0] (+)= plane[z, y]
+// + conj(plane[-z, -y]) and likewise for the weights (DC not mirrored).
+int launch_mirror(const f2v *plane, const float *wplane, const SliceGeom &g, f2v *out, float *wout, int accumulate,
+                  hipStream_t s);
+
+// Binned scatter (tt_binned.hip) for calls with few slice pixels per volume
+// brick: pixels sorted into per-brick bins, one LDS accumulation per brick,
+// 64-bit fixed point (bitwise reproducible).  set_scatter_path takes a
+// TT_SCATTER_* value; binned_selected says whether engine_adjoint and
+// engine_insert take the binned path for this call.
+int set_scatter_path(int path);
+bool binned_selected(const SliceGeom &g, int64_t n_rot);
+int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float *rot, int64_t n_rot, f2v *out,
+                   float *wout, int accumulate, hipStream_t s);
+
 }  // namespace tt

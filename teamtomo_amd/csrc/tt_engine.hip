@@ -50,6 +50,7 @@
 #include "../../include/teamtomo_amd.h"
 #include "tt_common.h"
 #include "tt_engine.h"
+#include "tt_fixed.h"
 
 namespace tt {
 
@@ -364,30 +365,11 @@ __device__ __forceinline__ void scale_of(const ScaleArgs &sa, const SliceGeom &g
   limit = ldexpf(1.0f, e + sa.headroom);
 }
 
-constexpr float kWScale = 1073741824.0f;  // 2^30: fixed-point scale of the weights
 constexpr float kNear = 2e-5f;            // FP32 guard band around integer coordinates
-
-typedef unsigned long long u64;
-
-__device__ __forceinline__ int cvt_floor_i32(float x) {
-  int r;
-  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));  // floor and convert in one VALU op
-  return r;
-}
 
 // Adds round(w * v) (half up) as a 64-bit fixed-point value (|w v| < 2^31).
 __device__ __forceinline__ void lds_add_fixed(u64 *p, float w, float v) {
-  atomicAdd(p, (u64)(long long)cvt_floor_i32(__fmaf_rn(w, v, 0.5f)));
-}
-
-// round(p) (half to even) as a 64-bit integer, |p| < 2^62.  hi = trunc(p 2^-31)
-// and lo = p - hi 2^31 are exact in FP32: below 2^31 hi is 0 and lo = p;
-// above, p's last bit weighs at least 2^8, so lo is a multiple of 2^8 below
-// 2^31 (at most 23 significant bits).
-__device__ __forceinline__ long long to_fixed64(float p) {
-  const float hi = truncf(p * 4.656612873077393e-10f);  // 2^-31
-  const float lo = __fmaf_rn(hi, -2147483648.0f, p);
-  return (long long)((u64)(long long)(int)hi << 31) + (long long)(int)rintf(lo);
+  atomicAdd(p, (u64)(long long)to_fixed32(w, v));
 }
 
 // Adds the FP32 product w v (v already multiplied by the power-of-two scale,
@@ -1734,18 +1716,14 @@ int check_engine(const SliceGeom &g, int64_t n_rot) {
   return TT_OK;
 }
 
-// ceil(log2(n)) for n >= 1 (0 for n <= 1).
-__host__ __device__ inline int ceil_log2(int64_t n) {
-  int b = 0;
-  while (b < 62 && ((int64_t)1 << b) < n) ++b;
-  return b;
-}
 
 // Scatter driver shared by the adjoint and the insertion.
 template <int BZ, int BY, int BX, int MODE>
 int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_rot, f2v *out,
                 float *wout, int accumulate, hipStream_t s) {
   typedef Cfg<BZ, BY, BX, MODE> C;
+  // few pixels per brick (a tilt series into a large volume): the binned scatter
+  if (binned_selected(g, n_rot)) return binned_scatter(MODE == kInsert, g, src, rot, n_rot, out, wout, accumulate, s);
   PlanEntry pe;
   if (int st = get_plan(g, BZ, BY, BX, true, n_rot, pe)) return st;
   const BrickGrid bg = grid_of(g, BZ, BY, BX);
@@ -2502,6 +2480,14 @@ int engine_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64
   if (int st = check_engine(g, n_rot)) return st;
   if (n_rot == 0 && accumulate) return TT_OK;
   return run_scatter<kIBZ, kIBY, kIBX, kInsert>(g, slices, rot, n_rot, vol, w, accumulate, s);
+}
+
+int launch_mirror(const f2v *plane, const float *wplane, const SliceGeom &g, f2v *out, float *wout, int accumulate,
+                  hipStream_t s) {
+  const size_t np = (size_t)g.vD * g.vH;
+  hipLaunchKernelGGL(mirror_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, plane, wplane, g.vD, g.vH,
+                     g.vW, out, wout, accumulate);
+  return check_launch("mirror_kernel");
 }
 
 }  // namespace tt

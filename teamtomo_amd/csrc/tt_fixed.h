@@ -1,0 +1,44 @@
+// Fixed-point conversions shared by the brick scatter and the binned scatter.
+// A scatter multiplies its values by a power of two (which commutes with
+// FP32 rounding), rounds each weighted contribution to an integer and adds
+// the integers in LDS: the sums do not depend on the order of the additions,
+// so every scatter is bitwise reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tt {
+
+typedef unsigned long long u64;
+
+constexpr float kWScale = 1073741824.0f;  // 2^30: fixed-point scale of the weights
+
+__device__ __forceinline__ int cvt_floor_i32(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));  // floor and convert in one VALU op
+  return r;
+}
+
+// round(w * v) (half up) as a 32-bit integer (|w v| < 2^31).
+__device__ __forceinline__ int to_fixed32(float w, float v) { return cvt_floor_i32(__fmaf_rn(w, v, 0.5f)); }
+
+// round(p) (half to even) as a 64-bit integer, |p| < 2^62.  hi = trunc(p 2^-31)
+// and lo = p - hi 2^31 are exact in FP32: below 2^31 hi is 0 and lo = p;
+// above, p's last bit weighs at least 2^8, so lo is a multiple of 2^8 below
+// 2^31 (at most 23 significant bits).
+__device__ __forceinline__ long long to_fixed64(float p) {
+  const float hi = truncf(p * 4.656612873077393e-10f);  // 2^-31
+  const float lo = __fmaf_rn(hi, -2147483648.0f, p);
+  return (long long)((u64)(long long)(int)hi << 31) + (long long)(int)rintf(lo);
+}
+
+// ceil(log2(n)) for n >= 1 (0 for n <= 1).
+__host__ __device__ inline int ceil_log2(int64_t n) {
+  int b = 0;
+  while (b < 62 && ((int64_t)1 << b) < n) ++b;
+  return b;
+}
+
+}  // namespace tt

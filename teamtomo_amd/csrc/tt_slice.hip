@@ -347,7 +347,7 @@ extern "C" {
 
 const char *tt_last_error(void) { return g_last_error.c_str(); }
 
-int tt_abi_version(void) { return 12; }
+int tt_abi_version(void) { return 13; }
 
 static int g_gather_path = TT_GATHER_AUTO;
 
@@ -357,6 +357,8 @@ int tt_set_gather_path(int path) {
   g_gather_path = path;
   return TT_OK;
 }
+
+int tt_set_scatter_path(int path) { return set_scatter_path(path); }
 
 int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img_d, int img_h,
                    int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,

@@ -5,8 +5,10 @@ can hide a loss of precision at high frequency, where cryo-EM consumers judge
 a reconstruction (per-shell FSC).  These tests insert, and apply the
 extraction adjoint to, slices of a steep spectrum (the rfft of a 64^3 blob
 volume, whose DC is about 1e4 times its Nyquist amplitude) and compare the
-result shell by shell with the FP64 oracle on the same complex64 inputs, in
-the default (FP32 atomics) and the deterministic (fixed-point) modes.
+result shell by shell with the FP64 oracle on the same complex64 inputs, for
+both fixed-point scatters: the brick engine (per-brick scales from sampled
+band maxima, 32- or 64-bit contributions) and the binned scatter (per-brick
+scales from the exact maxima of its records, 64-bit contributions).
 """
 
 import numpy as np
@@ -48,38 +50,38 @@ def _steep_slices(device):
     return vol, R, S
 
 
-@pytest.mark.parametrize("deterministic", [False, True])
-def test_insertion_precision_per_shell(device, deterministic):
+@pytest.mark.parametrize("path", ["brick", "binned"])
+def test_insertion_precision_per_shell(device, path):
     vol, R, S = _steep_slices(device)
     dc, top = np.abs(S).max(), np.abs(S[:, 0, -1]).max()
     print(f"slice spectrum: max |S| {dc:.3e}, largest Nyquist-column |S| {top:.3e} (ratio {dc / top:.1e})")
-    tt.set_deterministic(deterministic)
+    tt.set_scatter_path(path)
     try:
         V, W = tt.insert_central_slices_rfft_3d(torch.from_numpy(S).to(device), (N,) * 3, R.to(device))
     finally:
-        tt.set_deterministic(False)
+        tt.set_scatter_path("auto")
     rV, rW = O.insert_central_slices(S.astype(np.complex128), (N,) * 3, O.prepare_rotations(R.numpy(), False, None))
     sh = _shells(rV.shape)
     ev = _per_shell(V.cpu().numpy(), rV, sh, N // 2)
     ew = _per_shell(W.cpu().numpy(), rW, sh, N // 2)
-    print(f"deterministic={deterministic}: per-shell rel error, volume max {ev.max():.2e} "
+    print(f"{path}: per-shell rel error, volume max {ev.max():.2e} "
           f"(shells 12-31 max {ev[11:].max():.2e}), weights max {ew.max():.2e}")
     assert ev.max() <= SHELL_TOL and ew.max() <= SHELL_TOL
 
 
-@pytest.mark.parametrize("deterministic", [False, True])
-def test_adjoint_precision_per_shell(device, deterministic):
+@pytest.mark.parametrize("path", ["brick", "binned"])
+def test_adjoint_precision_per_shell(device, path):
     vol, R, S = _steep_slices(device)
     V = vol.to(device).requires_grad_(True)
-    tt.set_deterministic(deterministic)
+    tt.set_scatter_path(path)
     try:
         tt.extract_central_slices_rfft_3d(V, (N,) * 3, R.to(device)).backward(torch.from_numpy(S).to(device))
     finally:
-        tt.set_deterministic(False)
+        tt.set_scatter_path("auto")
     Rz = O.prepare_rotations(R.numpy(), False, 1e-8)
     ref = O.extract_adjoint(S.astype(np.complex128), vol.shape, (N,) * 3, Rz)
     e = _per_shell(V.grad.cpu().numpy(), ref, _shells(ref.shape), N // 2)
-    print(f"deterministic={deterministic}: adjoint per-shell rel error max {e.max():.2e} "
+    print(f"{path}: adjoint per-shell rel error max {e.max():.2e} "
           f"(shells 12-31 max {e[11:].max():.2e})")
     assert e.max() <= SHELL_TOL
 
@@ -87,12 +89,33 @@ def test_adjoint_precision_per_shell(device, deterministic):
 @pytest.mark.parametrize("n_rot", [200, 1100])
 def test_guarded_redo_after_an_unsampled_spike(device, n_rot):
     """A value 1e6 times the rest sits in a slice the sampled band maxima skip
-    (every n_rot // 64-th slice is sampled): the first pass sees it exceed its
-    brick's limit and the call is redone with the exact maximum.  Insertion
-    and adjoint must still match the FP64 oracle, with few rotations (the
-    brick-crossing pre-pass) and many."""
+    (every n_rot // 64-th slice is sampled): the brick engine's first pass
+    sees it exceed its brick's limit and the call is redone with the exact
+    maximum.  Insertion and adjoint must still match the FP64 oracle, with few
+    rotations (the brick-crossing pre-pass) and many."""
     from oracle import c_oracle as C
 
+    tt.set_scatter_path("brick")
+    try:
+        _spike_case(device, n_rot, C)
+    finally:
+        tt.set_scatter_path("auto")
+
+
+def test_binned_scatter_with_a_spike(device):
+    """The binned scatter takes each brick's exact maximum: a 1e6 spike only
+    coarsens the bricks its pixel reaches."""
+    from oracle import c_oracle as C
+
+    tt.set_scatter_path("binned")
+    try:
+        e_far, e_afar = _spike_case(device, 200, C)
+    finally:
+        tt.set_scatter_path("auto")
+    assert e_far < 1e-5 and e_afar < 1e-5
+
+
+def _spike_case(device, n_rot, C):
     n = 48
     R = uniform_rotations(n_rot, 50, device=device)
     g = torch.Generator(device=device).manual_seed(51)
@@ -117,5 +140,7 @@ def test_guarded_redo_after_an_unsampled_spike(device, n_rot):
     print(f"{n_rot} rotations with a spike: insert rel_l2 {e_v:.2e} (away from it {e_far:.2e}), "
           f"adjoint {e_a:.2e} (away {e_afar:.2e})")
     assert e_v < 1e-5 and e_a < 1e-5
-    # the redo scale is the global maximum: 2^-30 of 1.2e6, about 1e-3, per contribution
+    # the brick engine's redo scale is the global maximum: 2^-30 of 1.2e6,
+    # about 1e-3, per contribution
     assert e_far < 1e-2 and e_afar < 1e-2
+    return e_far, e_afar

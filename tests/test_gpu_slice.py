@@ -403,42 +403,60 @@ def test_multichannel_shapes(device):
 
 
 # ------------------------------------------------------ scatter variants agree
+@pytest.fixture
+def scatter_path():
+    yield tt.set_scatter_path
+    tt.set_scatter_path("auto")
+
+
+def _scatter_variants(path_setter):
+    """(name, enter) pairs: the brick engine, the binned scatter and the
+    pixel-driven atomic scatter."""
+    def use(kind):
+        def enter():
+            path_setter("binned" if kind == "binned" else "brick")
+            tt.set_pixel_atomic(kind == "atomic")
+        return enter
+    return [(k, use(k)) for k in ("brick", "binned", "atomic")]
+
+
 @pytest.mark.parametrize("shape,image_shape,fmax", [
     ((48, 48, 48), (48, 48, 48), None), ((33, 33, 33), (33, 33, 33), 0.3), ((20, 20, 20), (64, 64, 64), None),
 ])
-def test_brick_scatter_matches_pixel_atomic(device, shape, image_shape, fmax):
-    """Brick-stationary LDS scatter == pixel-driven atomic scatter == FP64 oracle,
-    for the adjoint and the insertion, incl. odd sizes and image_shape != volume."""
+def test_scatter_variants_match_oracle(device, scatter_path, shape, image_shape, fmax):
+    """Brick-engine LDS scatter, binned scatter and pixel-driven atomic scatter
+    against the FP64 oracle and each other, for the adjoint and the insertion,
+    incl. odd sizes and image_shape != volume."""
     n = 40
     R = uniform_rotations(n, 11, device=device)
     Vr = volume_rfft(torch.randn(*shape, generator=torch.Generator().manual_seed(12))).to(device)
     Hi, Wri = image_shape[1], image_shape[2] // 2 + 1
     g = torch.randn(n, Hi, Wri, dtype=torch.complex64, generator=torch.Generator().manual_seed(13)).to(device)
-    outs = []
-    for atomic in (False, True):
-        tt.set_pixel_atomic(atomic)
+    d = torch.randn(n, shape[1], shape[2] // 2 + 1, dtype=torch.complex64,
+                    generator=torch.Generator().manual_seed(14)).to(device)
+    ref = O.extract_adjoint(_np(g), tuple(Vr.shape), image_shape, O.prepare_rotations(_np(R), False, 1e-8), fmax)
+    rV, rW = O.insert_central_slices(_np(d), shape, O.prepare_rotations(_np(R), False, None), fmax)
+    outs, ins = {}, {}
+    for name, enter in _scatter_variants(scatter_path):
+        enter()
         try:
             V = Vr.clone().requires_grad_(True)
             tt.extract_central_slices_rfft_3d(V, image_shape, R, fftfreq_max=fmax).backward(g)
-            outs.append(_np(V.grad))
+            outs[name] = _np(V.grad)
+            if image_shape == shape:
+                ins[name] = [_np(x) for x in tt.insert_central_slices_rfft_3d(d, shape, R, fftfreq_max=fmax)]
         finally:
             tt.set_pixel_atomic(False)
-    ref = O.extract_adjoint(_np(g), tuple(Vr.shape), image_shape, O.prepare_rotations(_np(R), False, 1e-8), fmax)
-    assert rel_l2(outs[0], ref) < INS_TOL and rel_l2(outs[1], ref) < INS_TOL
-    assert rel_l2(outs[0], outs[1]) < INS_TOL
-    if image_shape == shape:
-        d = torch.randn(n, shape[1], shape[2] // 2 + 1, dtype=torch.complex64,
-                        generator=torch.Generator().manual_seed(14)).to(device)
-        res = []
-        for atomic in (False, True):
-            tt.set_pixel_atomic(atomic)
-            try:
-                res.append([_np(x) for x in tt.insert_central_slices_rfft_3d(d, shape, R, fftfreq_max=fmax)])
-            finally:
-                tt.set_pixel_atomic(False)
-        rV, rW = O.insert_central_slices(_np(d), shape, O.prepare_rotations(_np(R), False, None), fmax)
-        for Vi, Wi in res:
-            assert rel_l2(Vi, rV) < INS_TOL and rel_l2(Wi, rW) < INS_TOL
+        e_a = rel_l2(outs[name], ref)
+        e_i = (rel_l2(ins[name][0], rV), rel_l2(ins[name][1], rW)) if name in ins else None
+        print(f"{name}: adjoint rel_l2 {e_a:.2e}, insertion (volume, weights) {e_i}")
+        assert e_a < INS_TOL
+        if e_i is not None:
+            assert e_i[0] < INS_TOL and e_i[1] < INS_TOL
+    # the two fixed-point scatters: FP32 coordinate roundings apart, nothing more
+    assert rel_l2(outs["brick"], outs["binned"]) < 1e-5
+    if ins:
+        assert rel_l2(ins["brick"][0], ins["binned"][0]) < 1e-5 and rel_l2(ins["brick"][1], ins["binned"][1]) < 1e-5
 
 
 def test_split_bricks_adjoint_and_insert_vs_oracle(device):
@@ -468,7 +486,9 @@ def test_split_bricks_adjoint_and_insert_vs_oracle(device):
         assert rel_l2(_np(Vi), rV) < INS_TOL and rel_l2(_np(Wi), rW) < INS_TOL, atomic
 
 
-def test_deterministic_scatter_is_bitwise_reproducible(device):
+@pytest.mark.parametrize("path", ["brick", "binned"])
+def test_deterministic_scatter_is_bitwise_reproducible(device, scatter_path, path):
+    scatter_path(path)
     n = 64
     R = uniform_rotations(n, 15, device=device)
     V0 = volume_rfft(torch.randn(40, 40, 40, generator=torch.Generator().manual_seed(16))).to(device)
@@ -491,10 +511,12 @@ def test_deterministic_scatter_is_bitwise_reproducible(device):
     assert rel_l2(_np(ins[0][0]), rV) < 1e-5 and rel_l2(_np(ins[0][1]), rW) < 1e-5
 
 
-def test_scatter_accumulate_flag(device):
+@pytest.mark.parametrize("path", ["brick", "binned"])
+def test_scatter_accumulate_flag(device, scatter_path, path):
     """Without TT_FLAG_OVERWRITE the C ABI adds into the caller's buffers."""
     from teamtomo_amd import _lib
 
+    scatter_path(path)
     n = 8
     R = uniform_rotations(n, 18, device=device)
     Rz = torch.flip(R, dims=(-2, -1)).contiguous()
@@ -506,6 +528,14 @@ def test_scatter_accumulate_flag(device):
               _lib.stream_of(d))
     Vi, Wi = tt.insert_central_slices_rfft_3d(d, (24, 24, 24), R)
     assert rel_l2(_np(vol), _np(base_v + Vi)) < 1e-6 and rel_l2(_np(w), _np(base_w + Wi)) < 1e-6
+    # the adjoint adds into grad_volume the same way
+    gv = base_v.clone()
+    Rz8 = tt.fourier_slice._prepare_rotations(R, device, False, 1e-8).reshape(-1, 3, 3).contiguous()
+    _lib.call("tt_extract_bwd_c64", _lib.ptr(d), _lib.ptr(base_v), 24, 24, 13, 24, 24, 24, _lib.ptr(Rz8), n,
+              -1.0, 1e-8, _lib.ptr(gv), None, 0, _lib.stream_of(d))
+    V = base_v.clone().requires_grad_(True)
+    tt.extract_central_slices_rfft_3d(V, (24, 24, 24), R).backward(d)
+    assert rel_l2(_np(gv), _np(base_v + V.grad)) < 1e-6
 
 
 # ------------------------------------------------------ brick gather (LDS)
