@@ -1,6 +1,6 @@
 // Binned scatter: the extraction adjoint and the insertion when the slice
 // pixels are few per volume brick, as for a tilt series inserted into a large
-// volume (61 slices of 512^2 into 512^3: ~300 pixels per 8x16x16 brick).
+// volume (61 slices of 512^2 into 512^3: ~230 pixels per 4x8x64 brick).
 //
 // The brick engine discovers each brick's slices itself (plane tests, FP64
 // item frames, row and chunk lists), a chain of dependent steps that costs
@@ -10,10 +10,13 @@
 //   1. bin_count_kernel: one lane per pixel evaluates the canonical FP64
 //      coordinate (the pixel-atomic kernels' code, tt_common.h) and counts it
 //      in every brick that owns one of its 8 trilinear corners with a
-//      nonzero weight (1, 2, 4 or 8 bricks; wave-aggregated atomics);
+//      nonzero weight (1, 2, 4 or 8 bricks); a block takes a 32 x 32 pixel
+//      patch and counts in an LDS table first, one global atomic per brick;
 //   2. bin_scan_kernel: exclusive prefix of the counts;
-//   3. bin_fill_kernel: the same evaluation again, each pixel written as a
-//      24-byte record (corner offset, fractions, value) into its bricks' bins;
+//   3. bin_fill_kernel: the same evaluation again; the block reserves a run
+//      of each bin it reaches and writes each pixel as a 24-byte record
+//      (corner offset, fractions, value) into its bricks' bins, and raises
+//      the bricks' maxima |value|;
 //   4. bin_brick_kernel: one block per brick sums its records' owned corners
 //      in LDS and writes every voxel it owns once (no halo pass), zeros for an
 //      empty brick;
@@ -40,20 +43,27 @@ namespace tt {
 
 namespace {
 
-constexpr int kBZ = 8, kBY = 16, kBX = 16;  // voxels a brick owns per axis
+// Voxels a brick owns per axis: x rows of 64 voxels (512 bytes of the output
+// row) keep most of the flush's cache lines whole; the rfft rows of 2^k + 1
+// voxels leave every brick row misaligned, so shorter rows write two partial
+// lines per row (61 tilts of 512^2 into 512^3, ms: 8x16x16 0.80, 4x8x32
+// 0.74, 4x8x64 0.70, 2x8x128 0.82).
+constexpr int kBZ = 4, kBY = 8, kBX = 64;
 constexpr int kNV = kBZ * kBY * kBX;
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;       // pixel passes
+constexpr int kBrickThreads = 256;  // brick pass
 // A pixel lands in at most 8 bricks: the record buffer holds 8 per pixel, and
 // the path is used only while that buffer stays below this size.
 constexpr int64_t kMaxRecordBytes = (int64_t)3 << 30;
 // Auto selection: at most this many pixels (slices x pixels per slice) per
 // brick of the volume on average.  Measured (insertion / adjoint, ms, brick
-// engine against binned): 61 tilts of 512^2 into 512^3 (230 per brick)
-// 2.29 / 1.45 against 0.99 / 0.80; 61 of 256^2 into 256^3 (437) 0.39 / 0.27
-// against 0.41 / 0.39; 256 random slices into 256^3 (1833) 0.48 / 0.31
-// against 1.14 / 1.07 (the bins of the bricks near DC then take most records,
-// and their counters serialize the atomics).
-constexpr double kAutoPixelsPerBrick = 320.0;
+// engine against binned): 61 tilts of 512^2 into 512^3 (230 per brick) 2.26 /
+// 1.47 against 0.70 / 0.52; 61 of 256^2 into 256^3 (437) 0.39 / 0.28 against
+// 0.20 / 0.15; 64 random slices into 256^3 (458) 0.39 / 0.26 against 0.21 /
+// 0.19; 256 into 256^3 (1833) 0.48 / 0.32 against 0.54 / 0.48; 256 into 128^3
+// (2080) 0.15 / 0.16 against 0.29 / 0.26 (the bins near DC then take most of
+// the records).
+constexpr double kAutoPixelsPerBrick = 1024.0;
 
 struct __align__(8) BinRec {
   unsigned off;  // corner offsets o + 1 (o in [-1, B-1]) per axis: z | y << 8 | x << 16
@@ -68,46 +78,11 @@ struct BinGrid {
 
 int g_scatter_path = TT_SCATTER_AUTO;
 
-__device__ __forceinline__ int lane_of() { return (int)(threadIdx.x & 63); }
-
-__device__ __forceinline__ int rank_below_mask(unsigned long long mask) {
-  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-}
-
-// Adds 1 to ctr[key] for every lane with key >= 0, one atomic per distinct
-// key in the wave; with RET, returns the lane's slot (the counter's value
-// before its own increment, lanes of one key taking consecutive slots).
-// Called with the whole wave active.
-template <bool RET>
-__device__ __forceinline__ int wave_claim(int key, int *__restrict__ ctr) {
-  const int lane = lane_of();
-  int slot = -1;
-  unsigned long long pending = __ballot(key >= 0);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int k = __builtin_amdgcn_readlane(key, leader);
-    const unsigned long long m = __ballot(key == k);
-    if (RET) {
-      int base = 0;
-      if (lane == leader) base = atomicAdd(ctr + k, __popcll(m));
-      base = __builtin_amdgcn_readlane(base, leader);
-      if (key == k) slot = base + rank_below_mask(m);
-    } else if (lane == leader) {
-      atomicAdd(ctr + k, __popcll(m));
-    }
-    pending &= ~m;
-  }
-  return slot;
-}
-
 // A slice pixel's cell, as the pixel-atomic scatter evaluates it; false for a
-// pixel the scatter skips (past the slice, the insertion's x = 0, y < 0 half
-// row, outside the disc or the volume).
+// pixel the scatter skips (the insertion's x = 0, y < 0 half row, outside the
+// disc or the volume).
 template <bool INSERT>
-__device__ __forceinline__ bool bin_pixel(const SliceGeom &g, const Rot &r, int p, Cell &k, bool &conj) {
-  if (p >= g.P) return false;
-  int i, j;
-  split_pixel(g, p, i, j);
+__device__ __forceinline__ bool bin_pixel(const SliceGeom &g, const Rot &r, int i, int j, Cell &k, bool &conj) {
   if (INSERT && j == 0 && i < g.half_H) return false;
   const Coord c = slice_coord<INSERT>(g, r, i, j);
   if (!(in_disc(g, i, j) && inside(g, c))) return false;
@@ -149,44 +124,160 @@ __device__ __forceinline__ int target_brick(const Targets &tg, int q, const BinG
   return ((tg.blo[0] + cz) * bg.nby + (tg.blo[1] + cy)) * bg.nbx + (tg.blo[2] + cx);
 }
 
-template <bool INSERT>
-__global__ __launch_bounds__(kThreads) void bin_count_kernel(SliceGeom g, const float *__restrict__ rot,
-                                                            BinGrid bg, int *__restrict__ count) {
-  const int rr = blockIdx.y;
-  const Rot r = load_rot(rot + (int64_t)rr * 9, g);
-  const int p = blockIdx.x * kThreads + threadIdx.x;
-  Cell k;
-  bool conj = false;
-  const bool ok = bin_pixel<INSERT>(g, r, p, k, conj);
-  Targets tg;
-  if (ok) tg = targets_of(k);
-#pragma unroll 1
-  for (int q = 0; q < 8; ++q) wave_claim<false>(ok ? target_brick(tg, q, bg) : -1, count);
+// Pixel passes: a block takes a kPR x kPC patch of one slice, whose pixels
+// fall in a compact region of the volume.  Its (pixel, brick) pairs are
+// counted per brick in an LDS hash table, and the block then makes one global
+// atomic per brick it reaches instead of one per pair (pairs beyond the
+// table's capacity go straight to the global counters).
+constexpr int kPR = 32, kPC = 32, kPerThread = kPR * kPC / kThreads;
+constexpr int kTableBits = 9, kTable = 1 << kTableBits;
+
+__device__ __forceinline__ int table_hash(int key) {
+  return (int)(((unsigned)key * 2654435761u) >> (32 - kTableBits));
+}
+
+// The entry of key, inserted if absent; -1 if the table is full.
+__device__ __forceinline__ int table_insert(int *__restrict__ keys, int key) {
+  int h = table_hash(key);
+  for (int probe = 0; probe < kTable; ++probe) {
+    const int old = atomicCAS(&keys[h], -1, key);
+    if (old == -1 || old == key) return h;
+    h = (h + 1) & (kTable - 1);
+  }
+  return -1;
+}
+
+// The entry of key, or -1 if it is not in the table (linear probing, no
+// deletions: the probe sequence of a present key has no empty entry before it).
+__device__ __forceinline__ int table_find(const int *__restrict__ keys, int key) {
+  int h = table_hash(key);
+  for (int probe = 0; probe < kTable; ++probe) {
+    const int k = keys[h];
+    if (k == key) return h;
+    if (k == -1) return -1;
+    h = (h + 1) & (kTable - 1);
+  }
+  return -1;
+}
+
+// Pixel u of this thread in the block's patch: (i, j), false past the slice.
+__device__ __forceinline__ bool patch_pixel(const SliceGeom &g, int u, int &i, int &j) {
+  const int npc = (g.Wr + kPC - 1) / kPC;
+  const int pi = (int)blockIdx.x / npc, pj = (int)blockIdx.x - pi * npc;
+  const int l = (int)threadIdx.x + u * kThreads;
+  i = pi * kPR + l / kPC;
+  j = pj * kPC + l % kPC;
+  return i < g.H && j < g.Wr;
 }
 
 template <bool INSERT>
+__global__ __launch_bounds__(kThreads) void bin_count_kernel(SliceGeom g, const float *__restrict__ rot,
+                                                            BinGrid bg, int *__restrict__ count) {
+  __shared__ int s_key[kTable], s_cnt[kTable];
+  for (int e = threadIdx.x; e < kTable; e += kThreads) {
+    s_key[e] = -1;
+    s_cnt[e] = 0;
+  }
+  __syncthreads();
+  const Rot r = load_rot(rot + (int64_t)blockIdx.y * 9, g);
+#pragma unroll 1
+  for (int u = 0; u < kPerThread; ++u) {
+    int i, j;
+    Cell k;
+    bool conj;
+    if (!patch_pixel(g, u, i, j) || !bin_pixel<INSERT>(g, r, i, j, k, conj)) continue;
+    const Targets tg = targets_of(k);
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+      const int b = target_brick(tg, q, bg);
+      if (b < 0) continue;
+      const int h = table_insert(s_key, b);
+      if (h >= 0)
+        atomicAdd(&s_cnt[h], 1);
+      else
+        atomicAdd(&count[b], 1);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kTable; e += kThreads)
+    if (s_key[e] >= 0) atomicAdd(&count[s_key[e]], s_cnt[e]);
+}
+
+// Writes the records: the block reserves, per brick it reaches, a run of
+// slots in the brick's bin with one global atomic (and raises the brick's
+// maximum |value|), then its pairs take consecutive slots of their runs.
+template <bool INSERT>
 __global__ __launch_bounds__(kThreads) void bin_fill_kernel(SliceGeom g, const f2v *__restrict__ src,
                                                            const float *__restrict__ rot, BinGrid bg,
-                                                           int *__restrict__ cursor, BinRec *__restrict__ recs,
-                                                           int64_t cap) {
+                                                           int *__restrict__ cursor, unsigned *__restrict__ bmax,
+                                                           BinRec *__restrict__ recs, int64_t cap) {
+  __shared__ int s_key[kTable], s_cnt[kTable], s_base[kTable];
+  __shared__ unsigned s_mx[kTable];
+  for (int e = threadIdx.x; e < kTable; e += kThreads) {
+    s_key[e] = -1;
+    s_cnt[e] = 0;
+    s_mx[e] = 0u;
+  }
+  __syncthreads();
   const int rr = blockIdx.y;
   const Rot r = load_rot(rot + (int64_t)rr * 9, g);
-  const int p = blockIdx.x * kThreads + threadIdx.x;
-  Cell k;
-  bool conj = false;
-  const bool ok = bin_pixel<INSERT>(g, r, p, k, conj);
-  f2v v = {0.0f, 0.0f};
-  Targets tg;
-  if (ok) {
-    tg = targets_of(k);
-    v = src[(int64_t)rr * g.ostride + p];
-    if (conj) v.y = -v.y;
+  Cell cell[kPerThread];
+  f2v val[kPerThread];
+  unsigned okm = 0u;
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    int i, j;
+    bool conj = false;
+    if (patch_pixel(g, u, i, j) && bin_pixel<INSERT>(g, r, i, j, cell[u], conj)) {
+      okm |= 1u << u;
+      f2v v = src[(int64_t)rr * g.ostride + (int64_t)i * g.Wr + j];
+      if (conj) v.y = -v.y;
+      val[u] = v;
+    }
   }
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    if (!((okm >> u) & 1u)) continue;
+    const Targets tg = targets_of(cell[u]);
+    const unsigned vb = __float_as_uint(fmaxf(fabsf(val[u].x), fabsf(val[u].y)));
 #pragma unroll 1
-  for (int q = 0; q < 8; ++q) {
-    const int b = ok ? target_brick(tg, q, bg) : -1;
-    const int slot = wave_claim<true>(b, cursor);
-    if (b >= 0 && slot >= 0 && slot < cap) {
+    for (int q = 0; q < 8; ++q) {
+      const int b = target_brick(tg, q, bg);
+      if (b < 0) continue;
+      const int h = table_insert(s_key, b);
+      if (h >= 0) {
+        atomicAdd(&s_cnt[h], 1);
+        atomicMax(&s_mx[h], vb);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kTable; e += kThreads) {
+    const int key = s_key[e];
+    if (key < 0) continue;
+    s_base[e] = atomicAdd(&cursor[key], s_cnt[e]);
+    atomicMax(&bmax[key], s_mx[e]);
+    s_cnt[e] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    if (!((okm >> u) & 1u)) continue;
+    const Cell &k = cell[u];
+    const Targets tg = targets_of(k);
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+      const int b = target_brick(tg, q, bg);
+      if (b < 0) continue;
+      const int h = table_find(s_key, b);
+      int slot;
+      if (h >= 0) {
+        slot = s_base[h] + atomicAdd(&s_cnt[h], 1);
+      } else {  // the table was full when this pair was counted
+        slot = atomicAdd(&cursor[b], 1);
+        atomicMax(&bmax[b], __float_as_uint(fmaxf(fabsf(val[u].x), fabsf(val[u].y))));
+      }
+      if (slot < 0 || slot >= cap) continue;
       const int cz = (q >> 2) & 1, cy = (q >> 1) & 1, cx = q & 1;
       // corner offset relative to the brick origin, in [-1, B-1]
       const int oz = k.z0 - (tg.blo[0] + cz) * kBZ, oy = k.y0 - (tg.blo[1] + cy) * kBY,
@@ -196,84 +287,94 @@ __global__ __launch_bounds__(kThreads) void bin_fill_kernel(SliceGeom g, const f
       rc.t[0] = k.tz;
       rc.t[1] = k.ty;
       rc.t[2] = k.tx;
-      rc.v[0] = v.x;
-      rc.v[1] = v.y;
+      rc.v[0] = val[u].x;
+      rc.v[1] = val[u].y;
       recs[slot] = rc;
     }
   }
 }
 
-// Exclusive prefix of the per-brick counts into offset and cursor (one block:
-// each thread sums a contiguous run, a scan over the runs in LDS, then each
-// thread writes its run's offsets).
+// Exclusive prefix of the per-brick counts into offset and cursor, one block
+// over chunks of 8192 counts: each thread sums a run of 8 in LDS, a scan over
+// the 1024 runs, then the runs' prefixes are written back.
 __global__ __launch_bounds__(1024) void bin_scan_kernel(const int *__restrict__ count, int n,
                                                        int *__restrict__ offset, int *__restrict__ cursor) {
-  __shared__ int s[1024];
-  const int per = (n + 1023) / 1024;
-  const int b0 = (int)threadIdx.x * per, b1 = min(b0 + per, n);
-  int sum = 0;
-  for (int b = b0; b < b1; ++b) sum += count[b];
-  s[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+  constexpr int kPer = 8, kChunk = 1024 * kPer;
+  __shared__ int s_v[kChunk];
+  __shared__ int s_sum[1024];
+  const int tid = (int)threadIdx.x;
+  int carry = 0;
+  for (int c0 = 0; c0 < n; c0 += kChunk) {
+    for (int e = tid; e < kChunk; e += 1024) s_v[e] = c0 + e < n ? count[c0 + e] : 0;
     __syncthreads();
-    s[threadIdx.x] += v;
+    int run = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) run += s_v[tid * kPer + q];
+    s_sum[tid] = run;
     __syncthreads();
-  }
-  int run = s[threadIdx.x] - sum;
-  for (int b = b0; b < b1; ++b) {
-    offset[b] = run;
-    cursor[b] = run;
-    run += count[b];
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int v = tid >= o ? s_sum[tid - o] : 0;
+      __syncthreads();
+      s_sum[tid] += v;
+      __syncthreads();
+    }
+    int x = carry + s_sum[tid] - run;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int c = s_v[tid * kPer + q];
+      s_v[tid * kPer + q] = x;
+      x += c;
+    }
+    const int total = s_sum[1023];
+    __syncthreads();
+    for (int e = tid; e < kChunk; e += 1024)
+      if (c0 + e < n) {
+        offset[c0 + e] = s_v[e];
+        cursor[c0 + e] = s_v[e];
+      }
+    carry += total;
+    __syncthreads();
   }
 }
 
 template <bool INSERT>
-__global__ __launch_bounds__(kThreads) void bin_brick_kernel(SliceGeom g, BinGrid bg, const int *__restrict__ count,
-                                                            const int *__restrict__ offset,
-                                                            const BinRec *__restrict__ recs, f2v *__restrict__ out,
-                                                            float *__restrict__ wout, f2v *__restrict__ plane,
-                                                            float *__restrict__ wplane, int accumulate) {
+__global__ __launch_bounds__(kBrickThreads) void bin_brick_kernel(SliceGeom g, BinGrid bg,
+                                                                 const int *__restrict__ count,
+                                                                 const int *__restrict__ offset,
+                                                                 const unsigned *__restrict__ bmax,
+                                                                 const BinRec *__restrict__ recs,
+                                                                 f2v *__restrict__ out, float *__restrict__ wout,
+                                                                 f2v *__restrict__ plane, float *__restrict__ wplane,
+                                                                 int accumulate) {
   constexpr int NA = INSERT ? 3 : 2;
   __shared__ u64 tile[NA * kNV];
-  __shared__ float s_max[kThreads / 64];
   const int b = blockIdx.x;
   const int bz = b / (bg.nby * bg.nbx), brem = b - bz * bg.nby * bg.nbx;
   const int by = brem / bg.nbx, bx = brem - by * bg.nbx;
   const int lo0 = bz * kBZ, lo1 = by * kBY, lo2 = bx * kBX;
   const int ext0 = min(kBZ, g.vD - lo0), ext1 = min(kBY, g.vH - lo1), ext2 = min(kBX, g.vW - lo2);
   const int cnt = count[b], off = offset[b];
+  const float m = __uint_as_float(bmax[b]);  // the exact maximum |value| of the brick's records
   const int tid = threadIdx.x;
   const int64_t sy = g.vW, sz = (int64_t)g.vH * g.vW;
   if (cnt == 0 && accumulate) {  // nothing to add; only the mirror plane needs its zeros
     if (INSERT && lo2 == 0)
-      for (int v = tid; v < ext0 * ext1; v += kThreads) {
+      for (int v = tid; v < ext0 * ext1; v += kBrickThreads) {
         const int64_t pe = (int64_t)(lo0 + v / ext1) * g.vH + lo1 + v % ext1;
         plane[pe] = f2v{0.0f, 0.0f};
         wplane[pe] = 0.0f;
       }
     return;
   }
-  // the exact maximum |value| of the brick's records
-  float m = 0.0f;
-  for (int t = tid; t < cnt; t += kThreads) {
-    const BinRec &rc = recs[off + t];
-    m = fmaxf(m, fmaxf(fabsf(rc.v[0]), fabsf(rc.v[1])));
-  }
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-  if ((tid & 63) == 0) s_max[tid >> 6] = m;
-  for (int v = tid; v < NA * kNV; v += kThreads) tile[v] = 0ull;
-  __syncthreads();
-#pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) m = fmaxf(m, s_max[w]);
+  for (int v = tid; v < NA * kNV; v += kBrickThreads) tile[v] = 0ull;
   // scale 2^(62 - ceil(log2 cnt) - e) with max < 2^e: a voxel sums at most
   // cnt contributions, each below 2^(62 - ceil(log2 cnt)) after scaling
   int e = 0;
   if (m > 0.0f && m < 3.0e38f) frexpf(m, &e);
   e = max(e, -60);
   const float vscale = ldexpf(1.0f, 62 - ceil_log2(cnt) - e);
-  for (int t = tid; t < cnt; t += kThreads) {
+  __syncthreads();
+  for (int t = tid; t < cnt; t += kBrickThreads) {
     const BinRec rc = recs[off + t];
     const int oz = (int)(rc.off & 255u) - 1, oy = (int)((rc.off >> 8) & 255u) - 1,
               ox = (int)((rc.off >> 16) & 255u) - 1;
@@ -304,7 +405,7 @@ __global__ __launch_bounds__(kThreads) void bin_brick_kernel(SliceGeom g, BinGri
   }
   __syncthreads();
   const float inv_v = 1.0f / vscale;
-  for (int v = tid; v < kNV; v += kThreads) {
+  for (int v = tid; v < kNV; v += kBrickThreads) {
     const int lz = v / (kBY * kBX), ly = (v / kBX) % kBY, lx = v % kBX;
     if (lz >= ext0 || ly >= ext1 || lx >= ext2) continue;
     const int gz = lo0 + lz, gy = lo1 + ly, gx = lo2 + lx;
@@ -368,7 +469,9 @@ int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float 
   const int64_t cap = record_cap(g, n_rot);
   const size_t np = (size_t)g.vD * g.vH;
   auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t o_count = 0, o_offset = up(o_count + (size_t)n_bricks * 4), o_cursor = up(o_offset + (size_t)n_bricks * 4);
+  // count and bmax adjacent: one zero pass clears both
+  const size_t o_count = 0, o_bmax = o_count + (size_t)n_bricks * 4, o_offset = up(o_bmax + (size_t)n_bricks * 4),
+               o_cursor = up(o_offset + (size_t)n_bricks * 4);
   const size_t o_plane = up(o_cursor + (size_t)n_bricks * 4);
   const size_t o_wplane = up(o_plane + (insert ? np * sizeof(f2v) : 0));
   const size_t o_recs = up(o_wplane + (insert ? np * sizeof(float) : 0));
@@ -380,14 +483,15 @@ int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float 
   }
   char *c = (char *)base;
   int *count = (int *)(c + o_count), *offset = (int *)(c + o_offset), *cursor = (int *)(c + o_cursor);
+  unsigned *bmax = (unsigned *)(c + o_bmax);
   f2v *plane = insert ? (f2v *)(c + o_plane) : nullptr;
   float *wplane = insert ? (float *)(c + o_wplane) : nullptr;
   BinRec *recs = (BinRec *)(c + o_recs);
   int st = TT_OK;
-  hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)std::min((n_bricks + kThreads - 1) / kThreads, 4096)),
-                     dim3(kThreads), 0, s, count, n_bricks);
+  hipLaunchKernelGGL(bin_zero_kernel, dim3((unsigned)std::min((2 * n_bricks + kThreads - 1) / kThreads, 4096)),
+                     dim3(kThreads), 0, s, count, 2 * n_bricks);
   st = check_launch("bin_zero_kernel");
-  const dim3 pgrid((unsigned)((g.P + kThreads - 1) / kThreads), (unsigned)n_rot);
+  const dim3 pgrid((unsigned)(((g.H + kPR - 1) / kPR) * ((g.Wr + kPC - 1) / kPC)), (unsigned)n_rot);
   if (!st && n_rot > 0) {
     if (insert)
       hipLaunchKernelGGL(bin_count_kernel<true>, pgrid, dim3(kThreads), 0, s, g, rot, bg, count);
@@ -401,20 +505,22 @@ int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float 
   }
   if (!st && n_rot > 0) {
     if (insert)
-      hipLaunchKernelGGL(bin_fill_kernel<true>, pgrid, dim3(kThreads), 0, s, g, src, rot, bg, cursor, recs, cap);
+      hipLaunchKernelGGL(bin_fill_kernel<true>, pgrid, dim3(kThreads), 0, s, g, src, rot, bg, cursor, bmax, recs,
+                         cap);
     else
-      hipLaunchKernelGGL(bin_fill_kernel<false>, pgrid, dim3(kThreads), 0, s, g, src, rot, bg, cursor, recs, cap);
+      hipLaunchKernelGGL(bin_fill_kernel<false>, pgrid, dim3(kThreads), 0, s, g, src, rot, bg, cursor, bmax, recs,
+                         cap);
     st = check_launch("bin_fill_kernel");
   }
   if (!st) {
     if (insert)
-      hipLaunchKernelGGL(bin_brick_kernel<true>, dim3((unsigned)n_bricks), dim3(kThreads), 0, s, g, bg,
-                         (const int *)count, (const int *)offset, (const BinRec *)recs, out, wout, plane, wplane,
-                         accumulate);
+      hipLaunchKernelGGL(bin_brick_kernel<true>, dim3((unsigned)n_bricks), dim3(kBrickThreads), 0, s, g, bg,
+                         (const int *)count, (const int *)offset, (const unsigned *)bmax, (const BinRec *)recs, out,
+                         wout, plane, wplane, accumulate);
     else
-      hipLaunchKernelGGL(bin_brick_kernel<false>, dim3((unsigned)n_bricks), dim3(kThreads), 0, s, g, bg,
-                         (const int *)count, (const int *)offset, (const BinRec *)recs, out, wout, plane, wplane,
-                         accumulate);
+      hipLaunchKernelGGL(bin_brick_kernel<false>, dim3((unsigned)n_bricks), dim3(kBrickThreads), 0, s, g, bg,
+                         (const int *)count, (const int *)offset, (const unsigned *)bmax, (const BinRec *)recs, out,
+                         wout, plane, wplane, accumulate);
     st = check_launch("bin_brick_kernel");
   }
   if (!st && insert) st = launch_mirror(plane, wplane, g, out, wout, accumulate, s);

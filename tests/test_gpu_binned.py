@@ -119,9 +119,10 @@ def test_binned_empty_calls(device, binned):
 
 
 def test_auto_path_choice(device):
-    """"auto" bins a 61-tilt series into 256^3 (few pixels per brick) and keeps
-    the brick engine for 4096 slices of 64^2 into 64^3: both results match
-    the binned and brick results forced by name, bit for bit."""
+    """"auto" bins a 61-tilt series of 512^2 into 512^3 (230 pixels per brick
+    of 2048 voxels) and keeps the brick engine for 4096 slices of 64^2 into
+    64^3 (70000 per brick): both
+    results match the binned and brick results forced by name, bit for bit."""
     n = 64
     R = uniform_rotations(4096, 9, device=device)
     S = _normal((4096, n, n // 2 + 1), 10, device)
@@ -133,7 +134,7 @@ def test_auto_path_choice(device):
         finally:
             tt.set_scatter_path("auto")
     assert torch.equal(outs["auto"][0], outs["brick"][0]) and torch.equal(outs["auto"][1], outs["brick"][1])
-    n = 256
+    n = 512
     Rt = _tilt_series(np.arange(-60, 61, 2), 3.0).to(device)
     St = _normal((Rt.shape[0], n, n // 2 + 1), 11, device)
     outs = {}
