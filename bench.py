@@ -31,6 +31,7 @@ default is the headline above.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -485,6 +486,35 @@ def copy_bandwidth(device, mib=1024, reps=5):
     return 2 * (mib << 20) / ms / 1e6
 
 
+def kernel_source_digest():
+    """sha256 (first 16 hex digits) over the HIP sources the library is built
+    from.  profiles/traffic.json records the digest of the sources its
+    counters were measured on; its figures are reported only while the
+    sources are unchanged."""
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "teamtomo_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".h")):
+            h.update(name.encode())
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(key):
+    """HBM-side bytes per launch sequence of `key` from profiles/traffic.json,
+    or (None, reason) when absent or measured on other kernel sources."""
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None, "no profiles/traffic.json"
+    with open(tpath) as f:
+        t = json.load(f)
+    if t.get("source_digest") != kernel_source_digest():
+        return None, "profiles/traffic.json was measured on other kernel sources"
+    v = t.get(key)
+    return v, (None if v is not None else f"no {key} entry in profiles/traffic.json")
+
+
 def run_insert3d(args, device):
     """Central-slice insertion as ``insert_central_slices_rfft_3d`` runs it
     (volume and weights written): 4096 N(0,1) complex64 slices with uniform
@@ -559,6 +589,13 @@ def run_insert3d(args, device):
                      "frac": c2["GBs"] / HBM_PEAK_GBS, "traffic": None},
         "kernels": {"insert_256": c2, "insert_512_tilts": dict(c5, frac=c5["GBs"] / HBM_PEAK_GBS)},
     }
+    for key, c in (("insert_256", c2), ("insert_512_tilts", c5)):
+        tr, why = measured_traffic(key)
+        line["kernels"][key]["traffic"] = tr
+        line["kernels"][key]["traffic_per_algorithmic_byte"] = tr / c["bytes"] if tr else None
+        if why:
+            line["kernels"][key]["traffic_note"] = why
+    line["roofline"]["traffic"] = line["kernels"]["insert_256"]["traffic"]
     if torch_line is not None:
         line["torch_same_gpu"] = torch_line
     if not args.no_cpu:
@@ -804,11 +841,11 @@ def main():
         "extract": {"ms": ext_ms, "bytes": ext_b, "GBs": ext_b / ext_ms / 1e6,
                     "launch": "tt_extract_c64: gather_zero_kernel + plane_rec_kernel + brick_kernel (gather)"},
         "adjoint": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6,
-                    "launch": "tt_extract_bwd_c64 volume gradient: fill_zero_kernel + maxabs_chunks_kernel "
-                              "(1/64 sample) + plane_rec_kernel + brick_kernel (adjoint) + split_combine_kernel + "
-                              "halo_combine_kernel, then the guarded exact-scale redo (maxabs_chunks_kernel, "
-                              "brick_kernel, split_combine_kernel, halo_combine_kernel; each exits at once "
-                              "unless a value exceeded the sampled scale)"},
+                    "launch": "tt_extract_bwd_c64 volume gradient: fill_zero_kernel + band_max_kernel "
+                              "(|f|-band maxima of every 64th slice) + plane_rec_kernel + brick_kernel (adjoint) + "
+                              "split_combine_kernel + halo_combine_kernel, then the guarded exact-scale redo "
+                              "(maxabs_chunks_kernel, brick_kernel, split_combine_kernel, halo_combine_kernel; "
+                              "each exits at once unless a value exceeded its brick's sampled limit)"},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["GBs"]
@@ -816,11 +853,7 @@ def main():
     step_gbs = step_bytes / ms_step / 1e6
     value = N_ROT / (ms_step / 1e3)
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+    traffic, traffic_note = measured_traffic(dom)
 
     line = {
         "metric": "slices/s fwd+adjoint at 256^3 c64 vs plain torch on same GPU; % HBM roofline",
@@ -840,7 +873,8 @@ def main():
                    "deterministic": bool(args.deterministic)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_per_algorithmic_byte": traffic / kernels[dom]["bytes"] if traffic else None},
+                     "traffic_per_algorithmic_byte": traffic / kernels[dom]["bytes"] if traffic else None,
+                     "traffic_note": traffic_note},
         "roofline_step": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "bytes": step_bytes},
         "kernels": kernels,
