@@ -86,6 +86,21 @@ int tt_set_gather_path(int path);
 int tt_set_scatter_path(int path);
 
 /*
+ * Host-side check of the brick engine's work plan for one call (kind 0: the
+ * gather of tt_extract_c64, 1: the adjoint of tt_extract_bwd_c64, 2: the
+ * insertion of tt_insert_c64; rotations are not read, only their number):
+ * every unit's brick, rotation segment and partial slot, every split
+ * record's slots, the extents of the segment and halo buffers the kernels
+ * index, and that each brick's units cover its rotation segments exactly
+ * once.  Needs no GPU.  stats[8] = bricks, units, split bricks, partial
+ * slots, segment-buffer bytes, halo-buffer bytes, empty units, largest
+ * segment count.  TT_ERR_INVALID with the failed condition in tt_last_error.
+ * No counterpart in the reference: a diagnostics entry of this library.
+ */
+int tt_plan_check(int kind, int vol_d, int vol_h, int vol_wr, int img_d, int img_h, int img_w, int64_t n_rot,
+                  double fftfreq_max, int64_t *stats);
+
+/*
  * Rotation preparation: out (n, 3, 3) float32 = the (n, 3, 3) float32 or
  * float64 matrices cast to float32, entries with |R| < rot_tolerance set to
  * zero (rot_tolerance < 0: no clamp; the comparison in float32), and both

@@ -44,6 +44,7 @@ _d = ctypes.c_double
 _SIGNATURES = {
     "tt_set_gather_path": [_i],
     "tt_set_scatter_path": [_i],
+    "tt_plan_check": [_i, _i, _i, _i, _i, _i, _i, _l, _d, _p],
     "tt_extract_c64": [_p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],
     "tt_extract_c64_noalloc": [_p, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],
     "tt_extract_mc_c64": [_p, _i, _i, _i, _i, _i, _i, _i, _p, _l, _d, _d, _p, _p],

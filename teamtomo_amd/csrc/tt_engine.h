@@ -37,6 +37,10 @@ int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t 
 int engine_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64_t n_rot, f2v *vol,
                   float *w, int accumulate, hipStream_t s);
 
+// Host-only check of the brick engine's plan for a call (kind 0 gather, 1
+// adjoint, 2 insertion): see tt_plan_check.
+int engine_plan_check(const SliceGeom &g, int kind, int64_t n_rot, int64_t stats[8]);
+
 // The insertion's x=0-plane Friedel mirror pass: out[z, y, 0] (+)= plane[z, y]
 // + conj(plane[-z, -y]) and likewise for the weights (DC not mirrored).
 int launch_mirror(const f2v *plane, const float *wplane, const SliceGeom &g, f2v *out, float *wout, int accumulate,

@@ -1571,8 +1571,12 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
              bool device_tables = true) {
   int dev = 0;
   if (hipError_t e0 = hipGetDevice(&dev)) {
-    set_error(std::string("brick plan: hipGetDevice: ") + hipGetErrorString(e0));
-    return TT_ERR_HIP;
+    if (device_tables) {
+      set_error(std::string("brick plan: hipGetDevice: ") + hipGetErrorString(e0));
+      return TT_ERR_HIP;
+    }
+    (void)hipGetLastError();  // host tables only (no device, as in tt_plan_check): clear the query's error
+    dev = -1;
   }
   int64_t nq = 64;
   while (nq < n_rot) nq <<= 1;
@@ -1587,7 +1591,7 @@ int get_plan(const SliceGeom &g, int BZ, int BY, int BX, bool scatter, int64_t n
     if (device_tables)
       if (int st = upload_plan(it->second)) return st;
     pe = it->second;
-    if (hipError_t e1 = hipGetLastError()) {
+    if (hipError_t e1 = device_tables ? hipGetLastError() : hipSuccess) {
       set_error(std::string("brick plan (cached): ") + hipGetErrorString(e1));
       return TT_ERR_HIP;
     }
@@ -1846,6 +1850,82 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
                        g.vD, g.vH, g.vW, out, wout, accumulate);
     if (int st = check_launch("mirror_kernel")) return st;
   }
+  return TT_OK;
+}
+
+// Host-side check of a brick plan against the buffers run_scatter and the
+// gather size from it: every unit's brick, segment and partial slot, every
+// split record's slots, the segment-buffer and halo-buffer extents the
+// kernels index, and that the units of each brick cover its rotation
+// segments exactly once.  stats: n_bricks, n_units, n_splits, n_slots,
+// segment-buffer bytes, halo-buffer bytes, empty units, largest nseg.
+int plan_check(const SliceGeom &g, int BZ, int BY, int BX, int mode, int64_t n_rot, int64_t stats[8]) {
+  PlanEntry pe;
+  if (int st = get_plan(g, BZ, BY, BX, mode != kGather, n_rot, pe, false)) return st;
+  const BrickGrid bg = grid_of(g, BZ, BY, BX);
+  const int64_t n_bricks = (int64_t)bg.nbz * bg.nby * bg.nbx;
+  const int64_t NT = (int64_t)(BZ + 1) * (BY + 1) * (BX + 1);
+  const int64_t NSH = (int64_t)(BY + 1) * (BX + 1) + (int64_t)BZ * (BX + 1) + (int64_t)BZ * BY;
+  const int64_t NA = mode == kGather ? 0 : (mode == kInsert ? 3 : 2);
+  const int64_t seg_words = (int64_t)pe.n_slots * NA * NT;  // u64 words of the segment buffer
+  auto bad = [](const std::string &m) { return fail_invalid("brick plan check: " + m); };
+  std::vector<int> nseg_of(n_bricks, -1), slot_of(n_bricks, -2);
+  std::vector<std::vector<char>> cover(n_bricks);
+  int64_t empty = 0, max_nseg = 0;
+  for (int u = 0; u < pe.n_units; ++u) {
+    const Unit &un = (*pe.h_units)[u];
+    if (un.brick < 0 || un.brick >= n_bricks) return bad("unit " + std::to_string(u) + ": brick out of range");
+    const int nseg = un.seg & 0xffff, sgi = un.seg >> 16;
+    if (nseg == 0) {  // an empty scatter brick: writes zeros
+      if (mode == kGather || sgi != 0 || un.slot != -1) return bad("unit " + std::to_string(u) + ": bad empty unit");
+      ++empty;
+    } else if (sgi < 0 || sgi >= nseg) {
+      return bad("unit " + std::to_string(u) + ": segment out of range");
+    }
+    max_nseg = std::max<int64_t>(max_nseg, nseg);
+    if (nseg_of[un.brick] == -1) {
+      nseg_of[un.brick] = nseg;
+      slot_of[un.brick] = un.slot;
+      cover[un.brick].assign(std::max(nseg, 1), 0);
+    } else if (nseg_of[un.brick] != nseg || slot_of[un.brick] != un.slot) {
+      return bad("brick " + std::to_string(un.brick) + ": units disagree on segments or slot");
+    }
+    if (cover[un.brick][sgi]++) return bad("brick " + std::to_string(un.brick) + ": segment listed twice");
+    if (un.slot >= 0) {
+      if (mode == kGather || nseg < 2) return bad("unit " + std::to_string(u) + ": slot on an unsplit unit");
+      // segbuf + (slot + seg) * NA * NT, NA * NT words
+      if ((int64_t)un.slot + nseg > pe.n_slots || ((int64_t)un.slot + sgi + 1) * NA * NT > seg_words)
+        return bad("unit " + std::to_string(u) + ": partial slot beyond the segment buffer");
+    } else if (mode != kGather && nseg > 1) {
+      return bad("unit " + std::to_string(u) + ": split scatter unit without a slot");
+    }
+  }
+  for (int64_t b = 0; b < n_bricks; ++b)
+    if (nseg_of[b] > 0)
+      for (int k = 0; k < nseg_of[b]; ++k)
+        if (cover[b][k] != 1) return bad("brick " + std::to_string(b) + ": segment missing");
+  std::vector<char> slot_used(std::max(pe.n_slots, 1), 0);
+  for (int k = 0; k < pe.n_splits; ++k) {
+    const SplitRec &sp = (*pe.h_splits)[k];
+    if (sp.brick < 0 || sp.brick >= n_bricks || sp.nseg < 2 || sp.slot < 0 || (int64_t)sp.slot + sp.nseg > pe.n_slots)
+      return bad("split " + std::to_string(k) + ": out of range");
+    if (nseg_of[sp.brick] != sp.nseg || slot_of[sp.brick] != sp.slot)
+      return bad("split " + std::to_string(k) + ": disagrees with its units");
+    for (int q = 0; q < sp.nseg; ++q)
+      if (slot_used[sp.slot + q]++) return bad("split " + std::to_string(k) + ": slot shared");
+  }
+  // the halo buffer: brick * NA * NSH + NA * NSH floats for the last brick
+  const int64_t halo_words = n_bricks * NA * NSH;
+  if (mode != kGather && (n_bricks - 1) * NA * NSH + NA * NSH > halo_words) return bad("halo buffer");
+  if (mode != kGather && (std::max<int64_t>(n_rot, 1) > ((int64_t)1 << 30))) return bad("rotations");
+  stats[0] = n_bricks;
+  stats[1] = pe.n_units;
+  stats[2] = pe.n_splits;
+  stats[3] = pe.n_slots;
+  stats[4] = seg_words * 8;
+  stats[5] = halo_words * 4;
+  stats[6] = empty;
+  stats[7] = max_nseg;
   return TT_OK;
 }
 
@@ -2473,6 +2553,16 @@ int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t 
   if (int st = check_engine(g, n_rot)) return st;
   if (n_rot == 0 && accumulate) return TT_OK;
   return run_scatter<kABZ, kABY, kABX, kAdjoint>(g, gS, rot, n_rot, gV, nullptr, accumulate, s);
+}
+
+int engine_plan_check(const SliceGeom &g, int kind, int64_t n_rot, int64_t stats[8]) {
+  if (int st = check_engine(g, n_rot)) return st;
+  switch (kind) {
+    case 0: return plan_check(g, kGBZ, kGBY, kGBX, kGather, n_rot, stats);
+    case 1: return plan_check(g, kABZ, kABY, kABX, kAdjoint, n_rot, stats);
+    case 2: return plan_check(g, kIBZ, kIBY, kIBX, kInsert, n_rot, stats);
+    default: return fail_invalid("plan check: kind must be 0 (gather), 1 (adjoint) or 2 (insertion)");
+  }
 }
 
 int engine_insert(const SliceGeom &g, const f2v *slices, const float *rot, int64_t n_rot, f2v *vol,

@@ -360,6 +360,16 @@ int tt_set_gather_path(int path) {
 
 int tt_set_scatter_path(int path) { return set_scatter_path(path); }
 
+int tt_plan_check(int kind, int vol_d, int vol_h, int vol_wr, int img_d, int img_h, int img_w, int64_t n_rot,
+                  double fftfreq_max, int64_t *stats) {
+  if (!stats) return fail_invalid("stats pointer is NULL");
+  const float dummy = 0.0f;  // no rotations are read
+  int st = check_common(vol_d, vol_h, vol_wr, img_d, img_h, img_w, n_rot, &dummy);
+  if (st) return st;
+  SliceGeom g = make_geom(img_d, img_h, img_w, vol_d, vol_h, vol_wr, fftfreq_max, 1e-8);
+  return engine_plan_check(g, kind, n_rot, stats);
+}
+
 int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr, int img_d, int img_h,
                    int img_w, const float *rot_zyx, int64_t n_rot, double fftfreq_max,
                    double x_tolerance, void *slices, void *stream) {
