@@ -1,6 +1,6 @@
 // Binned scatter: the extraction adjoint and the insertion when the slice
 // pixels are few per volume brick, as for a tilt series inserted into a large
-// volume (61 slices of 512^2 into 512^3: ~230 pixels per 4x8x64 brick).
+// volume (61 slices of 512^2 into 512^3: ~196 pixels per 4x8x64 brick).
 //
 // The brick engine discovers each brick's slices itself (plane tests, FP64
 // item frames, row and chunk lists), a chain of dependent steps that costs
@@ -57,11 +57,11 @@ constexpr int kBrickThreads = 256;  // brick pass
 constexpr int64_t kMaxRecordBytes = (int64_t)3 << 30;
 // Auto selection: at most this many pixels (slices x pixels per slice) per
 // brick of the volume on average.  Measured (insertion / adjoint, ms, brick
-// engine against binned): 61 tilts of 512^2 into 512^3 (230 per brick) 2.26 /
-// 1.47 against 0.70 / 0.52; 61 of 256^2 into 256^3 (437) 0.39 / 0.28 against
-// 0.20 / 0.15; 64 random slices into 256^3 (458) 0.39 / 0.26 against 0.21 /
-// 0.19; 256 into 256^3 (1833) 0.48 / 0.32 against 0.54 / 0.48; 256 into 128^3
-// (2080) 0.15 / 0.16 against 0.29 / 0.26 (the bins near DC then take most of
+// engine against binned): 61 tilts of 512^2 into 512^3 (196 per brick) 2.30 /
+// 1.46 against 0.72 / 0.55; 61 of 256^2 into 256^3 (328) 0.39 / 0.28 against
+// 0.19 / 0.17; 64 random slices into 256^3 (344) 0.39 / 0.25 against 0.22 /
+// 0.16; 256 into 256^3 (1376) 0.48 / 0.31 against 0.55 / 0.49; 256 into 128^3
+// (2080) 0.15 / 0.16 against 0.31 / 0.27 (the bins near DC then take most of
 // the records).
 constexpr double kAutoPixelsPerBrick = 1024.0;
 

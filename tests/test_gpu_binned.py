@@ -119,7 +119,7 @@ def test_binned_empty_calls(device, binned):
 
 
 def test_auto_path_choice(device):
-    """"auto" bins a 61-tilt series of 512^2 into 512^3 (230 pixels per brick
+    """"auto" bins a 61-tilt series of 512^2 into 512^3 (196 pixels per brick
     of 2048 voxels) and keeps the brick engine for 4096 slices of 64^2 into
     64^3 (70000 per brick): both
     results match the binned and brick results forced by name, bit for bit."""
