@@ -88,11 +88,13 @@ int tt_set_scatter_path(int path);
 /*
  * Host-side check of the brick engine's work plan for one call (kind 0: the
  * gather of tt_extract_c64, 1: the adjoint of tt_extract_bwd_c64, 2: the
- * insertion of tt_insert_c64; rotations are not read, only their number):
+ * insertion of tt_insert_c64, 3 and 4: the 8x8x8-cell gather and insertion
+ * tiles a batch may choose; rotations are not read, only their number):
  * every unit's brick, rotation segment and partial slot, every split
  * record's slots, the extents of the segment and halo buffers the kernels
- * index, and that each brick's units cover its rotation segments exactly
- * once.  Needs no GPU.  stats[8] = bricks, units, split bricks, partial
+ * index, that each brick's units cover its rotation segments exactly once,
+ * and (kinds 0, 2, 3, 4) that the batch builder's buffers for such an item
+ * are as large.  Needs no GPU.  stats[8] = bricks, units, split bricks, partial
  * slots, segment-buffer bytes, halo-buffer bytes, empty units, largest
  * segment count.  TT_ERR_INVALID with the failed condition in tt_last_error.
  * No counterpart in the reference: a diagnostics entry of this library.

@@ -2557,11 +2557,34 @@ int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t 
 
 int engine_plan_check(const SliceGeom &g, int kind, int64_t n_rot, int64_t stats[8]) {
   if (int st = check_engine(g, n_rot)) return st;
+  // the batch builder sizes an item's segment and halo buffers itself: they
+  // must hold what the plan's kernels index
+  auto batch_fits = [&](int bkind, int cfg) -> int {
+    size_t seg = 0, halo = 0;
+    int nb = 0;
+    batch_item_sizes(bkind, cfg, g, (int)stats[3], seg, halo, nb);
+    if (nb != stats[0] || seg < (size_t)stats[4] || halo < (size_t)stats[5])
+      return fail_invalid("brick plan check: the batch builder's buffers are smaller than the plan needs");
+    return TT_OK;
+  };
+  int st = TT_OK;
   switch (kind) {
-    case 0: return plan_check(g, kGBZ, kGBY, kGBX, kGather, n_rot, stats);
+    case 0:
+      if ((st = plan_check(g, kGBZ, kGBY, kGBX, kGather, n_rot, stats))) return st;
+      return batch_fits(TT_BATCH_EXTRACT, 0);
     case 1: return plan_check(g, kABZ, kABY, kABX, kAdjoint, n_rot, stats);
-    case 2: return plan_check(g, kIBZ, kIBY, kIBX, kInsert, n_rot, stats);
-    default: return fail_invalid("plan check: kind must be 0 (gather), 1 (adjoint) or 2 (insertion)");
+    case 2:
+      if ((st = plan_check(g, kIBZ, kIBY, kIBX, kInsert, n_rot, stats))) return st;
+      return batch_fits(TT_BATCH_INSERT, 0);
+    case 3:
+      if ((st = plan_check(g, kGatherCfg[1].bz, kGatherCfg[1].by, kGatherCfg[1].bx, kGather, n_rot, stats))) return st;
+      return batch_fits(TT_BATCH_EXTRACT, 1);
+    case 4:
+      if ((st = plan_check(g, kInsertCfg[1].bz, kInsertCfg[1].by, kInsertCfg[1].bx, kInsert, n_rot, stats))) return st;
+      return batch_fits(TT_BATCH_INSERT, 1);
+    default:
+      return fail_invalid("plan check: kind must be 0 (gather), 1 (adjoint), 2 (insertion), 3 or 4 (the batch's "
+                          "alternative gather and insertion tiles)");
   }
 }
 
