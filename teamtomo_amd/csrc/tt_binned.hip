@@ -52,9 +52,11 @@ constexpr int kBZ = 4, kBY = 8, kBX = 64;
 constexpr int kNV = kBZ * kBY * kBX;
 constexpr int kThreads = 256;       // pixel passes
 constexpr int kBrickThreads = 256;  // brick pass
-// A pixel lands in at most 8 bricks: the record buffer holds 8 per pixel, and
-// the path is used only while that buffer stays below this size.
-constexpr int64_t kMaxRecordBytes = (int64_t)3 << 30;
+// A pixel lands in at most 8 bricks: the record buffer holds 8 per pixel (a
+// call writes about 1.25 per pixel), and the path is used only while that
+// buffer stays below this size; if the allocation fails, the call falls back
+// to the brick engine.
+constexpr int64_t kMaxRecordBytes = (int64_t)8 << 30;
 // Auto selection: at most this many pixels (slices x pixels per slice) per
 // brick of the volume on average.  Measured (insertion / adjoint, ms, brick
 // engine against binned): 61 tilts of 512^2 into 512^3 (196 per brick) 2.30 /
@@ -478,6 +480,10 @@ int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float 
   const size_t bytes = o_recs + (size_t)std::max<int64_t>(cap, 1) * sizeof(BinRec);
   void *base = nullptr;
   if (hipError_t e = hipMallocAsync(&base, bytes, s)) {
+    if (e == hipErrorOutOfMemory) {  // the caller runs the brick engine instead
+      (void)hipGetLastError();
+      return kBinnedNoMemory;
+    }
     set_error(std::string("binned scatter: hipMallocAsync: ") + hipGetErrorString(e));
     return TT_ERR_HIP;
   }

@@ -53,6 +53,9 @@ int launch_mirror(const f2v *plane, const float *wplane, const SliceGeom &g, f2v
 // engine_insert take the binned path for this call.
 int set_scatter_path(int path);
 bool binned_selected(const SliceGeom &g, int64_t n_rot);
+// binned_scatter's result when its record buffer cannot be allocated: nothing
+// was launched, and the caller takes the brick engine
+constexpr int kBinnedNoMemory = -1000;
 int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float *rot, int64_t n_rot, f2v *out,
                    float *wout, int accumulate, hipStream_t s);
 

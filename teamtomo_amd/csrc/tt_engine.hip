@@ -1727,7 +1727,10 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
                 float *wout, int accumulate, hipStream_t s) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   // few pixels per brick (a tilt series into a large volume): the binned scatter
-  if (binned_selected(g, n_rot)) return binned_scatter(MODE == kInsert, g, src, rot, n_rot, out, wout, accumulate, s);
+  if (binned_selected(g, n_rot)) {
+    const int st = binned_scatter(MODE == kInsert, g, src, rot, n_rot, out, wout, accumulate, s);
+    if (st != kBinnedNoMemory) return st;
+  }
   PlanEntry pe;
   if (int st = get_plan(g, BZ, BY, BX, true, n_rot, pe)) return st;
   const BrickGrid bg = grid_of(g, BZ, BY, BX);
