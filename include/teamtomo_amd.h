@@ -75,7 +75,7 @@ int tt_set_gather_path(int path);
 /*
  * Selects the scatter the complex64 adjoint and insertion entry points use
  * (TT_SCATTER_AUTO by default: the binned scatter when the call has at most
- * 1024 slice pixels per 4x8x64 brick of the volume, such as a tilt series
+ * 800 slice pixels per 4x8x64 brick of the volume, such as a tilt series
  * into a large volume, and its record buffer, 8 x 24 bytes per slice pixel,
  * stays within 8 GiB and can be allocated; the brick engine otherwise).  TT_SCATTER_BINNED uses the binned scatter wherever it fits.
  * Both accumulate in fixed point and are bitwise reproducible; they agree to

@@ -270,9 +270,10 @@ __global__ __launch_bounds__(kLineBlock) void line_bwd_kernel(const f2v *__restr
 // decision as the forward kernel) and adds their bilinear contributions with
 // 64-bit LDS atomics in fixed point (scale 2^(30-e) from the exact max|g|;
 // 64-bit integer LDS atomics run far faster than float ones); the flush then
-// adds the tile to the image, with global atomics only on the rows and
-// columns a neighbouring tile also touches (and for tiles split over several
-// units).
+// adds the tile to the image, with 64-bit integer global atomics into a
+// staging buffer only on the rows and columns a neighbouring tile also
+// touches (and for tiles split over several units), converted by one more
+// pass: the gradient does not depend on the order of the atomics.
 // ---------------------------------------------------------------------------
 constexpr int kTY = 32, kTX = 32;  // cells per tile
 constexpr int kTP = (kTY + 1) * (kTX + 1);
@@ -309,6 +310,11 @@ __global__ void line_zero_word_kernel(unsigned *w) {
   if (threadIdx.x == 0) *w = 0u;
 }
 
+__global__ __launch_bounds__(kLineBlock) void line_zero_kernel(u64 *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kLineBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLineBlock)
+    p[i] = 0ull;
+}
+
 // Fixed-point scale 2^(30 - e), 2^e > max|g|: every contribution w g (|w| <= 1)
 // converts to a 32-bit integer, and 2^33 of them fit the 64-bit sums.
 __device__ __forceinline__ float line_scale(const unsigned *maxbits) {
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(kLineBlock) void line_adj_tile_kernel(const f2v *__
                                                                   const float *__restrict__ rot,
                                                                   const LineUnit *__restrict__ units, int ntx,
                                                                   const unsigned *__restrict__ maxbits,
-                                                                  f2v *__restrict__ gimg) {
+                                                                  f2v *__restrict__ gimg, u64 *__restrict__ stage) {
   constexpr int NW = kLineBlock / 64;
   __shared__ u64 acc[2][kTP];
   // per wave: inclusive prefix of the candidate counts, first column, rotation
@@ -429,10 +435,11 @@ __global__ __launch_bounds__(kLineBlock) void line_adj_tile_kernel(const f2v *__
     __builtin_amdgcn_wave_barrier();  // the next batch rewrites this wave's arrays
   }
   __syncthreads();
-  // flush: exclusive pixels by a plain read-modify-write, shared ones
-  // (the tile's first row / column, its far halo row / column, or all of a
-  // split tile) by atomics
-  float *gf = reinterpret_cast<float *>(gimg);
+  // flush: exclusive pixels by a plain read-modify-write, shared ones (the
+  // tile's first row / column, its far halo row / column, or all of a split
+  // tile) as 64-bit integer atomics into the staging sums, which
+  // line_stage_flush_kernel converts: integer sums do not depend on the
+  // order of the atomics, so the gradient is bitwise reproducible
   const float inv = 1.0f / vscale;
   for (int v = tid; v < kTP; v += kLineBlock) {
     const int ly = v / (kTX + 1), lx = v - ly * (kTX + 1);
@@ -443,14 +450,30 @@ __global__ __launch_bounds__(kLineBlock) void line_adj_tile_kernel(const f2v *__
     const int64_t e = (int64_t)(y_lo + ly) * g.WR + (x_lo + lx);
     const bool shared = u.split || ly == 0 || lx == 0 || ly == nyc || lx == nxc;
     if (shared) {
-      unsafeAtomicAdd(gf + 2 * e, ar);
-      unsafeAtomicAdd(gf + 2 * e + 1, ai);
+      atomicAdd(&stage[2 * e], qr);
+      atomicAdd(&stage[2 * e + 1], qi);
     } else {
       f2v cur = gimg[e];
       cur.x += ar;
       cur.y += ai;
       gimg[e] = cur;
     }
+  }
+}
+
+// Adds the staged integer sums of the shared pixels to the image gradient,
+// at the scale of the tile kernel.
+__global__ __launch_bounds__(kLineBlock) void line_stage_flush_kernel(const u64 *__restrict__ stage, int64_t n_px,
+                                                                     const unsigned *__restrict__ maxbits,
+                                                                     f2v *__restrict__ gimg) {
+  const float inv = 1.0f / line_scale(maxbits);
+  for (int64_t e = (int64_t)blockIdx.x * kLineBlock + threadIdx.x; e < n_px; e += (int64_t)gridDim.x * kLineBlock) {
+    const u64 qr = stage[2 * e], qi = stage[2 * e + 1];
+    if (qr == 0ull && qi == 0ull) continue;
+    f2v cur = gimg[e];
+    cur.x += (float)(long long)qr * inv;
+    cur.y += (float)(long long)qi * inv;
+    gimg[e] = cur;
   }
 }
 
@@ -599,18 +622,26 @@ int tt_extract2d_bwd_c64(const void *grad_lines, const void *image, int h, int w
     LineUnit *units = nullptr;
     int n_units = 0, ntx = 0;
     if ((st = line_plan(g, n_rot, &units, &n_units, &ntx))) return st;
-    unsigned *word = nullptr;
-    if (hipError_t e = hipMallocAsync((void **)&word, 256, s)) {
+    // scratch: the scale word, then two 64-bit staging sums per image pixel
+    const int64_t n_px = (int64_t)h * wr;
+    void *scratch = nullptr;
+    if (hipError_t e = hipMallocAsync(&scratch, 256 + (size_t)n_px * 2 * sizeof(u64), s)) {
       set_error(std::string("tt_extract2d_bwd_c64: hipMallocAsync: ") + hipGetErrorString(e));
       return TT_ERR_HIP;
     }
+    unsigned *word = (unsigned *)scratch;
+    u64 *stage = (u64 *)((char *)scratch + 256);
+    const unsigned zgrid = (unsigned)std::min<int64_t>(4096, (2 * n_px + kLineBlock - 1) / kLineBlock);
     hipLaunchKernelGGL(line_zero_word_kernel, dim3(1), dim3(64), 0, s, word);
+    hipLaunchKernelGGL(line_zero_kernel, dim3(zgrid), block, 0, s, stage, 2 * n_px);
     hipLaunchKernelGGL(line_maxabs_kernel, dim3((unsigned)std::min<int64_t>(2048, (total + kLineBlock - 1) / kLineBlock)),
                        block, 0, s, (const f2v *)grad_lines, total, word);
     hipLaunchKernelGGL(line_adj_tile_kernel, dim3((unsigned)n_units), block, 0, s, (const f2v *)grad_lines,
-                       g, rot_yx, (const LineUnit *)units, ntx, (const unsigned *)word, (f2v *)grad_image);
-    st = check_launch("line_adj_tile_kernel");
-    if (hipError_t e = hipFreeAsync(word, s)) {
+                       g, rot_yx, (const LineUnit *)units, ntx, (const unsigned *)word, (f2v *)grad_image, stage);
+    hipLaunchKernelGGL(line_stage_flush_kernel, dim3((unsigned)std::min<int64_t>(4096, (n_px + kLineBlock - 1) / kLineBlock)),
+                       block, 0, s, (const u64 *)stage, n_px, (const unsigned *)word, (f2v *)grad_image);
+    st = check_launch("line_adj_tile_kernel / line_stage_flush_kernel");
+    if (hipError_t e = hipFreeAsync(scratch, s)) {
       if (!st) {
         set_error(std::string("tt_extract2d_bwd_c64: hipFreeAsync: ") + hipGetErrorString(e));
         st = TT_ERR_HIP;

@@ -78,7 +78,8 @@ def test_lines_large(device):
 
 def test_lines_adjoint_large(device):
     # the tiled image gradient (split tiles near DC, shared tile edges) against
-    # the oracle's adjoint, and the dot-product identity with the forward
+    # the oracle's adjoint, the dot-product identity with the forward, and
+    # bitwise reruns
     rng = np.random.default_rng(15)
     Fn = (rng.standard_normal((512, 257)) + 1j * rng.standard_normal((512, 257))).astype(np.complex64)
     R = _rots(3000, 16)
@@ -92,6 +93,12 @@ def test_lines_adjoint_large(device):
     lhs = np.real(np.vdot(Gn.astype(np.complex128), _np(L).astype(np.complex128)))
     rhs = np.real(np.vdot(_np(F.grad).astype(np.complex128), Fn.astype(np.complex128)))
     assert abs(lhs - rhs) < 1e-4 * abs(lhs)
+    # integer sums at the shared tile edges and split tiles: bitwise reruns
+    for _ in range(2):
+        F2 = _t(Fn, device).requires_grad_(True)
+        L2 = tt.extract_central_slices_rfft_2d(F2, (512, 512), _t(R, device))
+        torch.real(torch.sum(torch.conj(_t(Gn, device)) * L2)).backward()
+        assert torch.equal(F2.grad, F.grad)
 
 
 def test_project_2d_to_1d_vs_golden(device):
