@@ -378,6 +378,32 @@ __device__ __forceinline__ void lds_add_wide(u64 *p, float w, float v) {
   atomicAdd(p, (u64)to_fixed64(__fmul_rn(w, v)));
 }
 
+// Split accumulators of the 32-bit contributions.  Each accumulator of the
+// tile is a plane of NT low words followed by a plane of NT high words, and a
+// voxel's 64-bit fixed-point sum is hi 2^32 + lo - kLoBias.  A contribution c
+// (|c| <= 2^30) is added to the low word alone with a returning 32-bit LDS
+// atomic (at random addresses it costs about 60 % of a 64-bit one); the high
+// word changes only when the low word wraps, which needs the low word it
+// replaced outside [2^30, 3 2^30).  Low words start at kLoBias = 2^31, the
+// middle of that window, so wraps are rare: every returned low word is
+// tested, and a lane that saw one outside the window adds the exact carries
+// to the high words.  The sums are integers, so the result does not depend
+// on the order of the atomics, and equals the 64-bit accumulation's.
+constexpr unsigned kLoBias = 0x80000000u;
+
+// bit 31 set: the low word was inside the window (no carry possible)
+__device__ __forceinline__ unsigned lo_window(unsigned old) { return old + 0x40000000u; }
+
+// Change of the high word when c is added to a low word that held old.
+__device__ __forceinline__ unsigned hi_step(unsigned old, int c) {
+  const bool carry = old + (unsigned)c < old;
+  return carry ? (c < 0 ? 0u : 1u) : (c < 0 ? 0xffffffffu : 0u);
+}
+
+__device__ __forceinline__ u64 split_value(unsigned lo, unsigned hi) {
+  return (((u64)hi << 32) | lo) - (u64)kLoBias;
+}
+
 // The exact canonical FP64 cell of pixel (i, j) under rotation r and Friedel
 // half `neg`, for pixels whose FP32 tile coordinate lies within kNear of an
 // integer: snap, faces and the Friedel half decide exactly as in every other
@@ -483,18 +509,32 @@ __device__ __forceinline__ void flush_voxel(const u64 *__restrict__ a, int astri
   }
 }
 
+// acc: NACC planes of 64-bit words, or (split) the first two as planes of
+// low and high words.
 template <int BZ, int BY, int BX, int MODE>
-__device__ void flush_tile(const u64 *__restrict__ acc, const SliceGeom &g, const BrickGrid &bg,
+__device__ void flush_tile(const u64 *__restrict__ acc, bool split, const SliceGeom &g, const BrickGrid &bg,
                            int brick, float vscale, int accumulate, f2v *__restrict__ out,
                            float *__restrict__ wout, f2v *__restrict__ plane,
                            float *__restrict__ wplane, float *__restrict__ halo, int tid, int nthreads,
                            bool write_shell) {
-  constexpr int NT = Cfg<BZ, BY, BX, MODE>::NT;
+  typedef Cfg<BZ, BY, BX, MODE> C;
+  constexpr int NT = C::NT, NA = C::NACC;
   const BrickBox bb = brick_box<BZ, BY, BX>(g, bg, brick);
   const float inv_v = 1.0f / vscale;
-  for (int v = tid; v < NT; v += nthreads)
-    flush_voxel<BZ, BY, BX, MODE>(acc + v, NT, v, bb, g, brick, inv_v, accumulate, out, wout, plane,
-                                  wplane, halo, write_shell);
+  const unsigned *a32 = reinterpret_cast<const unsigned *>(acc);
+  for (int v = tid; v < NT; v += nthreads) {
+    if (split) {
+      u64 a[NA > 0 ? NA : 1];
+#pragma unroll
+      for (int k = 0; k < NA; ++k)
+        a[k] = k < 2 ? split_value(a32[2 * k * NT + v], a32[(2 * k + 1) * NT + v]) : acc[k * NT + v];
+      flush_voxel<BZ, BY, BX, MODE>(a, 1, v, bb, g, brick, inv_v, accumulate, out, wout, plane, wplane, halo,
+                                    write_shell);
+    } else {
+      flush_voxel<BZ, BY, BX, MODE>(acc + v, NT, v, bb, g, brick, inv_v, accumulate, out, wout, plane,
+                                    wplane, halo, write_shell);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -512,6 +552,9 @@ __device__ __forceinline__ void brick_body(
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
+  // 32-bit contributions of the value (re, im) go to split accumulators;
+  // the insertion's weights stay 64-bit words
+  constexpr bool SPLIT = SCATTER;
   static_assert(CH >= 1 && CH <= 4, "the chunk list holds pixel counts 1 to 4");
   static_assert(NCH == 1 || (NCH == 2 && MODE == kGather), "two channels: gather only");
   // gather: NCH x NT complex voxels; scatter: NACC fixed-point planes of NT words
@@ -564,8 +607,11 @@ __device__ __forceinline__ void brick_body(
         if (gz < g.vD && gy < g.vH && gx < g.vW) val = src[ch * g.vstride + ((int64_t)gz * g.vH + gy) * g.vW + gx];
         tv[ch * NT + v] = val;
       }
-  } else {
+  } else if (wide || !SPLIT) {
     for (int v = tid; v < NWORDS; v += C::T) s_tile[v] = 0ull;
+  } else {  // split accumulators (re, im): low words kLoBias, high words 0; weights 0
+    unsigned *t32 = reinterpret_cast<unsigned *>(s_tile);
+    for (int v = tid; v < 2 * NWORDS; v += C::T) t32[v] = (v < 4 * NT && !((v / NT) & 1)) ? kLoBias : 0u;
   }
   __syncthreads();
 
@@ -976,7 +1022,7 @@ __device__ __forceinline__ void brick_body(
                       if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
                     }
                   }
-              } else {
+              } else if (!SPLIT) {
 #pragma unroll
                 for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
@@ -991,6 +1037,52 @@ __device__ __forceinline__ void brick_body(
                       if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
                     }
                   }
+              } else {
+                // split accumulators (re, im): low words first, then the rare carries
+                unsigned *t32 = reinterpret_cast<unsigned *>(s_tile);
+                unsigned old[8][2];
+#pragma unroll
+                for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                  for (int dy = 0; dy < 2; ++dy) {
+                    const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                      const int k = dz * 4 + dy * 2 + dx;
+                      const float w = wzy * wx[dx];
+                      const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                      old[k][0] = atomicAdd(&t32[o], (unsigned)to_fixed32(w, v.x));
+                      old[k][1] = atomicAdd(&t32[2 * NT + o], (unsigned)to_fixed32(w, v.y));
+                      if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                    }
+                  }
+                unsigned inside = 0xffffffffu;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) inside &= lo_window(old[k][0]) & lo_window(old[k][1]);
+                if (!(inside >> 31)) {
+                  // the contributions again, from the same operands (the
+                  // empty asm keeps the compiler from holding the first
+                  // products in registers across the atomics)
+                  float rx = v.x, ry = v.y, rz = tz, rw = ty, rt = tx;
+                  asm volatile("" : "+v"(rx), "+v"(ry), "+v"(rz), "+v"(rw), "+v"(rt));
+                  const float uz[2] = {1.0f - rz, rz}, uy[2] = {1.0f - rw, rw}, ux[2] = {1.0f - rt, rt};
+#pragma unroll
+                  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                    for (int dy = 0; dy < 2; ++dy) {
+                      const float wzy = uz[dz] * uy[dy];
+#pragma unroll
+                      for (int dx = 0; dx < 2; ++dx) {
+                        const int k = dz * 4 + dy * 2 + dx;
+                        const float w = wzy * ux[dx];
+                        const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                        const unsigned h0 = hi_step(old[k][0], to_fixed32(w, rx));
+                        const unsigned h1 = hi_step(old[k][1], to_fixed32(w, ry));
+                        if (h0) atomicAdd(&t32[NT + o], h0);
+                        if (h1) atomicAdd(&t32[3 * NT + o], h1);
+                      }
+                    }
+                }
               }
             }
           }
@@ -1007,7 +1099,11 @@ __device__ __forceinline__ void brick_body(
   const bool any = __syncthreads_or((int)had_items) != 0;
   if (nseg > 1) {  // partial sums of a rotation segment: combined by split_combine_kernel
     u64 *dst = segbuf + ((int64_t)un.slot + seg) * (C::NACC * NT);
-    for (int v = tid; v < C::NACC * NT; v += C::T) dst[v] = s_tile[v];
+    const unsigned *t32 = reinterpret_cast<const unsigned *>(s_tile);
+    for (int v = tid; v < C::NACC * NT; v += C::T) {
+      const int a = v / NT, u = v - a * NT;
+      dst[v] = (wide || !SPLIT || a >= 2) ? s_tile[v] : split_value(t32[2 * a * NT + u], t32[(2 * a + 1) * NT + u]);
+    }
     return;
   }
   // touched[brick]: whether any slice item reached the tile.  An untouched
@@ -1017,7 +1113,7 @@ __device__ __forceinline__ void brick_body(
     if (tid == 0) touched[brick] = any ? 1 : 0;
     if (!any && accumulate) return;
   }
-  flush_tile<BZ, BY, BX, MODE>(s_tile, g, bg, brick, vscale, accumulate, out, wout, plane, wplane,
+  flush_tile<BZ, BY, BX, MODE>(s_tile, SPLIT && !wide, g, bg, brick, vscale, accumulate, out, wout, plane, wplane,
                                halo, tid, C::T, any || !touched);
 }
 

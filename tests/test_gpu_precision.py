@@ -144,3 +144,43 @@ def _spike_case(device, n_rot, C):
     # about 1e-3, per contribution
     assert e_far < 1e-2 and e_afar < 1e-2
     return e_far, e_afar
+
+
+@pytest.mark.parametrize("signs", ["positive", "negative", "mixed"])
+def test_split_accumulator_carries(device, signs):
+    """1024 copies of one rotation: every voxel near its plane sums about a
+    thousand contributions.  The sampled slices (every 16th) hold values of
+    magnitude about 1 and the others values about 200 times larger (under the
+    2^8 headroom above the sampled band maxima, so there is no redo), so a
+    32-bit contribution is about 2^28.6 and the low words of the brick
+    engine's split accumulators wrap many times in every unit: carries for
+    positive values, borrows for negative ones, both for mixed signs.  A
+    wrong carry would move a voxel by 2^32 quanta (about 2048).  Insertion and
+    adjoint must match the FP64 oracle."""
+    from conftest import rel_l2
+    from oracle import c_oracle as C
+
+    n, k = 64, 1024
+    R = uniform_rotations(1, 60).expand(k, 3, 3).contiguous()
+    g = torch.Generator().manual_seed(61)
+    sign = {"positive": torch.ones(k), "negative": -torch.ones(k),
+            "mixed": torch.randint(0, 2, (k,), generator=g).float() * 2.0 - 1.0}[signs]
+    mag = torch.full((k,), 200.0)
+    mag[:: k // 64] = 1.0  # the slices the band maxima sample
+    jitter = 1.0 + 0.1 * torch.rand((k, n, n // 2 + 1), generator=g)
+    S = torch.complex(jitter, 0.5 * jitter) * (sign * mag).view(k, 1, 1)
+    Sd, Rd = S.to(device), R.to(device)
+    tt.set_scatter_path("brick")
+    try:
+        V, W = tt.insert_central_slices_rfft_3d(Sd, (n,) * 3, Rd)
+        vol = torch.zeros((n, n, n // 2 + 1), dtype=torch.complex64, device=device, requires_grad=True)
+        tt.extract_central_slices_rfft_3d(vol, (n,) * 3, Rd).backward(Sd)
+    finally:
+        tt.set_scatter_path("auto")
+    rV, rW = C.insert(S.numpy(), (n,) * 3, O.prepare_rotations(R.numpy(), False, None))
+    ref = C.extract_adjoint(S.numpy(), tuple(vol.shape), (n,) * 3, O.prepare_rotations(R.numpy(), False, 1e-8))
+    e_v, e_w = rel_l2(V.cpu().numpy(), rV), rel_l2(W.cpu().numpy(), rW)
+    e_a = rel_l2(vol.grad.cpu().numpy(), ref)
+    print(f"{signs}: insertion volume {e_v:.2e}, weights {e_w:.2e}, adjoint {e_a:.2e}; "
+          f"largest |V| {np.abs(rV).max():.3e}")
+    assert e_v < 1e-6 and e_w < 1e-6 and e_a < 1e-6
