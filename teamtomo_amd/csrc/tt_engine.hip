@@ -831,7 +831,12 @@ __device__ __forceinline__ void brick_body(
     const int ia = wave_incl_scan(lane < nitems ? (items[lane].rows >> 16) : 0);
     const int total_rows = __builtin_amdgcn_readlane(ia, 63);
 
-    for (int rb = 0; rb < total_rows; rb += 64) {
+    // row passes of 64 rows from row rb, whose first cskip chunks an earlier
+    // pass processed: a pass takes whole 64-lane steps of chunks while rows
+    // remain after it, and the next pass starts at the row of the first chunk
+    // left, so that chunk steps run with every lane busy except at the end
+    int rb = 0, cskip = 0;
+    while (rb < total_rows) {
       // item of each lane's row: the few items ending inside this pass are
       // walked with wave-uniform reads of the prefix
       const int t = rb + lane;
@@ -876,23 +881,29 @@ __device__ __forceinline__ void brick_body(
           if (MODE == kInsert && i < g.half_H) jl = max(jl, 1);  // x = 0, y < 0 is skipped
           len = max(0, jh - jl + 1);
         }
+        if (lane == 0) {  // chunks of the first row processed by the previous pass
+          jl += cskip * CH;
+          len = max(0, len - cskip * CH);
+        }
       }
       const int nch = (len + CH - 1) / CH;
       const int cincl = wave_incl_scan(nch);
       const int cexcl = cincl - nch;
-      const int total_ch = __shfl(cincl, 63, 64);
+      const int total_ch = __builtin_amdgcn_readfirstlane(__shfl(cincl, 63, 64));
+      // (the gather measured no faster with partial steps carried over)
+      const int done = (!SCATTER || rb + 64 >= total_rows || total_ch < 64) ? total_ch : (total_ch & ~63);
 
       // (3) the rows' chunks of CH pixels are listed (windows of kSegCap),
       // then taken one per lane, consecutive lanes on consecutive chunks
-      for (int w0 = 0; w0 < total_ch; w0 += kSegCap) {
-        const int c0 = max(cexcl, w0), c1 = min(cexcl + nch, w0 + kSegCap);
+      for (int w0 = 0; w0 < done; w0 += kSegCap) {
+        const int c0 = max(cexcl, w0), c1 = min(cexcl + nch, min(w0 + kSegCap, done));
         for (int c = c0; c < c1; ++c) {
           const int q0 = (c - cexcl) * CH;
           segs[c - w0] = (unsigned)k | ((unsigned)lane << 6) | ((unsigned)(min(CH, len - q0) - 1) << 12) |
                          ((unsigned)(jl + q0) << 14);
         }
         __builtin_amdgcn_wave_barrier();
-        const int wn = min(total_ch - w0, kSegCap);
+        const int wn = min(done - w0, kSegCap);
         for (int pc = 0; pc < wn; pc += 64) {
           const bool live = pc + lane < wn;
           const unsigned sgv = segs[live ? pc + lane : 0];
@@ -1088,6 +1099,14 @@ __device__ __forceinline__ void brick_body(
           }
         }
         __builtin_amdgcn_wave_barrier();  // the next window rewrites the list
+      }
+      if (done < total_ch) {  // the next pass starts at the row holding chunk `done`
+        const int kr = __popcll(__ballot(cexcl + nch <= done));
+        cskip = __builtin_amdgcn_readfirstlane(done - __shfl(cexcl, kr, 64) + (kr == 0 ? cskip : 0));
+        rb = __builtin_amdgcn_readfirstlane(rb + kr);
+      } else {
+        rb += 64;
+        cskip = 0;
       }
       __builtin_amdgcn_wave_barrier();  // the next row pass rewrites this wave's row table
     }
