@@ -9,22 +9,26 @@
 //           copied into LDS once and every slice crossing the brick is served
 //           from there; pixels outside the volume or the fftfreq_max disc are
 //           written by a row-interval zero pass that runs first.
-//  scatter: the brick accumulates in LDS with ds_add_f32 (8 corners per
-//           pixel, the upper corners land in the halo), then adds its voxels
-//           to the output with global float atomics.
+//  scatter: the brick accumulates in LDS in fixed point with integer atomics
+//           (8 corners per pixel, the upper corners land in a halo shell),
+//           then stores its own voxels and passes the shell to the
+//           neighbouring bricks through a halo buffer.
 //
 // Inside a workgroup the waves are independent: each takes a contiguous share
 // of the unit's rotations and runs
 //  (1) discovery: one lane per rotation tests the slice plane against the
-//      brick's coordinate box and records, for each Friedel half that reaches
-//      it, the range of slice rows (item);
-//  (2) rows: one lane per row evaluates the canonical row constants and the
-//      interval of columns whose coordinates can lie in the box (FP64 slab
-//      constraints, widened by 1e-6 voxel);
-//  (3) pixels: the rows' columns are cut into chunks of CH pixels; a wave
-//      prefix sum hands one chunk to each lane, consecutive lanes taking
-//      consecutive chunks of a row.  Each pixel evaluates its canonical FP64
-//      coordinate and keeps it only if its cell is in the brick.
+//      brick's box; per crossing rotation and Friedel half, an item holds the
+//      range of candidate slice rows and an FP32 frame referred to the tile
+//      centre;
+//  (2) rows: one lane per row evaluates the row's FP32 frame coordinate and
+//      the interval of columns whose coordinates can lie in the box, and
+//      writes the row's record;
+//  (3) chunks: the rows' columns are cut into chunks of CH pixels, handed out
+//      64 at a time, consecutive lanes on consecutive chunks of a row (a lane
+//      finds its row by a running maximum over marks that the rows write at
+//      their first chunk).  A pixel keeps its FP32 cell if no coordinate lies
+//      near an integer, else evaluates its canonical FP64 cell, and is taken
+//      only if that cell is in the brick.
 // No workgroup barrier is taken between the tile load (or clear) and the
 // flush, so the 8 waves of a block overlap their latencies freely.
 //
@@ -150,13 +154,27 @@ __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// Inclusive prefix sum over the wave with DPP lane moves (row_shr 1, 2, 4, 8
+// inside rows of 16 lanes, then row_bcast 15 and 31 across rows): six VALU
+// operations and no LDS traffic.  Call with every lane active.
 __device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// Inclusive running maximum of unsigned values over the wave (same lane moves).
+__device__ __forceinline__ unsigned wave_incl_max(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
   return v;
 }
 
@@ -231,7 +249,6 @@ __device__ __forceinline__ int shell_index(int lz, int ly, int lx) {
   return SY * SX + BZ * SX + lz * BY + ly;
 }
 
-constexpr int kSegCap = 256;    // chunks listed per window
 
 // A slice's item in a brick: a rotation and Friedel half whose rows can reach
 // the brick, with its FP32 frame.  The coordinate of pixel (i, j) relative to
@@ -250,7 +267,19 @@ struct __align__(16) ItemRec {
   unsigned rnt;  // rotation | Friedel half (s = -1) << 30 | tie << 31
 };
 
-constexpr int kTake = 32;              // rotations per item pass (two halves each)
+// A row of a row pass, written once by the row's lane and read by each of its
+// chunks: the row's FP32 frame coordinate at column jref (rowc, the same FMA
+// the row's interval test evaluates), the column step, and where its
+// candidate columns start.
+struct __align__(8) RowRec {
+  float rowc[3];  // frame coordinate of (i, jref)
+  float sv[3];    // step per slice column
+  float djl;      // jl - jref: the first candidate column relative to jref
+  unsigned ijl;   // slice row i | first candidate column jl << 16
+  int len;        // candidate columns from jl
+  unsigned rnt;   // rotation | Friedel half (s = -1) << 30 | tie << 31
+};
+
 constexpr float kRowMargin = 1e-4f;    // widening of the row phase's tile box (voxels)
 
 // Fixed-point scale of the scatter, one per brick: values are multiplied by
@@ -557,6 +586,9 @@ __device__ __forceinline__ void brick_body(
   constexpr bool SPLIT = SCATTER;
   static_assert(CH >= 1 && CH <= 4, "the chunk list holds pixel counts 1 to 4");
   static_assert(NCH == 1 || (NCH == 2 && MODE == kGather), "two channels: gather only");
+  // rotations per item pass (two halves each): 32 where the LDS of two
+  // blocks per CU has room for 64 item records per wave, else 16
+  constexpr int kTake = (MODE == kGather && NCH == 1) ? 32 : 16;
   // gather: NCH x NT complex voxels; scatter: NACC fixed-point planes of NT words
   constexpr int NWORDS = SCATTER ? C::NACC * NT : NCH * NT;
   // one LDS block, the tile first: the trilinear corner offsets then fit the
@@ -565,9 +597,10 @@ __device__ __forceinline__ void brick_body(
     u64 tile[NWORDS];
     ItemRec item[NW][2 * kTake];
     int cross[NW][128];  // rotations whose plane meets the brick, pending
-    // chunks: item | row of the pass << 6 | (pixel count - 1) << 12 | first column << 14
-    unsigned seg[NW][kSegCap];
-    int rowi[NW][64];  // slice row i of each row of the pass
+    RowRec rec[NW][64];  // the rows of the current row pass
+    // chunk step marks: row lane << 26 | first chunk of that row, at the
+    // step slot where the row's chunks start
+    unsigned mark[NW][64];
   };
   __shared__ Smem sm;
   u64 *s_tile = sm.tile;
@@ -641,8 +674,8 @@ __device__ __forceinline__ void brick_body(
 
   ItemRec *items = sm.item[wave];
   int *cross = sm.cross[wave];
-  unsigned *segs = sm.seg[wave];
-  int *rowi = sm.rowi[wave];
+  RowRec *recs = sm.rec[wave];
+  unsigned *marks = sm.mark[wave];
   const bool have_cells = nseg_raw > 0 && ch0 >= lo0 && ch1 >= lo1 && ch2 >= lo2;
   const double cb0 = uniform_d(ctr[0] - beta[0]), cb1 = uniform_d(ctr[1] - beta[1]),
                cb2 = uniform_d(ctr[2] - beta[2]);
@@ -716,7 +749,7 @@ __device__ __forceinline__ void brick_body(
     const bool ineg = lane >= kTake;
     bool h = false;
     ItemRec it;
-    if (cidx < take) {
+    if (cidx < take && lane < 2 * kTake) {
       const int r = cross[cidx];
       const PlaneRec &p = pre[r];
       const double s = ineg ? -1.0 : 1.0;
@@ -831,10 +864,15 @@ __device__ __forceinline__ void brick_body(
     const int ia = wave_incl_scan(lane < nitems ? (items[lane].rows >> 16) : 0);
     const int total_rows = __builtin_amdgcn_readlane(ia, 63);
 
-    // row passes of 64 rows from row rb, whose first cskip chunks an earlier
-    // pass processed: a pass takes whole 64-lane steps of chunks while rows
-    // remain after it, and the next pass starts at the row of the first chunk
-    // left, so that chunk steps run with every lane busy except at the end
+    // Row passes of 64 rows from row rb, whose first cskip chunks an earlier
+    // pass processed.  Each row's lane writes the row's record; the chunks of
+    // CH pixels of all rows are then taken 64 at a time, consecutive lanes on
+    // consecutive chunks.  A chunk finds its row from marks: the lane of each
+    // row whose chunks start inside the step writes its index at that slot,
+    // and a running maximum over the wave carries it to the following slots.
+    // A pass takes whole 64-lane steps of chunks while rows remain after it
+    // (scatters), and the next pass starts at the row of the first chunk
+    // left, so that chunk steps run with every lane busy except at the end.
     int rb = 0, cskip = 0;
     while (rb < total_rows) {
       // item of each lane's row: the few items ending inside this pass are
@@ -852,27 +890,28 @@ __device__ __forceinline__ void brick_body(
         }
       }
       // one lane per row: the candidate columns, from the item's FP32 frame
-      int len = 0, jl = 0;
+      int len = 0;
       if (t < total_rows) {
         const ItemRec &ir = items[k];
         const int i = (ir.rows & 0xffff) + (t - excl);
-        rowi[lane] = i;
         const int iref = (int)(ir.ref & 0xffffu), jref = (int)(ir.ref >> 16);
         const float fdm = (float)(i - iref);
         float lo = (float)(-jref), hi = (float)(g.Wr - 1 - jref);
+        float C[3], sv[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const float C = __fmaf_rn(fdm, ir.sa[d], ir.st[d]);  // at column jref
-          const float sv = ir.sv[d];
-          if (fabsf(sv) > 1e-20f) {
-            const float iv = __builtin_amdgcn_rcpf(sv);
-            const float t1 = (Lt[d] - C) * iv, t2 = (Ut[d] - C) * iv;
+          C[d] = __fmaf_rn(fdm, ir.sa[d], ir.st[d]);  // at column jref
+          sv[d] = ir.sv[d];
+          if (fabsf(sv[d]) > 1e-20f) {
+            const float iv = __builtin_amdgcn_rcpf(sv[d]);
+            const float t1 = (Lt[d] - C[d]) * iv, t2 = (Ut[d] - C[d]) * iv;
             lo = fmaxf(lo, fminf(t1, t2));
             hi = fminf(hi, fmaxf(t1, t2));
-          } else if (C < Lt[d] || C > Ut[d]) {
+          } else if (C[d] < Lt[d] || C[d] > Ut[d]) {
             hi = -1.0e9f;
           }
         }
+        int jl = 0;
         if (lo <= hi) {
           jl = jref + (int)ceilf(lo - 1e-3f - 1e-6f * fabsf(lo));
           int jh = jref + (int)floorf(hi + 1e-3f + 1e-6f * fabsf(hi));
@@ -885,224 +924,238 @@ __device__ __forceinline__ void brick_body(
           jl += cskip * CH;
           len = max(0, len - cskip * CH);
         }
+        if (len > 0) {
+          RowRec rec;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            rec.rowc[d] = C[d];
+            rec.sv[d] = sv[d];
+          }
+          rec.djl = (float)(jl - jref);
+          rec.ijl = (unsigned)i | ((unsigned)jl << 16);
+          rec.len = len;
+          rec.rnt = ir.rnt;
+          recs[lane] = rec;
+        }
       }
       const int nch = (len + CH - 1) / CH;
       const int cincl = wave_incl_scan(nch);
       const int cexcl = cincl - nch;
-      const int total_ch = __builtin_amdgcn_readfirstlane(__shfl(cincl, 63, 64));
+      const int total_ch = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(cincl, 63));
       // (the gather measured no faster with partial steps carried over)
       const int done = (!SCATTER || rb + 64 >= total_rows || total_ch < 64) ? total_ch : (total_ch & ~63);
 
-      // (3) the rows' chunks of CH pixels are listed (windows of kSegCap),
-      // then taken one per lane, consecutive lanes on consecutive chunks
-      for (int w0 = 0; w0 < done; w0 += kSegCap) {
-        const int c0 = max(cexcl, w0), c1 = min(cexcl + nch, min(w0 + kSegCap, done));
-        for (int c = c0; c < c1; ++c) {
-          const int q0 = (c - cexcl) * CH;
-          segs[c - w0] = (unsigned)k | ((unsigned)lane << 6) | ((unsigned)(min(CH, len - q0) - 1) << 12) |
-                         ((unsigned)(jl + q0) << 14);
-        }
+      // (3) chunk steps
+      for (int cb = 0; cb < done; cb += 64) {
+        // the row holding chunk cb (every chunk index below total_ch has one)
+        const unsigned long long hb = __ballot(cexcl <= cb && cb < cincl);
+        const int k0 = (int)__builtin_ctzll(hb);
+        const int cx0 = __builtin_amdgcn_readlane(cexcl, k0);
+        __builtin_amdgcn_wave_barrier();  // the previous step's marks have been read
+        marks[lane] = lane == 0 ? ((unsigned)k0 << 26) | (unsigned)cx0 : 0u;
+        if (nch > 0 && cexcl > cb && cexcl < cb + 64) marks[cexcl - cb] = ((unsigned)lane << 26) | (unsigned)cexcl;
         __builtin_amdgcn_wave_barrier();
-        const int wn = min(done - w0, kSegCap);
-        for (int pc = 0; pc < wn; pc += 64) {
-          const bool live = pc + lane < wn;
-          const unsigned sgv = segs[live ? pc + lane : 0];
-          const int ki = sgv & 63, cnt = live ? (int)((sgv >> 12) & 3) + 1 : 0;
-          const int j0 = (int)(sgv >> 14);
-          const ItemRec &ir = items[ki];
-          const int i = rowi[(sgv >> 6) & 63];
-          const int iref = (int)(ir.ref & 0xffffu), jref = (int)(ir.ref >> 16);
-          const int rr = (int)(ir.rnt & 0x3fffffffu);
-          const bool neg = (ir.rnt >> 30) & 1;
-          // flat index of the chunk's first pixel (i Wr + j0 < H Wr < 2^31);
-          // scatter input rotation stride: g.ostride (P, or C P for one
-          // channel of a multichannel stack)
-          const int64_t base = (int64_t)((u64)(unsigned)rr * (u64)(SCATTER ? g.ostride : (int64_t)g.P) +
-                                         (u64)(unsigned)(i * g.Wr + j0));
-          // scatter: the chunk's values are loaded first, so that the load
-          // overlaps the coordinate arithmetic (pixels q < cnt are in the row)
-          f2v vv[CH];
-          if (SCATTER) {
+        const unsigned mk = wave_incl_max(marks[lane]);
+        const int kr = (int)(mk >> 26);
+        const int gc = cb + lane;
+        const bool live = gc < done;
+        const RowRec &row = recs[kr];
+        // first pixel of the chunk in its row (lanes past the last chunk take
+        // the row's first, so that every address below stays inside the row)
+        const int q0 = live ? (gc - (int)(mk & 0x3ffffffu)) * CH : 0;
+        const int cnt = live ? min(CH, row.len - q0) : 0;
+        const int i = (int)(row.ijl & 0xffffu);
+        const int j0 = (int)(row.ijl >> 16) + q0;
+        const int rr = (int)(row.rnt & 0x3fffffffu);
+        const bool neg = (row.rnt >> 30) & 1;
+        // flat index of the chunk's first pixel (i Wr + j0 < H Wr < 2^31);
+        // scatter input rotation stride: g.ostride (P, or C P for one
+        // channel of a multichannel stack)
+        const int64_t base = (int64_t)((u64)(unsigned)rr * (u64)(SCATTER ? g.ostride : (int64_t)g.P) +
+                                       (u64)(unsigned)(i * g.Wr + j0));
+        // scatter: the chunk's values are loaded first, so that the load
+        // overlaps the coordinate arithmetic (pixels q < cnt are in the row)
+        f2v vv[CH];
+        if (SCATTER) {
 #pragma unroll
-            for (int q = 0; q < CH; ++q) vv[q] = src[q < cnt ? base + q : base];
-          }
-          // FP32 frame coordinates of the chunk's pixels: cell corner fl (the
-          // cell relative to the tile centre) and fraction t
-          const float fdm = (float)(i - iref);
-          float rowc[3], sv3[3];
+          for (int q = 0; q < CH; ++q) vv[q] = src[q < cnt ? base + q : base];
+        }
+        // FP32 frame coordinates of the chunk's pixels: cell corner fl (the
+        // cell relative to the tile centre) and fraction t; the column
+        // offset from jref is an integer, exact in FP32
+        float rowc[3], sv3[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          rowc[d] = row.rowc[d];
+          sv3[d] = row.sv[d];
+        }
+        const float djq = row.djl + (float)q0;
+        float fl[CH][3], t3[CH][3];
+        float tmin = 1.0f, tmax = 0.0f;
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const float qf = djq + (float)q;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            rowc[d] = __fmaf_rn(fdm, ir.sa[d], ir.st[d]);
-            sv3[d] = ir.sv[d];
+            const float l = __fmaf_rn(qf, sv3[d], rowc[d]);
+            fl[q][d] = floorf(l);
+            t3[q][d] = l - fl[q][d];
           }
-          float fl[CH][3], t3[CH][3];
-          float tmin = 1.0f, tmax = 0.0f;
+          tmin = fminf(tmin, fminf(fminf(t3[q][0], t3[q][1]), t3[q][2]));
+          tmax = fmaxf(tmax, fmaxf(fmaxf(t3[q][0], t3[q][1]), t3[q][2]));
+        }
+        bool own[CH];
 #pragma unroll
-          for (int q = 0; q < CH; ++q) {
-            const float qf = (float)(j0 + q - jref);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              const float l = __fmaf_rn(qf, sv3[d], rowc[d]);
-              fl[q][d] = floorf(l);
-              t3[q][d] = l - fl[q][d];
-            }
-            tmin = fminf(tmin, fminf(fminf(t3[q][0], t3[q][1]), t3[q][2]));
-            tmax = fmaxf(tmax, fmaxf(fmaxf(t3[q][0], t3[q][1]), t3[q][2]));
-          }
-          bool own[CH];
-#pragma unroll
-          for (int q = 0; q < CH; ++q) own[q] = q < cnt;
-          const bool anynear = cnt > 0 && (tmin < kNear || tmax > 1.0f - kNear);
-          if (__ballot(anynear)) {  // rare, wave-uniform: exact FP64 cells near integers
-            if (anynear) {
-#pragma unroll
-              for (int q = 0; q < CH; ++q) {
-                if (!own[q]) continue;
-                int cell[3];
-                own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (ir.rnt >> 31) & 1, i, j0 + q, lo0,
-                                                     lo1, lo2, cell, t3[q]);
-                fl[q][0] = (float)(cell[0] - HZ);
-                fl[q][1] = (float)(cell[1] - HY);
-                fl[q][2] = (float)(cell[2] - HX);
-              }
-            }
-          }
-          // owned: the cell lies in the brick; e = its tile index
-          int e[CH];
-#pragma unroll
-          for (int q = 0; q < CH; ++q) {
-            own[q] = own[q] & (__builtin_amdgcn_fmed3f(fl[q][0], (float)-HZ, fext0) == fl[q][0]) &
-                     (__builtin_amdgcn_fmed3f(fl[q][1], (float)-HY, fext1) == fl[q][1]) &
-                     (__builtin_amdgcn_fmed3f(fl[q][2], (float)-HX, fext2) == fl[q][2]);
-            if (MASK) own[q] = own[q] && in_disc(g, i, j0 + q);
-            const float ef = __fmaf_rn(fl[q][0], (float)(SY * SX), __fmaf_rn(fl[q][1], (float)SX, fl[q][2]));
-            e[q] = own[q] ? (int)ef + (HZ * SY * SX + HY * SX + HX) : 0;
-          }
-          if (MODE == kGather) {
-            // channel ch of the output: rotation stride g.ostride, channel stride g.P
-            int64_t obase = base;
-            if (NCH > 1)
-              obase = (int64_t)((u64)(unsigned)rr * (u64)g.ostride + (u64)(unsigned)(i * g.Wr + j0));
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) {
-              const f2v *tv = reinterpret_cast<const f2v *>(s_tile) + ch * NT;
-              f2v cv[CH][8];
-#pragma unroll
-              for (int q = 0; q < CH; ++q) {
-                const f2v *b = tv + e[q];
-                cv[q][0] = b[0];
-                cv[q][1] = b[1];
-                cv[q][2] = b[SX];
-                cv[q][3] = b[SX + 1];
-                cv[q][4] = b[SY * SX];
-                cv[q][5] = b[SY * SX + 1];
-                cv[q][6] = b[SY * SX + SX];
-                cv[q][7] = b[SY * SX + SX + 1];
-              }
-#pragma unroll
-              for (int q = 0; q < CH; ++q) {
-                f2v val = trilerp(cv[q][0], cv[q][1], cv[q][2], cv[q][3], cv[q][4], cv[q][5], cv[q][6],
-                                  cv[q][7], t3[q][0], t3[q][1], t3[q][2]);
-                if (neg) val.y = -val.y;
-                if (own[q]) out[obase + ch * g.P + q] = val;
-              }
-            }
-          } else {
+        for (int q = 0; q < CH; ++q) own[q] = q < cnt;
+        const bool anynear = cnt > 0 && (tmin < kNear || tmax > 1.0f - kNear);
+        if (__ballot(anynear)) {  // rare, wave-uniform: exact FP64 cells near integers
+          if (anynear) {
 #pragma unroll
             for (int q = 0; q < CH; ++q) {
               if (!own[q]) continue;
-              f2v v = vv[q];
-              if (neg) v.y = -v.y;
-              bad = bad || !(fabsf(v.x) < vlimit && fabsf(v.y) < vlimit);
-              v.x *= vscale;
-              v.y *= vscale;
-              const float tz = t3[q][0], ty = t3[q][1], tx = t3[q][2];
-              const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
-              if (wide) {
+              int cell[3];
+              own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (row.rnt >> 31) & 1, i, j0 + q, lo0,
+                                                   lo1, lo2, cell, t3[q]);
+              fl[q][0] = (float)(cell[0] - HZ);
+              fl[q][1] = (float)(cell[1] - HY);
+              fl[q][2] = (float)(cell[2] - HX);
+            }
+          }
+        }
+        // owned: the cell lies in the brick; e = its tile index
+        int e[CH];
 #pragma unroll
-                for (int dz = 0; dz < 2; ++dz)
+        for (int q = 0; q < CH; ++q) {
+          own[q] = own[q] & (__builtin_amdgcn_fmed3f(fl[q][0], (float)-HZ, fext0) == fl[q][0]) &
+                   (__builtin_amdgcn_fmed3f(fl[q][1], (float)-HY, fext1) == fl[q][1]) &
+                   (__builtin_amdgcn_fmed3f(fl[q][2], (float)-HX, fext2) == fl[q][2]);
+          if (MASK) own[q] = own[q] && in_disc(g, i, j0 + q);
+          const float ef = __fmaf_rn(fl[q][0], (float)(SY * SX), __fmaf_rn(fl[q][1], (float)SX, fl[q][2]));
+          e[q] = own[q] ? (int)ef + (HZ * SY * SX + HY * SX + HX) : 0;
+        }
+        if (MODE == kGather) {
+          // channel ch of the output: rotation stride g.ostride, channel stride g.P
+          int64_t obase = base;
+          if (NCH > 1)
+            obase = (int64_t)((u64)(unsigned)rr * (u64)g.ostride + (u64)(unsigned)(i * g.Wr + j0));
 #pragma unroll
-                  for (int dy = 0; dy < 2; ++dy) {
-                    const float wzy = wz[dz] * wy[dy];
+          for (int ch = 0; ch < NCH; ++ch) {
+            const f2v *tv = reinterpret_cast<const f2v *>(s_tile) + ch * NT;
+            f2v cv[CH][8];
 #pragma unroll
-                    for (int dx = 0; dx < 2; ++dx) {
-                      const float w = wzy * wx[dx];
-                      const int o = e[q] + dz * SY * SX + dy * SX + dx;
-                      lds_add_wide(&s_tile[o], w, v.x);
-                      lds_add_wide(&s_tile[NT + o], w, v.y);
-                      if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
-                    }
+            for (int q = 0; q < CH; ++q) {
+              const f2v *b = tv + e[q];
+              cv[q][0] = b[0];
+              cv[q][1] = b[1];
+              cv[q][2] = b[SX];
+              cv[q][3] = b[SX + 1];
+              cv[q][4] = b[SY * SX];
+              cv[q][5] = b[SY * SX + 1];
+              cv[q][6] = b[SY * SX + SX];
+              cv[q][7] = b[SY * SX + SX + 1];
+            }
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+              f2v val = trilerp(cv[q][0], cv[q][1], cv[q][2], cv[q][3], cv[q][4], cv[q][5], cv[q][6],
+                                cv[q][7], t3[q][0], t3[q][1], t3[q][2]);
+              if (neg) val.y = -val.y;
+              if (own[q]) out[obase + ch * g.P + q] = val;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < CH; ++q) {
+            if (!own[q]) continue;
+            f2v v = vv[q];
+            if (neg) v.y = -v.y;
+            bad = bad || !(fabsf(v.x) < vlimit && fabsf(v.y) < vlimit);
+            v.x *= vscale;
+            v.y *= vscale;
+            const float tz = t3[q][0], ty = t3[q][1], tx = t3[q][2];
+            const float wz[2] = {1.0f - tz, tz}, wy[2] = {1.0f - ty, ty}, wx[2] = {1.0f - tx, tx};
+            if (wide) {
+#pragma unroll
+              for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy) {
+                  const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+                  for (int dx = 0; dx < 2; ++dx) {
+                    const float w = wzy * wx[dx];
+                    const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                    lds_add_wide(&s_tile[o], w, v.x);
+                    lds_add_wide(&s_tile[NT + o], w, v.y);
+                    if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
                   }
-              } else if (!SPLIT) {
+                }
+            } else if (!SPLIT) {
 #pragma unroll
-                for (int dz = 0; dz < 2; ++dz)
+              for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
-                  for (int dy = 0; dy < 2; ++dy) {
-                    const float wzy = wz[dz] * wy[dy];
+                for (int dy = 0; dy < 2; ++dy) {
+                  const float wzy = wz[dz] * wy[dy];
 #pragma unroll
-                    for (int dx = 0; dx < 2; ++dx) {
-                      const float w = wzy * wx[dx];
-                      const int o = e[q] + dz * SY * SX + dy * SX + dx;
-                      lds_add_fixed(&s_tile[o], w, v.x);
-                      lds_add_fixed(&s_tile[NT + o], w, v.y);
-                      if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
-                    }
+                  for (int dx = 0; dx < 2; ++dx) {
+                    const float w = wzy * wx[dx];
+                    const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                    lds_add_fixed(&s_tile[o], w, v.x);
+                    lds_add_fixed(&s_tile[NT + o], w, v.y);
+                    if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
                   }
-              } else {
-                // split accumulators (re, im): low words first, then the rare carries
-                unsigned *t32 = reinterpret_cast<unsigned *>(s_tile);
-                unsigned old[8][2];
+                }
+            } else {
+              // split accumulators (re, im): low words first, then the rare carries
+              unsigned *t32 = reinterpret_cast<unsigned *>(s_tile);
+              unsigned old[8][2];
+#pragma unroll
+              for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy) {
+                  const float wzy = wz[dz] * wy[dy];
+#pragma unroll
+                  for (int dx = 0; dx < 2; ++dx) {
+                    const int k = dz * 4 + dy * 2 + dx;
+                    const float w = wzy * wx[dx];
+                    const int o = e[q] + dz * SY * SX + dy * SX + dx;
+                    old[k][0] = atomicAdd(&t32[o], (unsigned)to_fixed32(w, v.x));
+                    old[k][1] = atomicAdd(&t32[2 * NT + o], (unsigned)to_fixed32(w, v.y));
+                    if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                  }
+                }
+              unsigned inside = 0xffffffffu;
+#pragma unroll
+              for (int k = 0; k < 8; ++k) inside &= lo_window(old[k][0]) & lo_window(old[k][1]);
+              if (!(inside >> 31)) {
+                // the contributions again, from the same operands (the
+                // empty asm keeps the compiler from holding the first
+                // products in registers across the atomics)
+                float rx = v.x, ry = v.y, rz = tz, rw = ty, rt = tx;
+                asm volatile("" : "+v"(rx), "+v"(ry), "+v"(rz), "+v"(rw), "+v"(rt));
+                const float uz[2] = {1.0f - rz, rz}, uy[2] = {1.0f - rw, rw}, ux[2] = {1.0f - rt, rt};
 #pragma unroll
                 for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
                   for (int dy = 0; dy < 2; ++dy) {
-                    const float wzy = wz[dz] * wy[dy];
+                    const float wzy = uz[dz] * uy[dy];
 #pragma unroll
                     for (int dx = 0; dx < 2; ++dx) {
                       const int k = dz * 4 + dy * 2 + dx;
-                      const float w = wzy * wx[dx];
+                      const float w = wzy * ux[dx];
                       const int o = e[q] + dz * SY * SX + dy * SX + dx;
-                      old[k][0] = atomicAdd(&t32[o], (unsigned)to_fixed32(w, v.x));
-                      old[k][1] = atomicAdd(&t32[2 * NT + o], (unsigned)to_fixed32(w, v.y));
-                      if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
+                      const unsigned h0 = hi_step(old[k][0], to_fixed32(w, rx));
+                      const unsigned h1 = hi_step(old[k][1], to_fixed32(w, ry));
+                      if (h0) atomicAdd(&t32[NT + o], h0);
+                      if (h1) atomicAdd(&t32[3 * NT + o], h1);
                     }
                   }
-                unsigned inside = 0xffffffffu;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) inside &= lo_window(old[k][0]) & lo_window(old[k][1]);
-                if (!(inside >> 31)) {
-                  // the contributions again, from the same operands (the
-                  // empty asm keeps the compiler from holding the first
-                  // products in registers across the atomics)
-                  float rx = v.x, ry = v.y, rz = tz, rw = ty, rt = tx;
-                  asm volatile("" : "+v"(rx), "+v"(ry), "+v"(rz), "+v"(rw), "+v"(rt));
-                  const float uz[2] = {1.0f - rz, rz}, uy[2] = {1.0f - rw, rw}, ux[2] = {1.0f - rt, rt};
-#pragma unroll
-                  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-                    for (int dy = 0; dy < 2; ++dy) {
-                      const float wzy = uz[dz] * uy[dy];
-#pragma unroll
-                      for (int dx = 0; dx < 2; ++dx) {
-                        const int k = dz * 4 + dy * 2 + dx;
-                        const float w = wzy * ux[dx];
-                        const int o = e[q] + dz * SY * SX + dy * SX + dx;
-                        const unsigned h0 = hi_step(old[k][0], to_fixed32(w, rx));
-                        const unsigned h1 = hi_step(old[k][1], to_fixed32(w, ry));
-                        if (h0) atomicAdd(&t32[NT + o], h0);
-                        if (h1) atomicAdd(&t32[3 * NT + o], h1);
-                      }
-                    }
-                }
               }
             }
           }
         }
-        __builtin_amdgcn_wave_barrier();  // the next window rewrites the list
       }
       if (done < total_ch) {  // the next pass starts at the row holding chunk `done`
         const int kr = __popcll(__ballot(cexcl + nch <= done));
-        cskip = __builtin_amdgcn_readfirstlane(done - __shfl(cexcl, kr, 64) + (kr == 0 ? cskip : 0));
+        cskip = __builtin_amdgcn_readfirstlane(done - __builtin_amdgcn_readlane(cexcl, kr) + (kr == 0 ? cskip : 0));
         rb = __builtin_amdgcn_readfirstlane(rb + kr);
       } else {
         rb += 64;
