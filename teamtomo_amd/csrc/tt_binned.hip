@@ -51,7 +51,11 @@ namespace {
 // lines per row (61 tilts of 512^2 into 512^3, ms: 8x16x16 0.80, 4x8x32
 // 0.74, 4x8x64 0.70, 2x8x128 0.82).
 constexpr int kBZ = 4, kBY = 8, kBX = 64;
-constexpr int kNV = kBZ * kBY * kBX;
+// The last brick along x also owns the volume's final x voxel when the
+// other bricks leave one over (rfft rows of 2^k + 1 voxels): its tile rows
+// hold kBX + 1 voxels, and no block serves a one-voxel column.
+constexpr int kBXT = kBX + 1;
+constexpr int kNV = kBZ * kBY * kBXT;
 constexpr int kThreads = 256;       // pixel passes
 constexpr int kBrickThreads = 256;  // brick pass
 // A pixel lands in at most 8 bricks: the record buffer holds 8 per pixel (a
@@ -140,7 +144,7 @@ struct Targets {
   unsigned lo_ok, hi_ok;  // bit d: axis d
 };
 
-__device__ __forceinline__ Targets targets_of(const Cell &k) {
+__device__ __forceinline__ Targets targets_of(const Cell &k, const BinGrid &bg) {
   Targets tg;
   const int c[3] = {k.z0, k.y0, k.x0};
   const float t[3] = {k.tz, k.ty, k.tx};
@@ -148,7 +152,11 @@ __device__ __forceinline__ Targets targets_of(const Cell &k) {
   tg.lo_ok = tg.hi_ok = 0u;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int lo = c[d] / B[d], hi = (c[d] + 1) / B[d];
+    int lo = c[d] / B[d], hi = (c[d] + 1) / B[d];
+    if (d == 2) {  // the last x brick owns the final voxel
+      lo = min(lo, bg.nbx - 1);
+      hi = min(hi, bg.nbx - 1);
+    }
     tg.blo[d] = lo;
     if (t[d] < 1.0f || hi == lo) tg.lo_ok |= 1u << d;
     if (hi != lo && t[d] > 0.0f) tg.hi_ok |= 1u << d;
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void bin_count_kernel(SliceGeom g, const 
     Cell k;
     bool conj;
     if (!patch_pixel(g, u, i, j) || !bin_pixel<INSERT>(g, r, i, j, k, conj)) continue;
-    const Targets tg = targets_of(k);
+    const Targets tg = targets_of(k, bg);
 #pragma unroll 1
     for (int q = 0; q < 8; ++q) {
       const int b = target_brick(tg, q, bg);
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void bin_fill_kernel(SliceGeom g, const f
 #pragma unroll
   for (int u = 0; u < kPerThread; ++u) {
     if (!((okm >> u) & 1u)) continue;
-    const Targets tg = targets_of(cell[u]);
+    const Targets tg = targets_of(cell[u], bg);
     const unsigned vb = __float_as_uint(fmaxf(fabsf(val[u].x), fabsf(val[u].y)));
 #pragma unroll 1
     for (int q = 0; q < 8; ++q) {
@@ -305,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void bin_fill_kernel(SliceGeom g, const f
   for (int u = 0; u < kPerThread; ++u) {
     if (!((okm >> u) & 1u)) continue;
     const Cell &k = cell[u];
-    const Targets tg = targets_of(k);
+    const Targets tg = targets_of(k, bg);
 #pragma unroll 1
     for (int q = 0; q < 8; ++q) {
       const int b = target_brick(tg, q, bg);
@@ -336,14 +344,26 @@ __global__ __launch_bounds__(kThreads) void bin_fill_kernel(SliceGeom g, const f
 }
 
 // Exclusive prefix of the per-brick counts into offset and cursor, one block
-// over chunks of 8192 counts: each thread sums a run of 8 in LDS, a scan over
-// the 1024 runs, then the runs' prefixes are written back.
+// of 1024 threads over chunks of 8192 counts staged in LDS (coalesced loads
+// and stores): each thread sums a run of 8, the runs are scanned with DPP
+// lane moves inside each wave and the 16 wave totals, then the runs'
+// prefixes are written back.
+__device__ __forceinline__ int bin_wave_scan(int v) {  // inclusive, every lane active
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 __global__ __launch_bounds__(1024) void bin_scan_kernel(const int *__restrict__ count, int n,
                                                        int *__restrict__ offset, int *__restrict__ cursor) {
   constexpr int kPer = 8, kChunk = 1024 * kPer;
   __shared__ int s_v[kChunk];
-  __shared__ int s_sum[1024];
-  const int tid = (int)threadIdx.x;
+  __shared__ int s_wave[16];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int carry = 0;
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     for (int e = tid; e < kChunk; e += 1024) s_v[e] = c0 + e < n ? count[c0 + e] : 0;
@@ -351,22 +371,23 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(const int *__restrict__ 
     int run = 0;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) run += s_v[tid * kPer + q];
-    s_sum[tid] = run;
+    const int incl = bin_wave_scan(run);
+    if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      const int v = tid >= o ? s_sum[tid - o] : 0;
-      __syncthreads();
-      s_sum[tid] += v;
-      __syncthreads();
+    int wbase = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int t = s_wave[k];
+      wbase += k < wave ? t : 0;
+      total += t;
     }
-    int x = carry + s_sum[tid] - run;
+    int x = carry + wbase + incl - run;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int c = s_v[tid * kPer + q];
       s_v[tid * kPer + q] = x;
       x += c;
     }
-    const int total = s_sum[1023];
     __syncthreads();
     for (int e = tid; e < kChunk; e += 1024)
       if (c0 + e < n) {
@@ -374,7 +395,7 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(const int *__restrict__ 
         cursor[c0 + e] = s_v[e];
       }
     carry += total;
-    __syncthreads();
+    __syncthreads();  // the next chunk rewrites s_v and the wave totals
   }
 }
 
@@ -389,11 +410,14 @@ __global__ __launch_bounds__(kBrickThreads) void bin_brick_kernel(SliceGeom g, B
                                                                  int accumulate) {
   constexpr int NA = INSERT ? 3 : 2;
   __shared__ u64 tile[NA * kNV];
+  // (grouping the x bricks of a brick row on one XCD, so that the partial
+  // cache lines they share meet in one L2, measured 5 % slower)
   const int b = blockIdx.x;
-  const int bz = b / (bg.nby * bg.nbx), brem = b - bz * bg.nby * bg.nbx;
-  const int by = brem / bg.nbx, bx = brem - by * bg.nbx;
+  const int row = b / bg.nbx, bx = b - row * bg.nbx;
+  const int bz = row / bg.nby, by = row - bz * bg.nby;
   const int lo0 = bz * kBZ, lo1 = by * kBY, lo2 = bx * kBX;
-  const int ext0 = min(kBZ, g.vD - lo0), ext1 = min(kBY, g.vH - lo1), ext2 = min(kBX, g.vW - lo2);
+  const int ext0 = min(kBZ, g.vD - lo0), ext1 = min(kBY, g.vH - lo1);
+  const int ext2 = bx == bg.nbx - 1 ? g.vW - lo2 : kBX;
   const int cnt = count[b], off = offset[b];
   const float m = __uint_as_float(bmax[b]);  // the exact maximum |value| of the brick's records
   const int tid = threadIdx.x;
@@ -434,9 +458,9 @@ __global__ __launch_bounds__(kBrickThreads) void bin_brick_kernel(SliceGeom g, B
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
           const int lx = ox + dx;
-          if ((unsigned)lx >= (unsigned)kBX) continue;
+          if ((unsigned)lx >= (unsigned)ext2) continue;
           const float w = wzy * wx[dx];
-          const int o = (lz * kBY + ly) * kBX + lx;
+          const int o = (lz * kBY + ly) * kBXT + lx;
           atomicAdd(&tile[o], (u64)to_fixed64(__fmul_rn(w, vx)));
           atomicAdd(&tile[kNV + o], (u64)to_fixed64(__fmul_rn(w, vy)));
           if (INSERT && wout) atomicAdd(&tile[2 * kNV + o], (u64)(long long)to_fixed32(w, kWScale));
@@ -446,9 +470,13 @@ __global__ __launch_bounds__(kBrickThreads) void bin_brick_kernel(SliceGeom g, B
   }
   __syncthreads();
   const float inv_v = 1.0f / vscale;
-  for (int v = tid; v < kNV; v += kBrickThreads) {
-    const int lz = v / (kBY * kBX), ly = (v / kBX) % kBY, lx = v % kBX;
+  // a wave writes one 64-voxel x row; the last brick's extra column follows
+  for (int u = tid; u < kBZ * kBY * (kBX + 1); u += kBrickThreads) {
+    const bool extra = u >= kBZ * kBY * kBX;
+    const int r = extra ? u - kBZ * kBY * kBX : u / kBX;  // x row lz kBY + ly
+    const int lz = r / kBY, ly = r % kBY, lx = extra ? kBX : u % kBX;
     if (lz >= ext0 || ly >= ext1 || lx >= ext2) continue;
+    const int v = r * kBXT + lx;
     const int gz = lo0 + lz, gy = lo1 + ly, gx = lo2 + lx;
     const f2v val = {(float)(long long)tile[v] * inv_v, (float)(long long)tile[kNV + v] * inv_v};
     if (INSERT) {
@@ -473,7 +501,8 @@ __global__ __launch_bounds__(kThreads) void bin_zero_kernel(int *__restrict__ p,
 }
 
 BinGrid bin_grid(const SliceGeom &g) {
-  return BinGrid{(g.vD + kBZ - 1) / kBZ, (g.vH + kBY - 1) / kBY, (g.vW + kBX - 1) / kBX};
+  // x: bricks of kBX voxels, the last one kBX + 1 wide when that covers the row
+  return BinGrid{(g.vD + kBZ - 1) / kBZ, (g.vH + kBY - 1) / kBY, std::max(1, (g.vW - 1 + kBX - 1) / kBX)};
 }
 
 int64_t record_cap(const SliceGeom &g, int64_t n_rot) { return 8 * n_rot * (int64_t)g.P; }
@@ -497,9 +526,11 @@ int set_scatter_path(int path) {
 bool binned_selected(const SliceGeom &g, int64_t n_rot) {
   if (g_scatter_path == TT_SCATTER_BRICK || !binned_fits(g, n_rot)) return false;
   if (g_scatter_path == TT_SCATTER_BINNED) return true;
+  // pixels per 4 x 8 x 64 block of the volume (the measure the threshold was
+  // measured in, whether or not the last x brick is wider)
   const BinGrid bg = bin_grid(g);
-  const double n_bricks = (double)bg.nbz * bg.nby * bg.nbx;
-  return (double)n_rot * (double)g.P <= kAutoPixelsPerBrick * n_bricks;
+  const double n_blocks = (double)bg.nbz * bg.nby * (double)((g.vW + kBX - 1) / kBX);
+  return (double)n_rot * (double)g.P <= kAutoPixelsPerBrick * n_blocks;
 }
 
 int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float *rot, int64_t n_rot, f2v *out,
@@ -562,12 +593,13 @@ int binned_scatter(bool insert, const SliceGeom &g, const f2v *src, const float 
     st = check_launch("bin_fill_kernel");
   }
   if (!st) {
+    const unsigned bgrid = (unsigned)n_bricks;
     if (insert)
-      hipLaunchKernelGGL(bin_brick_kernel<true>, dim3((unsigned)n_bricks), dim3(kBrickThreads), 0, s, g, bg,
+      hipLaunchKernelGGL(bin_brick_kernel<true>, dim3(bgrid), dim3(kBrickThreads), 0, s, g, bg,
                          (const int *)count, (const int *)offset, (const unsigned *)bmax, (const BinRec *)recs, out,
                          wout, plane, wplane, accumulate);
     else
-      hipLaunchKernelGGL(bin_brick_kernel<false>, dim3((unsigned)n_bricks), dim3(kBrickThreads), 0, s, g, bg,
+      hipLaunchKernelGGL(bin_brick_kernel<false>, dim3(bgrid), dim3(kBrickThreads), 0, s, g, bg,
                          (const int *)count, (const int *)offset, (const unsigned *)bmax, (const BinRec *)recs, out,
                          wout, plane, wplane, accumulate);
     st = check_launch("bin_brick_kernel");

@@ -1196,7 +1196,8 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
     f2v *__restrict__ out, float *__restrict__ wout, f2v *__restrict__ plane,
     float *__restrict__ wplane, u64 *__restrict__ segbuf, float *__restrict__ halo, ScaleArgs sa,
     int accumulate, int *__restrict__ touched) {
-  // one unit per block (a stride loop over units measured 4-25 % slower)
+  // one unit per block (a stride loop over units measured 4-25 % slower, and
+  // triples the adjoint's register spills)
   const Unit un = units[blockIdx.x];
   brick_body<BZ, BY, BX, MODE, MASK, NCH>(src, g, bg, pre, ptest, un, n_rot, out, wout, plane, wplane, segbuf,
                                           halo, sa, accumulate, touched);
@@ -1398,7 +1399,7 @@ __global__ __launch_bounds__(256) void maxabs_chunks_kernel(const f2v *__restric
 // then one atomic per band and block.
 __device__ __forceinline__ void band_max_body(const f2v *__restrict__ x, const SliceGeom &g, int64_t n_rot,
                                               int64_t rot_stride, unsigned *__restrict__ bands, int block,
-                                              int nblocks) {
+                                              int nblocks, int sgroup = 0, int ngroups = 1) {
   __shared__ unsigned s_b[kBands];
   for (int k = threadIdx.x; k < kBands; k += blockDim.x) s_b[k] = 0u;
   __syncthreads();
@@ -1409,16 +1410,18 @@ __device__ __forceinline__ void band_max_body(const f2v *__restrict__ x, const S
     int i, j;
     split_pixel(g, p, i, j);
     const float fy = (float)(i - g.half_H) * ih, fx = (float)j * iw;
+    // this block's sampled slices: sgroup, sgroup + ngroups, ...
     float m = 0.0f;
-    int64_t sl = 0;
-    for (; sl + 4 <= nsl; sl += 4) {  // four loads in flight
-      f2v v[4];
+    int64_t sl = sgroup;
+    const int64_t gstep = (int64_t)ngroups * step;
+    for (; sl + 7 * (int64_t)ngroups < nsl; sl += 8 * (int64_t)ngroups) {  // eight loads in flight
+      f2v v[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = x[(sl + u) * step + p];
+      for (int u = 0; u < 8; ++u) v[u] = x[sl * step + u * gstep + p];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) m = fmaxf(m, fmaxf(fabsf(v[u].x), fabsf(v[u].y)));
+      for (int u = 0; u < 8; ++u) m = fmaxf(m, fmaxf(fabsf(v[u].x), fabsf(v[u].y)));
     }
-    for (; sl < nsl; ++sl) {
+    for (; sl < nsl; sl += ngroups) {
       const f2v v = x[sl * step + p];
       m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
     }
@@ -1431,14 +1434,25 @@ __device__ __forceinline__ void band_max_body(const f2v *__restrict__ x, const S
 
 __global__ __launch_bounds__(256) void band_max_kernel(const f2v *__restrict__ x, SliceGeom g, int64_t n_rot,
                                                        int64_t rot_stride, unsigned *__restrict__ bands) {
-  band_max_body(x, g, n_rot, rot_stride, bands, (int)blockIdx.x, (int)gridDim.x);
+  band_max_body(x, g, n_rot, rot_stride, bands, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y, (int)gridDim.y);
 }
+
+// Slices sampled for the band maxima: about 64 spread over the stack.
+__host__ __device__ inline int64_t band_stride(int64_t n_rot) { return n_rot > 64 ? n_rot / 64 : 1; }
 
 // Blocks of a band-maximum pass: pixels of one slice over 256 threads.
 inline int band_blocks(const SliceGeom &g) { return std::max(1, std::min((g.P + 255) / 256, 256)); }
 
-// Slices sampled for the band maxima: about 64 spread over the stack.
-__host__ __device__ inline int64_t band_stride(int64_t n_rot) { return n_rot > 64 ? n_rot / 64 : 1; }
+// Groups of sampled slices of the one-call band-maximum pass (grid y): four,
+// when there are 32 slices or more.  (More groups make the blocks' global
+// atomicMax on the 64 band words contend: 15 groups measured 30 us against
+// 22 us for one at 256^3 x 4096.)
+inline int band_groups(const SliceGeom &g, int64_t n_rot) {
+  (void)g;
+  const int64_t nsl = (n_rot + band_stride(n_rot) - 1) / band_stride(n_rot);
+  return nsl >= 32 ? 4 : 1;
+}
+
 
 __global__ void set_word_kernel(unsigned *p, unsigned v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -1955,7 +1969,8 @@ int run_scatter(const SliceGeom &g, const f2v *src, const float *rot, int64_t n_
   const int mgrid = (int)std::min<int64_t>(std::max<int64_t>(n / 4096, 1), 2048);
   if (n_rot > 0) {
     if (sampled) {
-      hipLaunchKernelGGL(band_max_kernel, dim3((unsigned)band_blocks(g)), dim3(256), 0, s, src, g, n_rot,
+      hipLaunchKernelGGL(band_max_kernel, dim3((unsigned)band_blocks(g), (unsigned)band_groups(g, n_rot)), dim3(256),
+                         0, s, src, g, n_rot,
                          band_stride(n_rot), words + kBandWord);
     } else {
       hipLaunchKernelGGL(maxabs_chunks_kernel, dim3(mgrid), dim3(256), 0, s, src, n, (int64_t)1,
