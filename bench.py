@@ -72,7 +72,102 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB and exit (calibrates the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters against a copy of known size)")
+    ap.add_argument("--write-traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "FETCH_DIR_INSERT",
+                                                         "WRITE_DIR_INSERT"),
+                    help="only write profiles/traffic.json from four rocprofv3 --pmc output directories "
+                         "(FETCH_SIZE and WRITE_SIZE runs of the default workload, then of --workload "
+                         "insert3d) and print their per-dispatch tables; no GPU is used")
     return ap.parse_args()
+
+
+# Launch sequence of one call of each measured entry point, in dispatch order
+# (kernel names as rocprofv3 prints them, without the argument list).
+CALL_KERNELS = {
+    "extract": ["gather_zero_kernel", "plane_rec_kernel", "brick_kernel<8, 16, 16, 0, false, 1>"],
+    "adjoint": ["fill_zero_kernel<unsigned int>", "band_max_kernel", "plane_rec_kernel",
+                "brick_kernel<16, 16, 8, 1, false, 1>", "split_combine_kernel<16, 16, 8, 1>",
+                "halo_combine_kernel<16, 16, 8, 1>", "maxabs_chunks_kernel",
+                "brick_kernel<16, 16, 8, 1, false, 1>", "split_combine_kernel<16, 16, 8, 1>",
+                "halo_combine_kernel<16, 16, 8, 1>"],
+    "insert_256": ["fill_zero_kernel<unsigned int>", "band_max_kernel", "plane_rec_kernel",
+                   "brick_kernel<16, 8, 8, 2, false, 1>", "split_combine_kernel<16, 8, 8, 2>",
+                   "halo_combine_kernel<16, 8, 8, 2>", "maxabs_chunks_kernel",
+                   "brick_kernel<16, 8, 8, 2, false, 1>", "split_combine_kernel<16, 8, 8, 2>",
+                   "halo_combine_kernel<16, 8, 8, 2>", "mirror_kernel"],
+    "insert_512_tilts": ["bin_zero_kernel", "bin_count_kernel<true>", "bin_scan_kernel", "bin_fill_kernel<true>",
+                         "bin_brick_kernel<true>", "mirror_kernel"],
+}
+# Bytes per counter unit, from a 1 GiB device-to-device copy (--copy-probe
+# 1024): FETCH_SIZE reads 524300 KiB and WRITE_SIZE 1048576 KiB per copy.
+READ_BYTES_PER_FETCH_UNIT = 2.0 * 1024
+WRITE_BYTES_PER_WRITE_UNIT = 1.0 * 1024
+
+
+def _kernel_name(full):
+    """rocprofv3's kernel name without 'void', the tt namespaces and the argument list."""
+    n = full[5:] if full.startswith("void ") else full
+    n = n.replace("tt::(anonymous namespace)::", "").replace("tt::", "")
+    depth = 0
+    for i, c in enumerate(n):
+        if c == "<":
+            depth += 1
+        elif c == ">":
+            depth -= 1
+        elif c == "(" and depth == 0:
+            return n[:i]
+    return n
+
+
+def _pmc_dispatches(d, counter):
+    """[(dispatch id, kernel name, value)] of one rocprofv3 --pmc run, in dispatch order."""
+    import csv
+
+    rows = []
+    with open(os.path.join(d, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                rows.append((int(r["Dispatch_Id"]), _kernel_name(r["Kernel_Name"]), float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def _last_call(rows, seq):
+    """The dispatches of the last occurrence of the launch sequence seq."""
+    names = [r[1] for r in rows]
+    for i in range(len(rows) - len(seq), -1, -1):
+        if names[i:i + len(seq)] == seq:
+            return rows[i:i + len(seq)]
+    raise SystemExit(f"launch sequence {seq} not found")
+
+
+def write_traffic(dirs):
+    """profiles/traffic.json from FETCH_SIZE / WRITE_SIZE runs (see --write-traffic), and
+    the per-dispatch tables of the last call of each entry point on stdout."""
+    fh, wh, fi, wi = dirs
+    runs = {"extract": (fh, wh), "adjoint": (fh, wh), "insert_256": (fi, wi), "insert_512_tilts": (fi, wi)}
+    out = {"_note": "HBM-side bytes per launch sequence (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, KiB->bytes) "
+                    "from rocprofv3 --pmc runs of bench.py (--write-traffic) on the kernel sources of "
+                    "source_digest; see extract_adjoint_256c64_4096rot.hbm_traffic.md and "
+                    "insert_256c64_4096rot_512c64_61tilts.hbm_traffic.md",
+           "source_digest": kernel_source_digest()}
+    per = {}
+    for key, (fd, wd) in runs.items():
+        seq = CALL_KERNELS[key]
+        fr = _last_call(_pmc_dispatches(fd, "FETCH_SIZE"), seq)
+        wr = _last_call(_pmc_dispatches(wd, "WRITE_SIZE"), seq)
+        total = 0.0
+        print(f"\n{key}\n\n| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read MB | written MB |\n|---|---|---|---|---|")
+        for (_, name, f), (_, _, w) in zip(fr, wr):
+            rb, wb = f * READ_BYTES_PER_FETCH_UNIT, w * WRITE_BYTES_PER_WRITE_UNIT
+            total += rb + wb
+            per[f"{key}/{name}"] = per.get(f"{key}/{name}", 0.0) + rb + wb
+            print(f"| {name} | {f:.0f} | {w:.0f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} |")
+        out[key] = total
+        print(f"\n{key}: {total / 1e6:.0f} MB per call")
+    out["per_kernel"] = per
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"\nwrote profiles/traffic.json (source digest {out['source_digest']})")
 
 
 def make_inputs(device):
@@ -787,6 +882,9 @@ def run_project(args, device):
 
 def main():
     args = parse()
+    if args.write_traffic:
+        write_traffic(args.write_traffic)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not launched)"}))

@@ -194,3 +194,32 @@ def test_bench_algorithmic_bytes():
     P, V = 256 * 129, 256 * 256 * 129
     assert ext == adj == 4096 * (8 * P + 36) + 8 * V
     assert abs(ext / 1e9 - 1.150) < 1e-3
+
+
+def test_bench_traffic_parsing(tmp_path):
+    """bench.py --write-traffic reads rocprofv3 counter CSVs: kernel names lose
+    'void', the library's namespaces and the argument list; a call's
+    dispatches are the last occurrence of its launch sequence."""
+    import importlib.util
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    full = "void tt::(anonymous namespace)::brick_kernel<8, 16, 16, 0, false, 1>(float __vector(2) const*, tt::SliceGeom)"
+    assert mod._kernel_name(full) == "brick_kernel<8, 16, 16, 0, false, 1>"
+    assert mod._kernel_name("tt::(anonymous namespace)::gather_zero_kernel(tt::SliceGeom, long)") == "gather_zero_kernel"
+    seq = mod.CALL_KERNELS["extract"]
+    hdr = '"Dispatch_Id","Kernel_Name","Counter_Name","Counter_Value"\n'
+    lines = []
+    did = 0
+    for rep in range(3):  # three calls, with a foreign kernel between them
+        for k, name in enumerate(seq):
+            did += 1
+            lines.append(f'{did},"void tt::{name}(int)","FETCH_SIZE",{100 * rep + k}\n')
+        did += 1
+        lines.append(f'{did},"void at::native::copy(int)","FETCH_SIZE",7\n')
+    (tmp_path / "run_counter_collection.csv").write_text(hdr + "".join(lines))
+    rows = mod._pmc_dispatches(str(tmp_path), "FETCH_SIZE")
+    last = mod._last_call(rows, seq)
+    assert [r[1] for r in last] == seq and [r[2] for r in last] == [200.0, 201.0, 202.0]
