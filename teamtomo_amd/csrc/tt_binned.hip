@@ -348,16 +348,6 @@ __global__ __launch_bounds__(kThreads) void bin_fill_kernel(SliceGeom g, const f
 // and stores): each thread sums a run of 8, the runs are scanned with DPP
 // lane moves inside each wave and the 16 wave totals, then the runs'
 // prefixes are written back.
-__device__ __forceinline__ int bin_wave_scan(int v) {  // inclusive, every lane active
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return v;
-}
-
 __global__ __launch_bounds__(1024) void bin_scan_kernel(const int *__restrict__ count, int n,
                                                        int *__restrict__ offset, int *__restrict__ cursor) {
   constexpr int kPer = 8, kChunk = 1024 * kPer;
@@ -371,7 +361,7 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(const int *__restrict__ 
     int run = 0;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) run += s_v[tid * kPer + q];
-    const int incl = bin_wave_scan(run);
+    const int incl = wave_incl_scan(run);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     int wbase = 0, total = 0;

@@ -154,30 +154,6 @@ __global__ __launch_bounds__(256) void plane_rec_kernel(SliceGeom g, const float
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
-// Inclusive prefix sum over the wave with DPP lane moves (row_shr 1, 2, 4, 8
-// inside rows of 16 lanes, then row_bcast 15 and 31 across rows): six VALU
-// operations and no LDS traffic.  Call with every lane active.
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-  return v;
-}
-
-// Inclusive running maximum of unsigned values over the wave (same lane moves).
-__device__ __forceinline__ unsigned wave_incl_max(unsigned v) {
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
-  return v;
-}
-
 __device__ __forceinline__ float uniform_f(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
@@ -405,32 +381,6 @@ __device__ __forceinline__ void lds_add_fixed(u64 *p, float w, float v) {
 // which commutes with its rounding) as an exact 64-bit fixed-point value.
 __device__ __forceinline__ void lds_add_wide(u64 *p, float w, float v) {
   atomicAdd(p, (u64)to_fixed64(__fmul_rn(w, v)));
-}
-
-// Split accumulators of the 32-bit contributions.  Each accumulator of the
-// tile is a plane of NT low words followed by a plane of NT high words, and a
-// voxel's 64-bit fixed-point sum is hi 2^32 + lo - kLoBias.  A contribution c
-// (|c| <= 2^30) is added to the low word alone with a returning 32-bit LDS
-// atomic (at random addresses it costs about 60 % of a 64-bit one); the high
-// word changes only when the low word wraps, which needs the low word it
-// replaced outside [2^30, 3 2^30).  Low words start at kLoBias = 2^31, the
-// middle of that window, so wraps are rare: every returned low word is
-// tested, and a lane that saw one outside the window adds the exact carries
-// to the high words.  The sums are integers, so the result does not depend
-// on the order of the atomics, and equals the 64-bit accumulation's.
-constexpr unsigned kLoBias = 0x80000000u;
-
-// bit 31 set: the low word was inside the window (no carry possible)
-__device__ __forceinline__ unsigned lo_window(unsigned old) { return old + 0x40000000u; }
-
-// Change of the high word when c is added to a low word that held old.
-__device__ __forceinline__ unsigned hi_step(unsigned old, int c) {
-  const bool carry = old + (unsigned)c < old;
-  return carry ? (c < 0 ? 0u : 1u) : (c < 0 ? 0xffffffffu : 0u);
-}
-
-__device__ __forceinline__ u64 split_value(unsigned lo, unsigned hi) {
-  return (((u64)hi << 32) | lo) - (u64)kLoBias;
 }
 
 // The exact canonical FP64 cell of pixel (i, j) under rotation r and Friedel

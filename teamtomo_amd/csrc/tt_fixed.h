@@ -34,6 +34,32 @@ __device__ __forceinline__ long long to_fixed64(float p) {
   return (long long)((u64)(long long)(int)hi << 31) + (long long)(int)rintf(lo);
 }
 
+// Split accumulators of the 32-bit contributions.  Each accumulator of a
+// tile is a plane of low words followed by a plane of high words, and a
+// voxel's 64-bit fixed-point sum is hi 2^32 + lo - kLoBias.  A contribution c
+// (|c| <= 2^30) is added to the low word alone with a returning 32-bit LDS
+// atomic (at random addresses it costs about 60 % of a 64-bit one); the high
+// word changes only when the low word wraps, which needs the low word it
+// replaced outside [2^30, 3 2^30).  Low words start at kLoBias = 2^31, the
+// middle of that window, so wraps are rare: every returned low word is
+// tested, and a lane that saw one outside the window adds the exact carries
+// to the high words.  The sums are integers, so the result does not depend
+// on the order of the atomics, and equals the 64-bit accumulation's.
+constexpr unsigned kLoBias = 0x80000000u;
+
+// bit 31 set: the low word was inside the window (no carry possible)
+__device__ __forceinline__ unsigned lo_window(unsigned old) { return old + 0x40000000u; }
+
+// Change of the high word when c is added to a low word that held old.
+__device__ __forceinline__ unsigned hi_step(unsigned old, int c) {
+  const bool carry = old + (unsigned)c < old;
+  return carry ? (c < 0 ? 0u : 1u) : (c < 0 ? 0xffffffffu : 0u);
+}
+
+__device__ __forceinline__ u64 split_value(unsigned lo, unsigned hi) {
+  return (((u64)hi << 32) | lo) - (u64)kLoBias;
+}
+
 // ceil(log2(n)) for n >= 1 (0 for n <= 1).
 __host__ __device__ inline int ceil_log2(int64_t n) {
   int b = 0;
