@@ -172,18 +172,6 @@ __device__ __forceinline__ int rank_below(unsigned long long mask) {
                                         __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// First lane k in [0, NL) whose inclusive prefix `incl` exceeds p (NL a power of 2).
-template <int NL>
-__device__ __forceinline__ int find_segment(int incl, int p) {
-  int k = 0;
-#pragma unroll
-  for (int step = NL / 2; step > 0; step >>= 1) {
-    const int v = __shfl(incl, k + step - 1, 64);
-    if (v <= p) k += step;
-  }
-  return k;
-}
-
 // ---------------------------------------------------------------------------
 // brick units
 // ---------------------------------------------------------------------------
@@ -1501,8 +1489,8 @@ __device__ __forceinline__ void lin_interval(double A, double B, double iB, doub
 __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
                                                  int64_t n_rows, f2v *__restrict__ out, int64_t block,
                                                  int n_chan = 1) {
-  constexpr int CH = 4;
   __shared__ ZeroRow s_zr[4][64];
+  __shared__ unsigned s_mark[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t t = (block * 4 + wave) * 64 + lane;
   int total = 0;
@@ -1560,27 +1548,32 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
     total = z.n1 + z.n2 + z.n3;
     s_zr[wave][lane] = z;
   }
-  const int nch = (total + CH - 1) / CH;
-  const int cincl = wave_incl_scan(nch);
-  const int total_ch = __shfl(cincl, 63, 64);
-  __builtin_amdgcn_wave_barrier();
-  for (int pc = 0; pc < total_ch; pc += 64) {
-    const int pidx = pc + lane;
-    const int k = find_segment<64>(cincl, pidx);
-    const int cexcl_raw = __shfl(cincl, k > 0 ? k - 1 : 0, 64);
-    const int cexcl = k > 0 ? cexcl_raw : 0;
-    const bool live = pidx < total_ch;
-    const ZeroRow z = s_zr[wave][k];
-    const int q0 = (pidx - cexcl) * CH;
-#pragma unroll
-    for (int q = 0; q < CH; ++q) {
-      const int qq = q0 + q;
+  // the rows' zero pixels 64 at a time, consecutive lanes on consecutive
+  // pixels of a row (coalesced stores); a lane finds its row from marks that
+  // the rows write at their first pixel of the step, carried to the pixels
+  // that follow by a running maximum over the wave
+  const int incl = wave_incl_scan(total);
+  const int excl = incl - total;
+  const int total_px = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(incl, 63));
+  unsigned *marks = s_mark[wave];
+  for (int pb = 0; pb < total_px; pb += 64) {
+    const unsigned long long hb = __ballot(excl <= pb && pb < incl);
+    const int k0 = (int)__builtin_ctzll(hb);
+    __builtin_amdgcn_wave_barrier();  // the previous step's marks have been read
+    marks[lane] = lane == 0 ? (unsigned)(k0 + 1) : 0u;
+    if (total > 0 && excl > pb && excl < pb + 64) marks[excl - pb] = (unsigned)(lane + 1);
+    __builtin_amdgcn_wave_barrier();
+    const int k = (int)wave_incl_max(marks[lane]) - 1;
+    const int ek = __shfl(excl, k, 64);  // (every lane active)
+    const int p = pb + lane;
+    if (p < total_px) {
+      const ZeroRow z = s_zr[wave][k];
+      const int qq = p - ek;
       int j;
       if (qq < z.n1) j = z.a1 + qq;
       else if (qq < z.n1 + z.n2) j = z.a2 + (qq - z.n1);
       else j = z.a3 + (qq - z.n1 - z.n2);
-      if (live && qq < z.n1 + z.n2 + z.n3)
-        for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
+      for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
     }
   }
 }
