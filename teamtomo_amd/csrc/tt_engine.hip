@@ -1460,17 +1460,23 @@ __global__ __launch_bounds__(256) void uncrossed_fill_kernel(SliceGeom g, BrickG
 }
 
 // ---------------------------------------------------------------------------
-// Gather zero pass: writes 0 to every pixel that is not certainly inside the
-// volume and the disc.  One lane per slice row computes, for each Friedel
-// half, the interval of columns whose canonical coordinates are inside the
-// volume by a margin of 1e-7 voxel, whose half decision is certain and (with
-// fftfreq_max) whose disc test is certain; the row's other columns (at most
-// three runs) get a zero.  Runs before the brick gather, which overwrites
-// the pixels of the margins that are inside after all.
+// Gather zero pass: writes 0 to every pixel that the brick gather does not
+// write, i.e. whose canonical coordinate (Friedel half included) lies
+// outside the volume or, with fftfreq_max, outside the disc.  One lane per
+// slice row computes, for each Friedel half, the interval of columns whose
+// canonical coordinates are inside the volume by a margin of 1e-7 voxel,
+// whose half decision is certain and (with fftfreq_max) whose disc test is
+// certain, and the wider interval of columns that are possibly inside; the
+// row's other columns (at most three runs) get a zero, except that a column
+// in a possible interval is first tested with the canonical coordinate, as
+// the pixel gather evaluates it.  The pass therefore writes exactly the
+// pixels no brick writes, and may run before, after or between them.
 // ---------------------------------------------------------------------------
 struct ZeroRow {
   long long base;
   int a1, n1, a2, n2, a3, n3;  // three runs of columns [a, a + n)
+  int r, i;                    // rotation and slice row
+  int p[4];                    // columns possibly inside, per Friedel half: [p0, p1], [p2, p3]
 };
 
 __device__ __forceinline__ void lin_interval(double A, double B, double iB, double lo, double hi,
@@ -1486,13 +1492,15 @@ __device__ __forceinline__ void lin_interval(double A, double B, double iB, doub
   }
 }
 
-__device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
-                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block,
-                                                 int n_chan = 1) {
-  __shared__ ZeroRow s_zr[4][64];
-  __shared__ unsigned s_mark[4][64];
+// Rows [block * NWZ * 64, (block + 1) * NWZ * 64) of the stack, one per lane
+// of the block's NWZ waves, with per-wave LDS tables s_zr (row records) and
+// s_mark (marks).
+template <int NWZ>
+__device__ __forceinline__ void zero_rows(const SliceGeom &g, const float *__restrict__ rot, int64_t n_rows,
+                                          f2v *__restrict__ out, int64_t block, int n_chan,
+                                          ZeroRow (*s_zr)[64], unsigned (*s_mark)[64]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t t = (block * 4 + wave) * 64 + lane;
+  const int64_t t = (block * NWZ + wave) * 64 + lane;
   int total = 0;
   if (t < n_rows) {
     const int64_t r = t / g.H;
@@ -1504,31 +1512,43 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
 #pragma unroll
     for (int d = 0; d < 3; ++d) iV[d] = fabs(rr.V[d]) > 1e-30 ? 1.0 / rr.V[d] : 0.0;
     const double top[3] = {(double)(g.vD - 1), (double)(g.vH - 1), (double)(g.vW - 1)};
-    int a[2], b[2];
+    int a[2], b[2], pa[2], pb[2];
 #pragma unroll
     for (int sh = 0; sh < 2; ++sh) {
       const double s = sh ? -1.0 : 1.0;
-      double jlo = 0.0, jhi = (double)(g.Wr - 1);
-      // certain Friedel half: the extraction decision flips within tolx of x' = 0
       const double ux = __dmul_rn(rr.U[2], fy);
-      lin_interval(s * ux, s * rr.V[2], s * iV[2], g.tolx + m, 1e300, jlo, jhi);
       const double C0 = __fma_rn(s * rr.U[0], fy, g.bz), C1 = __fma_rn(s * rr.U[1], fy, g.by);
-      lin_interval(C0, s * rr.V[0], s * iV[0], m, top[0] - m, jlo, jhi);
-      lin_interval(C1, s * rr.V[1], s * iV[1], m, top[1] - m, jlo, jhi);
-      lin_interval(s * ux, s * rr.V[2], s * iV[2], m, top[2] - m, jlo, jhi);
-      if (g.use_mask) {
-        // in_disc (float32) holds wherever the exact radius is 1e-6 below fmax
-        const double fm = (double)g.fmax * (1.0 - 1e-6);
-        const double fyv = (double)(i - g.half_H) * (double)g.inv_H32;
-        const double rem = fm * fm - fyv * fyv;
-        if (rem < 0.0 || fm < 0.0) jhi = -1e300;
-        else jhi = fmin(jhi, sqrt(rem) / (double)g.inv_W32 - 1e-6);
-      }
-      a[sh] = 0;
-      b[sh] = -1;
-      if (jlo <= jhi) {
-        a[sh] = max((int)ceil(jlo + 1e-7), 0);
-        b[sh] = min((int)floor(jhi - 1e-7), g.Wr - 1);
+      // e = +1: certainly inside (margins inward); e = -1: possibly inside
+#pragma unroll
+      for (int ce = 0; ce < 2; ++ce) {
+        const double e = ce ? -1.0 : 1.0;
+        double jlo = 0.0, jhi = (double)(g.Wr - 1);
+        // the Friedel half: the extraction decision flips within tolx of x' = 0
+        lin_interval(s * ux, s * rr.V[2], s * iV[2], e * (g.tolx + m), 1e300, jlo, jhi);
+        lin_interval(C0, s * rr.V[0], s * iV[0], e * m, top[0] - e * m, jlo, jhi);
+        lin_interval(C1, s * rr.V[1], s * iV[1], e * m, top[1] - e * m, jlo, jhi);
+        lin_interval(s * ux, s * rr.V[2], s * iV[2], e * m, top[2] - e * m, jlo, jhi);
+        if (g.use_mask) {
+          // in_disc (float32) holds wherever the exact radius is 1e-6 below
+          // fmax, and fails wherever it is 1e-6 above
+          const double fm = (double)g.fmax * (1.0 - e * 1e-6);
+          const double fyv = (double)(i - g.half_H) * (double)g.inv_H32;
+          const double rem = fm * fm - fyv * fyv;
+          if (rem < 0.0 || fm < 0.0) jhi = -1e300;
+          else jhi = fmin(jhi, sqrt(rem) / (double)g.inv_W32 - e * 1e-6);
+        }
+        int lo = 0, hi = -1;
+        if (jlo <= jhi) {
+          lo = max((int)ceil(jlo + e * 1e-7), 0);
+          hi = min((int)floor(jhi - e * 1e-7), g.Wr - 1);
+        }
+        if (ce == 0) {
+          a[sh] = lo;
+          b[sh] = hi;
+        } else {
+          pa[sh] = lo;
+          pb[sh] = hi;
+        }
       }
     }
     // sort the two certain intervals, drop empty ones, merge overlaps
@@ -1538,6 +1558,12 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
     if (a2 <= b2 && a2 <= b1 + 1) { b1 = max(b1, b2); a2 = 0; b2 = -1; }
     ZeroRow z;
     z.base = (long long)r * g.ostride + (long long)i * g.Wr;
+    z.r = (int)r;
+    z.i = i;
+    z.p[0] = pa[0];
+    z.p[1] = pb[0];
+    z.p[2] = pa[1];
+    z.p[3] = pb[1];
     if (a1 > b1) {  // nothing certain: the whole row
       z.a1 = 0; z.n1 = g.Wr; z.a2 = 0; z.n2 = 0; z.a3 = 0; z.n3 = 0;
     } else if (a2 > b2) {
@@ -1573,9 +1599,25 @@ __device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float
       if (qq < z.n1) j = z.a1 + qq;
       else if (qq < z.n1 + z.n2) j = z.a2 + (qq - z.n1);
       else j = z.a3 + (qq - z.n1 - z.n2);
-      for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
+      bool zero = true;
+      if ((j >= z.p[0] && j <= z.p[1]) || (j >= z.p[2] && j <= z.p[3])) {
+        // possibly inside: the canonical test of the pixel gather (a few
+        // columns at the ends of the certain intervals)
+        const Rot rj = load_rot(rot + (int64_t)z.r * 9, g);
+        zero = !(in_disc(g, z.i, j) && inside(g, slice_coord<false>(g, rj, z.i, j)));
+      }
+      if (zero)
+        for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
     }
   }
+}
+
+__device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
+                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block,
+                                                 int n_chan = 1) {
+  __shared__ ZeroRow s_zr[4][64];
+  __shared__ unsigned s_mark[4][64];
+  zero_rows<4>(g, rot, n_rows, out, block, n_chan, s_zr, s_mark);
 }
 
 __global__ __launch_bounds__(256) void gather_zero_kernel(SliceGeom g, const float *__restrict__ rot,
