@@ -83,7 +83,7 @@ def parse():
 # Launch sequence of one call of each measured entry point, in dispatch order
 # (kernel names as rocprofv3 prints them, without the argument list).
 CALL_KERNELS = {
-    "extract": ["gather_zero_kernel", "plane_rec_kernel", "brick_kernel<8, 16, 16, 0, false, 1>"],
+    "extract": ["plane_rec_kernel", "gather_kernel<8, 16, 16, false, 1>"],
     "adjoint": ["fill_zero_kernel<unsigned int>", "band_max_kernel", "plane_rec_kernel",
                 "brick_kernel<16, 16, 8, 1, false, 1>", "split_combine_kernel<16, 16, 8, 1>",
                 "halo_combine_kernel<16, 16, 8, 1>", "maxabs_chunks_kernel",
@@ -937,7 +937,7 @@ def main():
     ext_b, adj_b = algorithmic_bytes()
     kernels = {
         "extract": {"ms": ext_ms, "bytes": ext_b, "GBs": ext_b / ext_ms / 1e6,
-                    "launch": "tt_extract_c64: gather_zero_kernel + plane_rec_kernel + brick_kernel (gather)"},
+                    "launch": "tt_extract_c64: plane_rec_kernel + gather_kernel (brick units and zero rows)"},
         "adjoint": {"ms": adj_ms, "bytes": adj_b, "GBs": adj_b / adj_ms / 1e6,
                     "launch": "tt_extract_bwd_c64 volume gradient: fill_zero_kernel + band_max_kernel "
                               "(|f|-band maxima of every 64th slice) + plane_rec_kernel + brick_kernel (adjoint) + "

@@ -505,17 +505,187 @@ __device__ void flush_tile(const u64 *__restrict__ acc, bool split, const SliceG
 }
 
 // ---------------------------------------------------------------------------
+// Gather zero pass: writes 0 to every pixel that the brick gather does not
+// write, i.e. whose canonical coordinate (Friedel half included) lies
+// outside the volume or, with fftfreq_max, outside the disc.  One lane per
+// slice row computes, for each Friedel half, the interval of columns whose
+// canonical coordinates are inside the volume by a margin of 1e-7 voxel,
+// whose half decision is certain and (with fftfreq_max) whose disc test is
+// certain, and the wider interval of columns that are possibly inside; the
+// row's other columns (at most three runs) get a zero, except that a column
+// in a possible interval is first tested with the canonical coordinate, as
+// the pixel gather evaluates it.  The pass therefore writes exactly the
+// pixels no brick writes, and may run before, after or between them.
+// ---------------------------------------------------------------------------
+struct ZeroRow {
+  long long base;
+  int a1, n1, a2, n2, a3, n3;  // three runs of columns [a, a + n)
+  int r, i;                    // rotation and slice row
+  int p[4];                    // columns possibly inside, per Friedel half: [p0, p1], [p2, p3]
+};
+
+__device__ __forceinline__ void lin_interval(double A, double B, double iB, double lo, double hi,
+                                             double &jlo, double &jhi) {
+  // lo <= A + j B <= hi, iB = 1 / B (the bounds carry a 1e-7 margin, far
+  // above the rounding of the reciprocal)
+  if (fabs(B) > 1e-30) {
+    const double t1 = (lo - A) * iB, t2 = (hi - A) * iB;
+    jlo = fmax(jlo, fmin(t1, t2));
+    jhi = fmin(jhi, fmax(t1, t2));
+  } else if (A < lo || A > hi) {
+    jhi = -1e300;
+  }
+}
+
+// Rows [block * NWZ * 64, (block + 1) * NWZ * 64) of the stack, one per lane
+// of the block's NWZ waves, with per-wave LDS tables s_zr (row records) and
+// s_mark (marks).
+template <int NWZ>
+__device__ __forceinline__ void zero_rows(const SliceGeom &g, const float *__restrict__ rot, int64_t n_rows,
+                                          f2v *__restrict__ out, int64_t block, int n_chan,
+                                          ZeroRow (*s_zr)[64], unsigned (*s_mark)[64]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t t = (block * NWZ + wave) * 64 + lane;
+  int total = 0;
+  if (t < n_rows) {
+    const int64_t r = t / g.H;
+    const int i = (int)(t - r * g.H);
+    const Rot rr = load_rot(rot + r * 9, g);
+    const double fy = canon_fy(g, i);
+    const double m = 1e-7;
+    double iV[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) iV[d] = fabs(rr.V[d]) > 1e-30 ? 1.0 / rr.V[d] : 0.0;
+    const double top[3] = {(double)(g.vD - 1), (double)(g.vH - 1), (double)(g.vW - 1)};
+    int a[2], b[2], pa[2], pb[2];
+#pragma unroll
+    for (int sh = 0; sh < 2; ++sh) {
+      const double s = sh ? -1.0 : 1.0;
+      const double ux = __dmul_rn(rr.U[2], fy);
+      const double C0 = __fma_rn(s * rr.U[0], fy, g.bz), C1 = __fma_rn(s * rr.U[1], fy, g.by);
+      // e = +1: certainly inside (margins inward); e = -1: possibly inside
+#pragma unroll
+      for (int ce = 0; ce < 2; ++ce) {
+        const double e = ce ? -1.0 : 1.0;
+        double jlo = 0.0, jhi = (double)(g.Wr - 1);
+        // the Friedel half: the extraction decision flips within tolx of x' = 0
+        lin_interval(s * ux, s * rr.V[2], s * iV[2], e * (g.tolx + m), 1e300, jlo, jhi);
+        lin_interval(C0, s * rr.V[0], s * iV[0], e * m, top[0] - e * m, jlo, jhi);
+        lin_interval(C1, s * rr.V[1], s * iV[1], e * m, top[1] - e * m, jlo, jhi);
+        lin_interval(s * ux, s * rr.V[2], s * iV[2], e * m, top[2] - e * m, jlo, jhi);
+        if (g.use_mask) {
+          // in_disc (float32) holds wherever the exact radius is 1e-6 below
+          // fmax, and fails wherever it is 1e-6 above
+          const double fm = (double)g.fmax * (1.0 - e * 1e-6);
+          const double fyv = (double)(i - g.half_H) * (double)g.inv_H32;
+          const double rem = fm * fm - fyv * fyv;
+          if (rem < 0.0 || fm < 0.0) jhi = -1e300;
+          else jhi = fmin(jhi, sqrt(rem) / (double)g.inv_W32 - e * 1e-6);
+        }
+        int lo = 0, hi = -1;
+        if (jlo <= jhi) {
+          lo = max((int)ceil(jlo + e * 1e-7), 0);
+          hi = min((int)floor(jhi - e * 1e-7), g.Wr - 1);
+        }
+        if (ce == 0) {
+          a[sh] = lo;
+          b[sh] = hi;
+        } else {
+          pa[sh] = lo;
+          pb[sh] = hi;
+        }
+      }
+    }
+    // sort the two certain intervals, drop empty ones, merge overlaps
+    int a1 = a[0], b1 = b[0], a2 = a[1], b2 = b[1];
+    if (a1 > b1) { a1 = a2; b1 = b2; a2 = 0; b2 = -1; }
+    if (a2 <= b2 && a2 < a1) { int ta = a1, tb = b1; a1 = a2; b1 = b2; a2 = ta; b2 = tb; }
+    if (a2 <= b2 && a2 <= b1 + 1) { b1 = max(b1, b2); a2 = 0; b2 = -1; }
+    ZeroRow z;
+    z.base = (long long)r * g.ostride + (long long)i * g.Wr;
+    z.r = (int)r;
+    z.i = i;
+    z.p[0] = pa[0];
+    z.p[1] = pb[0];
+    z.p[2] = pa[1];
+    z.p[3] = pb[1];
+    if (a1 > b1) {  // nothing certain: the whole row
+      z.a1 = 0; z.n1 = g.Wr; z.a2 = 0; z.n2 = 0; z.a3 = 0; z.n3 = 0;
+    } else if (a2 > b2) {
+      z.a1 = 0; z.n1 = a1; z.a2 = b1 + 1; z.n2 = g.Wr - 1 - b1; z.a3 = 0; z.n3 = 0;
+    } else {
+      z.a1 = 0; z.n1 = a1; z.a2 = b1 + 1; z.n2 = a2 - b1 - 1; z.a3 = b2 + 1; z.n3 = g.Wr - 1 - b2;
+    }
+    total = z.n1 + z.n2 + z.n3;
+    s_zr[wave][lane] = z;
+  }
+  // the rows' zero pixels 64 at a time, consecutive lanes on consecutive
+  // pixels of a row (coalesced stores); a lane finds its row from marks that
+  // the rows write at their first pixel of the step, carried to the pixels
+  // that follow by a running maximum over the wave
+  const int incl = wave_incl_scan(total);
+  const int excl = incl - total;
+  const int total_px = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(incl, 63));
+  unsigned *marks = s_mark[wave];
+  for (int pb = 0; pb < total_px; pb += 64) {
+    const unsigned long long hb = __ballot(excl <= pb && pb < incl);
+    const int k0 = (int)__builtin_ctzll(hb);
+    __builtin_amdgcn_wave_barrier();  // the previous step's marks have been read
+    marks[lane] = lane == 0 ? (unsigned)(k0 + 1) : 0u;
+    if (total > 0 && excl > pb && excl < pb + 64) marks[excl - pb] = (unsigned)(lane + 1);
+    __builtin_amdgcn_wave_barrier();
+    const int k = (int)wave_incl_max(marks[lane]) - 1;
+    const int ek = __shfl(excl, k, 64);  // (every lane active)
+    const int p = pb + lane;
+    if (p < total_px) {
+      const ZeroRow z = s_zr[wave][k];
+      const int qq = p - ek;
+      int j;
+      if (qq < z.n1) j = z.a1 + qq;
+      else if (qq < z.n1 + z.n2) j = z.a2 + (qq - z.n1);
+      else j = z.a3 + (qq - z.n1 - z.n2);
+      bool zero = true;
+      if ((j >= z.p[0] && j <= z.p[1]) || (j >= z.p[2] && j <= z.p[3])) {
+        // possibly inside: the canonical test of the pixel gather (a few
+        // columns at the ends of the certain intervals)
+        const Rot rj = load_rot(rot + (int64_t)z.r * 9, g);
+        zero = !(in_disc(g, z.i, j) && inside(g, slice_coord<false>(g, rj, z.i, j)));
+      }
+      if (zero)
+        for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
+    }
+  }
+}
+
+__device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
+                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block,
+                                                 int n_chan = 1) {
+  __shared__ ZeroRow s_zr[4][64];
+  __shared__ unsigned s_mark[4][64];
+  zero_rows<4>(g, rot, n_rows, out, block, n_chan, s_zr, s_mark);
+}
+
+// A row block of the zero pass run by a block of the fused gather launch
+// (rot == nullptr: the block runs its brick unit instead).
+struct ZeroArgs {
+  const float *rot;
+  int64_t n_rows;  // slice rows of the stack
+  int64_t block;   // row block: rows [block * 64 * waves, (block + 1) * 64 * waves)
+};
+
+// ---------------------------------------------------------------------------
 // The brick kernel (gather, adjoint or insertion)
 // ---------------------------------------------------------------------------
 // One unit (a brick and a range of rotations) of the gather, the adjoint or
 // the insertion: shared by the one-call kernel and the batched persistent one.
+// A gather block may run a row block of the zero pass instead (za.rot set).
 template <int BZ, int BY, int BX, int MODE, bool MASK, int NCH = 1>
 __device__ __forceinline__ void brick_body(
     const f2v *__restrict__ src, const SliceGeom &g, const BrickGrid &bg, const PlaneRec *__restrict__ pre,
     const PlaneTest *__restrict__ ptest, const Unit &un, int n_rot, f2v *__restrict__ out,
     float *__restrict__ wout, f2v *__restrict__ plane, float *__restrict__ wplane,
     u64 *__restrict__ segbuf, float *__restrict__ halo, const ScaleArgs &sa, int accumulate,
-    int *__restrict__ touched) {
+    int *__restrict__ touched, const ZeroArgs &za = ZeroArgs{nullptr, 0, 0}) {
   typedef Cfg<BZ, BY, BX, MODE> C;
   constexpr int NW = C::NW, CH = C::CH, SX = C::SX, SY = C::SY, NT = C::NT;
   constexpr bool SCATTER = MODE != kGather;
@@ -543,6 +713,15 @@ __device__ __forceinline__ void brick_body(
   __shared__ Smem sm;
   u64 *s_tile = sm.tile;
 
+  if (MODE == kGather && za.rot) {
+    // the zero rows' tables alias the start of the block's LDS
+    constexpr size_t kZeroTab = (size_t)NW * 64 * sizeof(ZeroRow);
+    static_assert(sizeof(Smem) >= kZeroTab + (size_t)NW * 64 * sizeof(unsigned), "zero tables fit the LDS");
+    char *base = reinterpret_cast<char *>(&sm);
+    zero_rows<NW>(g, za.rot, za.n_rows, out, za.block, NCH, reinterpret_cast<ZeroRow(*)[64]>(base),
+                  reinterpret_cast<unsigned(*)[64]>(base + kZeroTab));
+    return;
+  }
   if (SCATTER && sa.guard && *sa.flag == 0u) return;  // exact redo not needed
   const int brick = un.brick;
   if (SCATTER && sa.crossed && !sa.crossed[brick]) return;  // no plane reaches it
@@ -1141,6 +1320,27 @@ __global__ __launch_bounds__((Cfg<BZ, BY, BX, MODE>::T), 4) void brick_kernel(
                                           halo, sa, accumulate, touched);
 }
 
+// The gather in one launch: the plan's U brick units and the zero pass's Z
+// row blocks (64 rows per wave), the row blocks spread evenly over the grid
+// so that their stores overlap the brick units' VALU-bound work.  Block b
+// runs row block k(b) - 1 where k(b) = floor((b + 1) Z / (U + Z)) steps up
+// at b, and otherwise unit b - k(b).  The zero pass writes exactly the pixels
+// that no unit writes, so the order of the blocks does not matter.
+template <int BZ, int BY, int BX, bool MASK, int NCH = 1>
+__global__ __launch_bounds__((Cfg<BZ, BY, BX, kGather>::T), 4) void gather_kernel(
+    const f2v *__restrict__ src, SliceGeom g, BrickGrid bg, const PlaneRec *__restrict__ pre,
+    const PlaneTest *__restrict__ ptest, const Unit *__restrict__ units, int n_units, int n_rot,
+    const float *__restrict__ rot, int n_zero, f2v *__restrict__ out) {
+  const int64_t b = blockIdx.x, tot = (int64_t)n_units + n_zero;
+  const int64_t k = (b + 1) * n_zero / tot;
+  const bool zero = k > b * n_zero / tot;
+  const ZeroArgs za = {zero ? rot : nullptr, (int64_t)n_rot * g.H, k - 1};
+  const Unit un = zero ? Unit{0, 0, 0} : units[b - k];
+  const ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr, nullptr};
+  brick_body<BZ, BY, BX, kGather, MASK, NCH>(src, g, bg, pre, ptest, un, n_rot, out, nullptr, nullptr, nullptr,
+                                             nullptr, nullptr, sa, 0, nullptr, za);
+}
+
 // Sums the rotation segments of a split brick (integer sums: the result does
 // not depend on the order) and flushes the tile like an unsplit brick; one
 // thread per tile voxel, blockIdx.x over the split bricks.
@@ -1459,171 +1659,6 @@ __global__ __launch_bounds__(256) void uncrossed_fill_kernel(SliceGeom g, BrickG
   }
 }
 
-// ---------------------------------------------------------------------------
-// Gather zero pass: writes 0 to every pixel that the brick gather does not
-// write, i.e. whose canonical coordinate (Friedel half included) lies
-// outside the volume or, with fftfreq_max, outside the disc.  One lane per
-// slice row computes, for each Friedel half, the interval of columns whose
-// canonical coordinates are inside the volume by a margin of 1e-7 voxel,
-// whose half decision is certain and (with fftfreq_max) whose disc test is
-// certain, and the wider interval of columns that are possibly inside; the
-// row's other columns (at most three runs) get a zero, except that a column
-// in a possible interval is first tested with the canonical coordinate, as
-// the pixel gather evaluates it.  The pass therefore writes exactly the
-// pixels no brick writes, and may run before, after or between them.
-// ---------------------------------------------------------------------------
-struct ZeroRow {
-  long long base;
-  int a1, n1, a2, n2, a3, n3;  // three runs of columns [a, a + n)
-  int r, i;                    // rotation and slice row
-  int p[4];                    // columns possibly inside, per Friedel half: [p0, p1], [p2, p3]
-};
-
-__device__ __forceinline__ void lin_interval(double A, double B, double iB, double lo, double hi,
-                                             double &jlo, double &jhi) {
-  // lo <= A + j B <= hi, iB = 1 / B (the bounds carry a 1e-7 margin, far
-  // above the rounding of the reciprocal)
-  if (fabs(B) > 1e-30) {
-    const double t1 = (lo - A) * iB, t2 = (hi - A) * iB;
-    jlo = fmax(jlo, fmin(t1, t2));
-    jhi = fmin(jhi, fmax(t1, t2));
-  } else if (A < lo || A > hi) {
-    jhi = -1e300;
-  }
-}
-
-// Rows [block * NWZ * 64, (block + 1) * NWZ * 64) of the stack, one per lane
-// of the block's NWZ waves, with per-wave LDS tables s_zr (row records) and
-// s_mark (marks).
-template <int NWZ>
-__device__ __forceinline__ void zero_rows(const SliceGeom &g, const float *__restrict__ rot, int64_t n_rows,
-                                          f2v *__restrict__ out, int64_t block, int n_chan,
-                                          ZeroRow (*s_zr)[64], unsigned (*s_mark)[64]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t t = (block * NWZ + wave) * 64 + lane;
-  int total = 0;
-  if (t < n_rows) {
-    const int64_t r = t / g.H;
-    const int i = (int)(t - r * g.H);
-    const Rot rr = load_rot(rot + r * 9, g);
-    const double fy = canon_fy(g, i);
-    const double m = 1e-7;
-    double iV[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) iV[d] = fabs(rr.V[d]) > 1e-30 ? 1.0 / rr.V[d] : 0.0;
-    const double top[3] = {(double)(g.vD - 1), (double)(g.vH - 1), (double)(g.vW - 1)};
-    int a[2], b[2], pa[2], pb[2];
-#pragma unroll
-    for (int sh = 0; sh < 2; ++sh) {
-      const double s = sh ? -1.0 : 1.0;
-      const double ux = __dmul_rn(rr.U[2], fy);
-      const double C0 = __fma_rn(s * rr.U[0], fy, g.bz), C1 = __fma_rn(s * rr.U[1], fy, g.by);
-      // e = +1: certainly inside (margins inward); e = -1: possibly inside
-#pragma unroll
-      for (int ce = 0; ce < 2; ++ce) {
-        const double e = ce ? -1.0 : 1.0;
-        double jlo = 0.0, jhi = (double)(g.Wr - 1);
-        // the Friedel half: the extraction decision flips within tolx of x' = 0
-        lin_interval(s * ux, s * rr.V[2], s * iV[2], e * (g.tolx + m), 1e300, jlo, jhi);
-        lin_interval(C0, s * rr.V[0], s * iV[0], e * m, top[0] - e * m, jlo, jhi);
-        lin_interval(C1, s * rr.V[1], s * iV[1], e * m, top[1] - e * m, jlo, jhi);
-        lin_interval(s * ux, s * rr.V[2], s * iV[2], e * m, top[2] - e * m, jlo, jhi);
-        if (g.use_mask) {
-          // in_disc (float32) holds wherever the exact radius is 1e-6 below
-          // fmax, and fails wherever it is 1e-6 above
-          const double fm = (double)g.fmax * (1.0 - e * 1e-6);
-          const double fyv = (double)(i - g.half_H) * (double)g.inv_H32;
-          const double rem = fm * fm - fyv * fyv;
-          if (rem < 0.0 || fm < 0.0) jhi = -1e300;
-          else jhi = fmin(jhi, sqrt(rem) / (double)g.inv_W32 - e * 1e-6);
-        }
-        int lo = 0, hi = -1;
-        if (jlo <= jhi) {
-          lo = max((int)ceil(jlo + e * 1e-7), 0);
-          hi = min((int)floor(jhi - e * 1e-7), g.Wr - 1);
-        }
-        if (ce == 0) {
-          a[sh] = lo;
-          b[sh] = hi;
-        } else {
-          pa[sh] = lo;
-          pb[sh] = hi;
-        }
-      }
-    }
-    // sort the two certain intervals, drop empty ones, merge overlaps
-    int a1 = a[0], b1 = b[0], a2 = a[1], b2 = b[1];
-    if (a1 > b1) { a1 = a2; b1 = b2; a2 = 0; b2 = -1; }
-    if (a2 <= b2 && a2 < a1) { int ta = a1, tb = b1; a1 = a2; b1 = b2; a2 = ta; b2 = tb; }
-    if (a2 <= b2 && a2 <= b1 + 1) { b1 = max(b1, b2); a2 = 0; b2 = -1; }
-    ZeroRow z;
-    z.base = (long long)r * g.ostride + (long long)i * g.Wr;
-    z.r = (int)r;
-    z.i = i;
-    z.p[0] = pa[0];
-    z.p[1] = pb[0];
-    z.p[2] = pa[1];
-    z.p[3] = pb[1];
-    if (a1 > b1) {  // nothing certain: the whole row
-      z.a1 = 0; z.n1 = g.Wr; z.a2 = 0; z.n2 = 0; z.a3 = 0; z.n3 = 0;
-    } else if (a2 > b2) {
-      z.a1 = 0; z.n1 = a1; z.a2 = b1 + 1; z.n2 = g.Wr - 1 - b1; z.a3 = 0; z.n3 = 0;
-    } else {
-      z.a1 = 0; z.n1 = a1; z.a2 = b1 + 1; z.n2 = a2 - b1 - 1; z.a3 = b2 + 1; z.n3 = g.Wr - 1 - b2;
-    }
-    total = z.n1 + z.n2 + z.n3;
-    s_zr[wave][lane] = z;
-  }
-  // the rows' zero pixels 64 at a time, consecutive lanes on consecutive
-  // pixels of a row (coalesced stores); a lane finds its row from marks that
-  // the rows write at their first pixel of the step, carried to the pixels
-  // that follow by a running maximum over the wave
-  const int incl = wave_incl_scan(total);
-  const int excl = incl - total;
-  const int total_px = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(incl, 63));
-  unsigned *marks = s_mark[wave];
-  for (int pb = 0; pb < total_px; pb += 64) {
-    const unsigned long long hb = __ballot(excl <= pb && pb < incl);
-    const int k0 = (int)__builtin_ctzll(hb);
-    __builtin_amdgcn_wave_barrier();  // the previous step's marks have been read
-    marks[lane] = lane == 0 ? (unsigned)(k0 + 1) : 0u;
-    if (total > 0 && excl > pb && excl < pb + 64) marks[excl - pb] = (unsigned)(lane + 1);
-    __builtin_amdgcn_wave_barrier();
-    const int k = (int)wave_incl_max(marks[lane]) - 1;
-    const int ek = __shfl(excl, k, 64);  // (every lane active)
-    const int p = pb + lane;
-    if (p < total_px) {
-      const ZeroRow z = s_zr[wave][k];
-      const int qq = p - ek;
-      int j;
-      if (qq < z.n1) j = z.a1 + qq;
-      else if (qq < z.n1 + z.n2) j = z.a2 + (qq - z.n1);
-      else j = z.a3 + (qq - z.n1 - z.n2);
-      bool zero = true;
-      if ((j >= z.p[0] && j <= z.p[1]) || (j >= z.p[2] && j <= z.p[3])) {
-        // possibly inside: the canonical test of the pixel gather (a few
-        // columns at the ends of the certain intervals)
-        const Rot rj = load_rot(rot + (int64_t)z.r * 9, g);
-        zero = !(in_disc(g, z.i, j) && inside(g, slice_coord<false>(g, rj, z.i, j)));
-      }
-      if (zero)
-        for (int ch = 0; ch < n_chan; ++ch) out[z.base + (int64_t)ch * g.P + j] = f2v{0.0f, 0.0f};
-    }
-  }
-}
-
-__device__ __forceinline__ void gather_zero_body(const SliceGeom &g, const float *__restrict__ rot,
-                                                 int64_t n_rows, f2v *__restrict__ out, int64_t block,
-                                                 int n_chan = 1) {
-  __shared__ ZeroRow s_zr[4][64];
-  __shared__ unsigned s_mark[4][64];
-  zero_rows<4>(g, rot, n_rows, out, block, n_chan, s_zr, s_mark);
-}
-
-__global__ __launch_bounds__(256) void gather_zero_kernel(SliceGeom g, const float *__restrict__ rot,
-                                                          int64_t n_rows, f2v *__restrict__ out, int n_chan) {
-  gather_zero_body(g, rot, n_rows, out, (int64_t)blockIdx.x, n_chan);
-}
 
 // x=0-plane Friedel mirror of the insertion, from the plane of direct splats
 // (slice_insertion/_insert_central_slices_rfft_3d.py:350-404):
@@ -2565,35 +2600,30 @@ bool engine_supports(const SliceGeom &g, int64_t n_rot) {
 
 namespace {
 
-// The gather's launches after the zero pass: plane records into pre/ptest,
-// then the brick kernel over the plan's units.
+// The gather: plane records into pre/ptest (when the plan has units), then
+// one launch of the plan's brick units and the zero pass's row blocks.
+template <int NCH>
 int gather_launch(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
                   PlaneRec *pre, PlaneTest *ptest, const Unit *units, int n_units, hipStream_t s) {
-  hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
-                     (int)n_rot, pre, ptest, (unsigned *)nullptr);
-  if (int st = check_launch("plane_rec_kernel")) return st;
+  constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;  // one slice row per thread of a zero block
+  const int64_t n_zero = (n_rot * g.H + T - 1) / T;
+  if (n_zero + n_units >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
+  if (n_units > 0) {
+    hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
+                       (int)n_rot, pre, ptest, (unsigned *)nullptr);
+    if (int st = check_launch("plane_rec_kernel")) return st;
+  }
   const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
-  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr, nullptr};
-  constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
+  const dim3 grid((unsigned)(n_units + n_zero));
   if (g.use_mask)
-    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true>), dim3((unsigned)n_units), dim3(T),
-                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, units, (int)n_rot, out,
-                       (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (float *)nullptr, sa, 0, (int *)nullptr);
+    hipLaunchKernelGGL((gather_kernel<kGBZ, kGBY, kGBX, true, NCH>), grid, dim3(T), 0, s, vol, g, bg,
+                       (const PlaneRec *)pre, (const PlaneTest *)ptest, units, n_units, (int)n_rot, rot,
+                       (int)n_zero, out);
   else
-    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false>), dim3((unsigned)n_units),
-                       dim3(T), 0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, units,
-                       (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr,
-                       (u64 *)nullptr, (float *)nullptr, sa, 0, (int *)nullptr);
-  return check_launch("brick_kernel");
-}
-
-int gather_zero(const SliceGeom &g, const float *rot, int64_t n_rot, f2v *out, hipStream_t s) {
-  const int64_t n_rows = n_rot * g.H;
-  const int64_t nblk = (n_rows + 255) / 256;
-  if (nblk >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
-  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out, 1);
-  return check_launch("gather_zero_kernel");
+    hipLaunchKernelGGL((gather_kernel<kGBZ, kGBY, kGBX, false, NCH>), grid, dim3(T), 0, s, vol, g, bg,
+                       (const PlaneRec *)pre, (const PlaneTest *)ptest, units, n_units, (int)n_rot, rot,
+                       (int)n_zero, out);
+  return check_launch("gather_kernel");
 }
 
 // Scratch of the allocation-free extraction (engine_extract_noalloc): the
@@ -2622,15 +2652,18 @@ int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t
                    hipStream_t s) {
   if (int st = check_engine(g, n_rot)) return st;
   if (n_rot == 0) return TT_OK;
-  if (int st = gather_zero(g, rot, n_rot, out, s)) return st;
   PlanEntry pe;
   if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
-  if (pe.n_units == 0) return TT_OK;
   Scratch sc(s);
-  const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
-  if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
-  return gather_launch(g, vol, rot, n_rot, out, (PlaneRec *)sc.p, (PlaneTest *)((char *)sc.p + off_test),
-                       pe.units, pe.n_units, s);
+  PlaneRec *pre = nullptr;
+  PlaneTest *ptest = nullptr;
+  if (pe.n_units > 0) {
+    const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
+    if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
+    pre = (PlaneRec *)sc.p;
+    ptest = (PlaneTest *)((char *)sc.p + off_test);
+  }
+  return gather_launch<1>(g, vol, rot, n_rot, out, pre, ptest, pe.units, pe.n_units, s);
 }
 
 int engine_extract_noalloc(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
@@ -2674,10 +2707,8 @@ int engine_extract_noalloc(const SliceGeom &g, const f2v *vol, const float *rot,
       nd.units = pe.h_units.get();
     }
   }
-  if (int st = gather_zero(g, rot, n_rot, out, s)) return st;
-  if (pe.n_units == 0) return TT_OK;
-  return gather_launch(g, vol, rot, n_rot, out, (PlaneRec *)(base + kNoAllocPreOff),
-                       (PlaneTest *)(base + kNoAllocTestOff), (const Unit *)base, pe.n_units, s);
+  return gather_launch<1>(g, vol, rot, n_rot, out, (PlaneRec *)(base + kNoAllocPreOff),
+                          (PlaneTest *)(base + kNoAllocTestOff), (const Unit *)base, pe.n_units, s);
 }
 
 // Two channels of a multichannel volume (channel stride g.vstride) into a
@@ -2687,36 +2718,18 @@ int engine_extract2(const SliceGeom &g, const f2v *vol, const float *rot, int64_
                     hipStream_t s) {
   if (int st = check_engine(g, n_rot)) return st;
   if (n_rot == 0) return TT_OK;
-  const int64_t n_rows = n_rot * g.H;
-  const int64_t nblk = (n_rows + 255) / 256;
-  if (nblk >= ((int64_t)1 << 31)) return fail_invalid("too many slice rows for one call");
-  hipLaunchKernelGGL(gather_zero_kernel, dim3((unsigned)nblk), dim3(256), 0, s, g, rot, n_rows, out, 2);
-  if (int st = check_launch("gather_zero_kernel")) return st;
   PlanEntry pe;
   if (int st = get_plan(g, kGBZ, kGBY, kGBX, false, n_rot, pe)) return st;
-  if (pe.n_units == 0) return TT_OK;
   Scratch sc(s);
-  const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
-  if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
-  PlaneRec *pre = (PlaneRec *)sc.p;
-  PlaneTest *ptest = (PlaneTest *)((char *)sc.p + off_test);
-  hipLaunchKernelGGL(plane_rec_kernel, dim3((unsigned)((n_rot + 255) / 256)), dim3(256), 0, s, g, rot,
-                     (int)n_rot, pre, ptest, (unsigned *)nullptr);
-  if (int st = check_launch("plane_rec_kernel")) return st;
-  const BrickGrid bg = grid_of(g, kGBZ, kGBY, kGBX);
-  ScaleArgs sa = {nullptr, nullptr, nullptr, 0, 0, 0, nullptr, nullptr};
-  constexpr int T = Cfg<kGBZ, kGBY, kGBX, kGather>::T;
-  if (g.use_mask)
-    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, true, 2>), dim3((unsigned)pe.n_units), dim3(T),
-                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
-                       (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (float *)nullptr, sa, 0, (int *)nullptr);
-  else
-    hipLaunchKernelGGL((brick_kernel<kGBZ, kGBY, kGBX, kGather, false, 2>), dim3((unsigned)pe.n_units), dim3(T),
-                       0, s, vol, g, bg, (const PlaneRec *)pre, (const PlaneTest *)ptest, (const Unit *)pe.units,
-                       (int)n_rot, out, (float *)nullptr, (f2v *)nullptr, (float *)nullptr, (u64 *)nullptr,
-                       (float *)nullptr, sa, 0, (int *)nullptr);
-  return check_launch("brick_kernel");
+  PlaneRec *pre = nullptr;
+  PlaneTest *ptest = nullptr;
+  if (pe.n_units > 0) {
+    const size_t off_test = ((size_t)n_rot * sizeof(PlaneRec) + 255) & ~(size_t)255;
+    if (int st = sc.alloc(off_test + (size_t)n_rot * sizeof(PlaneTest))) return st;
+    pre = (PlaneRec *)sc.p;
+    ptest = (PlaneTest *)((char *)sc.p + off_test);
+  }
+  return gather_launch<2>(g, vol, rot, n_rot, out, pre, ptest, pe.units, pe.n_units, s);
 }
 
 int engine_adjoint(const SliceGeom &g, const f2v *gS, const float *rot, int64_t n_rot, f2v *gV,
