@@ -208,7 +208,7 @@ def test_bench_traffic_parsing(tmp_path):
     spec.loader.exec_module(mod)
     full = "void tt::(anonymous namespace)::brick_kernel<8, 16, 16, 0, false, 1>(float __vector(2) const*, tt::SliceGeom)"
     assert mod._kernel_name(full) == "brick_kernel<8, 16, 16, 0, false, 1>"
-    assert mod._kernel_name("tt::(anonymous namespace)::gather_zero_kernel(tt::SliceGeom, long)") == "gather_zero_kernel"
+    assert mod._kernel_name("tt::(anonymous namespace)::plane_rec_kernel(tt::SliceGeom, long)") == "plane_rec_kernel"
     seq = mod.CALL_KERNELS["extract"]
     hdr = '"Dispatch_Id","Kernel_Name","Counter_Name","Counter_Value"\n'
     lines = []
@@ -222,4 +222,4 @@ def test_bench_traffic_parsing(tmp_path):
     (tmp_path / "run_counter_collection.csv").write_text(hdr + "".join(lines))
     rows = mod._pmc_dispatches(str(tmp_path), "FETCH_SIZE")
     last = mod._last_call(rows, seq)
-    assert [r[1] for r in last] == seq and [r[2] for r in last] == [200.0, 201.0, 202.0]
+    assert [r[1] for r in last] == seq and [r[2] for r in last] == [200.0 + k for k in range(len(seq))]
