@@ -1584,7 +1584,9 @@ inline int band_blocks(const SliceGeom &g) { return std::max(1, std::min((g.P + 
 // Groups of sampled slices of the one-call band-maximum pass (grid y): four,
 // when there are 32 slices or more.  (More groups make the blocks' global
 // atomicMax on the 64 band words contend: 15 groups measured 30 us against
-// 22 us for one at 256^3 x 4096.)
+// 22 us for one at 256^3 x 4096, and 8 groups with each block's atomics
+// skipped where a larger maximum was already stored 29 us against 16 us for
+// four.)
 inline int band_groups(const SliceGeom &g, int64_t n_rot) {
   (void)g;
   const int64_t nsl = (n_rot + band_stride(n_rot) - 1) / band_stride(n_rot);
