@@ -193,9 +193,12 @@ template <int BZ, int BY, int BX, int MODE>
 struct Cfg {
   static constexpr int NW = 8;                          // waves per block
   static constexpr int T = NW * 64;
-  // pixels per chunk (measured at 256^3 x 4096: one pixel per lane makes the
-  // scatter 20 % slower, four spill registers)
-  static constexpr int CH = 2;
+  // pixels per chunk: three for the adjoint, two otherwise (measured at
+  // 256^3 x 4096: adjoint 0.991 ms with three against 1.007 ms with two;
+  // three cost the insertion 2 % and the gather 5 %, where they spill 68
+  // bytes per lane; four slowed both; one pixel per lane makes the scatter
+  // 20 % slower)
+  static constexpr int CH = (MODE == kAdjoint) ? 3 : 2;
   // tile: the brick's cells lo..lo+B-1 and their corners, B + 1 voxels per axis
   static constexpr int SX = BX + 1, SY = BY + 1, SZ = BZ + 1;
   static constexpr int NT = SX * SY * SZ;
