@@ -90,10 +90,10 @@ CALL_KERNELS = {
                 "brick_kernel<16, 16, 8, 1, false, 1>", "split_combine_kernel<16, 16, 8, 1>",
                 "halo_combine_kernel<16, 16, 8, 1>"],
     "insert_256": ["fill_zero_kernel<unsigned int>", "band_max_kernel", "plane_rec_kernel",
-                   "brick_kernel<16, 8, 8, 2, false, 1>", "split_combine_kernel<16, 8, 8, 2>",
-                   "halo_combine_kernel<16, 8, 8, 2>", "maxabs_chunks_kernel",
-                   "brick_kernel<16, 8, 8, 2, false, 1>", "split_combine_kernel<16, 8, 8, 2>",
-                   "halo_combine_kernel<16, 8, 8, 2>", "mirror_kernel"],
+                   "brick_kernel<8, 8, 16, 2, false, 1>", "split_combine_kernel<8, 8, 16, 2>",
+                   "halo_combine_kernel<8, 8, 16, 2>", "maxabs_chunks_kernel",
+                   "brick_kernel<8, 8, 16, 2, false, 1>", "split_combine_kernel<8, 8, 16, 2>",
+                   "halo_combine_kernel<8, 8, 16, 2>", "mirror_kernel"],
     "insert_512_tilts": ["bin_zero_kernel", "bin_count_kernel<true>", "bin_scan_kernel", "bin_fill_kernel<true>",
                          "bin_brick_kernel<true>", "mirror_kernel"],
 }
