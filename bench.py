@@ -85,10 +85,10 @@ def parse():
 CALL_KERNELS = {
     "extract": ["plane_rec_kernel", "gather_kernel<8, 16, 16, false, 1>"],
     "adjoint": ["fill_zero_kernel<unsigned int>", "band_max_kernel", "plane_rec_kernel",
-                "brick_kernel<16, 16, 8, 1, false, 1>", "split_combine_kernel<16, 16, 8, 1>",
-                "halo_combine_kernel<16, 16, 8, 1>", "maxabs_chunks_kernel",
-                "brick_kernel<16, 16, 8, 1, false, 1>", "split_combine_kernel<16, 16, 8, 1>",
-                "halo_combine_kernel<16, 16, 8, 1>"],
+                "brick_kernel<16, 8, 16, 1, false, 1>", "split_combine_kernel<16, 8, 16, 1>",
+                "halo_combine_kernel<16, 8, 16, 1>", "maxabs_chunks_kernel",
+                "brick_kernel<16, 8, 16, 1, false, 1>", "split_combine_kernel<16, 8, 16, 1>",
+                "halo_combine_kernel<16, 8, 16, 1>"],
     "insert_256": ["fill_zero_kernel<unsigned int>", "band_max_kernel", "plane_rec_kernel",
                    "brick_kernel<8, 8, 16, 2, false, 1>", "split_combine_kernel<8, 8, 16, 2>",
                    "halo_combine_kernel<8, 8, 16, 2>", "maxabs_chunks_kernel",

@@ -100,7 +100,7 @@ def test_batched_extract_and_insert_match_one_call_and_replay(device):
     worst = 0.0
     for k in range(len(vols)):
         (va, wa), (vb, wb) = refi[k], outi[k]
-        if bi.tile_shapes[k] == (16, 8, 8):
+        if bi.tile_shapes[k] == (8, 8, 16):
             assert torch.equal(va, vb) and torch.equal(wa, wb), f"item {k}: batched insertion differs"
         e = max(rel_l2(vb.cpu().numpy(), va.cpu().numpy()), rel_l2(wb.cpu().numpy(), wa.cpu().numpy()))
         worst = max(worst, e)

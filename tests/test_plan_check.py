@@ -26,7 +26,7 @@ def _check(kind, vol, img, n_rot, fmax=-1.0):
 
 
 def _tile(kind):
-    return {GATHER: (8, 16, 16), ADJOINT: (16, 16, 8), INSERT: (16, 8, 8), GATHER8: (8, 8, 8), INSERT8: (8, 8, 8)}[kind]
+    return {GATHER: (8, 16, 16), ADJOINT: (16, 8, 16), INSERT: (8, 8, 16), GATHER8: (8, 8, 8), INSERT8: (8, 8, 8)}[kind]
 
 
 @pytest.mark.parametrize("kind,vol,img,n_rot,fmax", [
