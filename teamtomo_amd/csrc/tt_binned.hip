@@ -3,9 +3,9 @@
 // volume (61 slices of 512^2 into 512^3: ~196 pixels per 4x8x64 brick).
 //
 // The brick engine discovers each brick's slices itself (plane tests, FP64
-// item frames, row and chunk lists), a chain of dependent steps that costs
+// item frames, row records, chunk steps), a chain of dependent steps that costs
 // more than the few pixels it then scatters; its upper-shell halos also make
-// a second pass read and rewrite every cache line of the volume.  Here the
+// a second pass read and rewrite much of the volume.  Here the
 // pixels come to the bricks instead:
 //   1. bin_count_kernel: one lane per pixel evaluates the canonical FP64
 //      coordinate (the pixel-atomic kernels' code, tt_common.h) and counts it

@@ -6,13 +6,13 @@
 namespace tt {
 
 // Geometries the engine handles: fewer than 2^30 rotations, slice height
-// below 32768 and rfft width below 65536 (the packed chunk list and item
-// records hold the row and column in 16 bits).
+// below 32768 and rfft width below 65536 (the item and row records hold the
+// row and column in 16 bits).
 bool engine_supports(const SliceGeom &g, int64_t n_rot);
 
 // slices = extraction of `vol` at the rotations; writes every pixel of the
-// n_rot slices (a zero pass for pixels outside the volume or the disc, then
-// the brick gather for the rest).
+// n_rot slices in one launch (the brick gather, and zero-row blocks for the
+// pixels outside the volume or the disc).
 int engine_extract(const SliceGeom &g, const f2v *vol, const float *rot, int64_t n_rot, f2v *out,
                    hipStream_t s);
 

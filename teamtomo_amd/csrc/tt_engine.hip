@@ -8,7 +8,8 @@
 //  gather : the brick's voxels (cells + one upper halo voxel per axis) are
 //           copied into LDS once and every slice crossing the brick is served
 //           from there; pixels outside the volume or the fftfreq_max disc are
-//           written by a row-interval zero pass that runs first.
+//           zeroed by row blocks spread over the same launch, which write
+//           exactly the pixels no brick writes.
 //  scatter: the brick accumulates in LDS in fixed point with integer atomics
 //           (8 corners per pixel, the upper corners land in a halo shell),
 //           then stores its own voxels and passes the shell to the
@@ -695,7 +696,7 @@ __device__ __forceinline__ void brick_body(
   // 32-bit contributions of the value (re, im) go to split accumulators;
   // the insertion's weights stay 64-bit words
   constexpr bool SPLIT = SCATTER;
-  static_assert(CH >= 1 && CH <= 4, "the chunk list holds pixel counts 1 to 4");
+  static_assert(CH >= 1 && CH <= 4, "chunks of 1 to 4 pixels");
   static_assert(NCH == 1 || (NCH == 2 && MODE == kGather), "two channels: gather only");
   // rotations per item pass (two halves each): 32 where the LDS of two
   // blocks per CU has room for 64 item records per wave, else 16
