@@ -76,9 +76,14 @@ struct PlaneRec {
   int full;             // degenerate plane: every row and column is a candidate
 };
 
-// Plane test data: the unit normal of span(A, B) in FP32; w < 0 marks a
-// degenerate plane (every brick is a candidate).
-typedef float4 PlaneTest;
+// Plane test data in FP32: n = the unit normal of span(A, B), n.w < 0
+// marking a degenerate plane (every brick is a candidate); b = the column
+// dual vector B* (the FP64 one of PlaneRec rounded), which tells a brick which
+// Friedel halves of the slice can reach it (the sign of j = (x - beta) . s B*
+// over its box).
+struct PlaneTest {
+  float4 n, b;
+};
 
 // Bits of a bound on the pixels of one slice that contribute to one voxel:
 // their coordinates lie in the open cube of side 2 around it, a plane section
@@ -126,12 +131,13 @@ __device__ __forceinline__ int plane_rec_one(const SliceGeom &g, const float *__
   double n1 = A[2] * B[0] - A[0] * B[2];
   double n2 = A[0] * B[1] - A[1] * B[0];
   const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-  float4 t;
+  PlaneTest t;
   if (full || !(nn > 0.0)) {
-    t = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    t.n = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
   } else {
-    t = make_float4((float)(n0 / nn), (float)(n1 / nn), (float)(n2 / nn), 1.0f);
+    t.n = make_float4((float)(n0 / nn), (float)(n1 / nn), (float)(n2 / nn), 1.0f);
   }
+  t.b = make_float4((float)p.Bs[0], (float)p.Bs[1], (float)p.Bs[2], 0.0f);
   test[r] = t;
   return slice_count_bits(g, nn, bb);
 }
@@ -232,7 +238,7 @@ struct __align__(16) ItemRec {
   float sv[3];   // step per slice column
   int rows;      // first row i0 | row count << 16
   unsigned ref;  // reference row i_ref | reference column j_ref << 16
-  unsigned rnt;  // rotation | Friedel half (s = -1) << 30 | tie << 31
+  unsigned rnt;  // rotation | tie << 30 | Friedel half (s = -1) << 31
 };
 
 // A row of a row pass, written once by the row's lane and read by each of its
@@ -245,7 +251,7 @@ struct __align__(8) RowRec {
   float djl;      // jl - jref: the first candidate column relative to jref
   unsigned ijl;   // slice row i | first candidate column jl << 16
   int len;        // candidate columns from jl
-  unsigned rnt;   // rotation | Friedel half (s = -1) << 30 | tie << 31
+  unsigned rnt;   // rotation | tie << 30 | Friedel half (s = -1) << 31
 };
 
 constexpr float kRowMargin = 1e-4f;    // widening of the row phase's tile box (voxels)
@@ -708,7 +714,9 @@ __device__ __forceinline__ void brick_body(
   struct Smem {
     u64 tile[NWORDS];
     ItemRec item[NW][2 * kTake];
-    int cross[NW][128];  // rotations whose plane meets the brick, pending
+    // (rotation, Friedel half) entries whose plane meets the brick, pending:
+    // rotation | 2^31 for the half s = -1
+    int cross[NW][2 * kTake + 128];
     RowRec rec[NW][64];  // the rows of the current row pass
     // chunk step marks: row lane << 26 | first chunk of that row, at the
     // step slot where the row's chunks start
@@ -825,53 +833,69 @@ __device__ __forceinline__ void brick_body(
   int rd = r_beg, ncross = 0;
   bool had_items = false;  // this wave listed a slice item (the tile may have been written)
   while (have_cells) {
-    // (1a) plane test, 64 rotations per pass, up to 8 passes whose data are
-    // loaded at once: rotations whose plane meets the brick's coordinate box
-    // are listed (at most 95 pending)
-    if (ncross < kTake && rd < r_end) {
-      // all 8 passes are tested unconditionally (so that the 8 loads are in
+    // (1a) plane test, 64 rotations per pass, up to 4 passes whose data are
+    // loaded at once: a rotation whose plane meets the brick's coordinate box
+    // is listed once for each Friedel half s whose columns can reach the box,
+    // j = (x - beta) . s B* >= 0 somewhere on it (at most 2 kTake + 127
+    // entries pending).  A brick off the line j = 0 of a plane receives one
+    // half only, so the item lanes below are spent on items that exist.
+    if (ncross < 2 * kTake && rd < r_end) {
+      // all 4 passes are tested unconditionally (so that their loads are in
       // flight together); passes are then listed while the list has room and
       // the rest are tested again in the next round
-      unsigned long long hm[8];
-      float4 pt[8];
+      constexpr int kPass = 4;
+      unsigned long long hp[kPass], hn[kPass];
+      PlaneTest pt[kPass];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kPass; ++u) {
         const int r = rd + u * 64 + lane;
         pt[u] = ptest[min(r, r_end - 1)];
       }
-      __asm__ volatile("" ::: "memory");  // keep the 8 loads ahead of their use
+      __asm__ volatile("" ::: "memory");  // keep the loads ahead of their use
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kPass; ++u) {
         const int r = rd + u * 64 + lane;
-        const float4 q = pt[u];
+        const float4 q = pt[u].n, b = pt[u].b;
         const float dist = q.x * cf0 + q.y * cf1 + q.z * cf2;
         const float rad = fabsf(q.x) * hf0 + fabsf(q.y) * hf1 + fabsf(q.z) * hf2;
-        hm[u] = __ballot(r < r_end && ((q.w < 0.0f) | (fabsf(dist) <= rad + tmarg)));
+        const bool full = q.w < 0.0f;
+        const bool hit = r < r_end && (full | (fabsf(dist) <= rad + tmarg));
+        // j over the box for s = +1 spans cj -+ hj; half s reaches j >= 0
+        // when s cj + hj >= 0 (FP32, widened far beyond its rounding; an
+        // extra entry only costs an empty item)
+        const float cj = b.x * cf0 + b.y * cf1 + b.z * cf2;
+        const float hj = fabsf(b.x) * hf0 + fabsf(b.y) * hf1 + fabsf(b.z) * hf2;
+        const float jm = 1e-2f + 1e-5f * (fabsf(b.x * cf0) + fabsf(b.y * cf1) + fabsf(b.z * cf2) + hj);
+        hp[u] = __ballot(hit && (full || cj >= -hj - jm));
+        hn[u] = __ballot(hit && (full || cj <= hj + jm));
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (ncross < kTake && rd < r_end) {
-          if ((hm[u] >> lane) & 1ull) cross[ncross + rank_below(hm[u])] = rd + lane;
-          ncross += __popcll(hm[u]);
+      for (int u = 0; u < kPass; ++u) {
+        if (ncross < 2 * kTake && rd < r_end) {
+          const bool lp = (hp[u] >> lane) & 1ull, ln = (hn[u] >> lane) & 1ull;
+          const int at = ncross + rank_below(hp[u]) + rank_below(hn[u]);
+          if (lp) cross[at] = rd + lane;
+          if (ln) cross[at + (lp ? 1 : 0)] = (int)((unsigned)(rd + lane) | 0x80000000u);
+          ncross += __popcll(hp[u]) + __popcll(hn[u]);
           rd += 64;
         }
       }
     }
     if (ncross == 0) break;
     __builtin_amdgcn_wave_barrier();
-    // (1b) items: lane c handles listed rotation c & 31 and Friedel half
-    // c >> 5.  Its candidate rows and columns are those of the box: the ranges
+    // (1b) items: lane c handles listed entry c (a rotation and a Friedel
+    // half).  Its candidate rows and columns are those of the box: the ranges
     // of m = (x - beta) . s As and j = (x - beta) . s Bs over the box (a
     // superset of the box's section by the plane; rows without a pixel in
     // the box come out empty).  The item's FP32 frame is referred to the pixel
     // nearest the projection of the tile centre.
-    const int take = min(ncross, kTake);
-    const int cidx = lane & (kTake - 1);
-    const bool ineg = lane >= kTake;
+    const int take = min(ncross, 2 * kTake);
     bool h = false;
     ItemRec it;
-    if (cidx < take && lane < 2 * kTake) {
-      const int r = cross[cidx];
+    if (lane < take) {
+      const int ent = cross[lane];
+      const int r = ent & 0x7fffffff;
+      const bool ineg = ent < 0;
       const PlaneRec &p = pre[r];
       const double s = ineg ? -1.0 : 1.0;
       // every field is used before the branch below, so the loads are issued together
@@ -964,14 +988,17 @@ __device__ __forceinline__ void brick_body(
         }
         it.rows = (m0 + g.half_H) | ((m1 - m0 + 1) << 16);
         it.ref = ((unsigned)iref & 0xffffu) | ((unsigned)jref << 16);
-        it.rnt = (unsigned)r | ((unsigned)ineg << 30) | (rtie << 31);
+        it.rnt = (unsigned)r | (rtie << 30) | ((unsigned)ineg << 31);
       }
     }
-    {  // keep the rest of the list
+    {  // keep the rest of the list (up to 127 entries, 64 at a time)
       const int rest = ncross - take;
-      const int v = lane < rest ? cross[take + lane] : 0;
-      __builtin_amdgcn_wave_barrier();
-      if (lane < rest) cross[lane] = v;
+      for (int b0 = 0; b0 < rest; b0 += 64) {
+        const int v = b0 + lane < rest ? cross[take + b0 + lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (b0 + lane < rest) cross[b0 + lane] = v;
+        __builtin_amdgcn_wave_barrier();
+      }
       ncross = rest;
     }
     const unsigned long long bh = __ballot(h);
@@ -1088,7 +1115,9 @@ __device__ __forceinline__ void brick_body(
         const int i = (int)(row.ijl & 0xffffu);
         const int j0 = (int)(row.ijl >> 16) + q0;
         const int rr = (int)(row.rnt & 0x3fffffffu);
-        const bool neg = (row.rnt >> 30) & 1;
+        const bool neg = row.rnt >> 31;
+        // the Friedel half's sign bit, for one XOR with the imaginary part
+        const unsigned nbit = row.rnt & 0x80000000u;
         // flat index of the chunk's first pixel (i Wr + j0 < H Wr < 2^31);
         // scatter input rotation stride: g.ostride (P, or C P for one
         // channel of a multichannel stack)
@@ -1135,7 +1164,7 @@ __device__ __forceinline__ void brick_body(
             for (int q = 0; q < CH; ++q) {
               if (!own[q]) continue;
               int cell[3];
-              own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (row.rnt >> 31) & 1, i, j0 + q, lo0,
+              own[q] = exact_cell<MODE == kInsert>(g, pre, rr, neg, (row.rnt >> 30) & 1, i, j0 + q, lo0,
                                                    lo1, lo2, cell, t3[q]);
               fl[q][0] = (float)(cell[0] - HZ);
               fl[q][1] = (float)(cell[1] - HY);
@@ -1179,7 +1208,7 @@ __device__ __forceinline__ void brick_body(
             for (int q = 0; q < CH; ++q) {
               f2v val = trilerp(cv[q][0], cv[q][1], cv[q][2], cv[q][3], cv[q][4], cv[q][5], cv[q][6],
                                 cv[q][7], t3[q][0], t3[q][1], t3[q][2]);
-              if (neg) val.y = -val.y;
+              val.y = __uint_as_float(__float_as_uint(val.y) ^ nbit);  // conj on the s = -1 half
               if (own[q]) out[obase + ch * g.P + q] = val;
             }
           }
@@ -1188,7 +1217,7 @@ __device__ __forceinline__ void brick_body(
           for (int q = 0; q < CH; ++q) {
             if (!own[q]) continue;
             f2v v = vv[q];
-            if (neg) v.y = -v.y;
+            v.y = __uint_as_float(__float_as_uint(v.y) ^ nbit);
             bad = bad || !(fabsf(v.x) < vlimit && fabsf(v.y) < vlimit);
             v.x *= vscale;
             v.y *= vscale;
@@ -1243,10 +1272,7 @@ __device__ __forceinline__ void brick_body(
                     if (MODE == kInsert && wout) lds_add_fixed(&s_tile[2 * NT + o], w, kWScale);
                   }
                 }
-              unsigned inside = 0xffffffffu;
-#pragma unroll
-              for (int k = 0; k < 8; ++k) inside &= lo_window(old[k][0]) & lo_window(old[k][1]);
-              if (!(inside >> 31)) {
+              if (!lo_all_in_window<16>(&old[0][0])) {
                 // the contributions again, from the same operands (the
                 // empty asm keeps the compiler from holding the first
                 // products in registers across the atomics)
@@ -1627,7 +1653,7 @@ __global__ __launch_bounds__(256) void brick_cross_kernel(SliceGeom g, BrickGrid
                       1e-3f;
   int hit = 0;
   for (int r = 0; r < n_rot && !hit; ++r) {
-    const float4 q = ptest[r];
+    const float4 q = ptest[r].n;
     const float dist = q.x * cf[0] + q.y * cf[1] + q.z * cf[2];
     const float rad = fabsf(q.x) * hf[0] + fabsf(q.y) * hf[1] + fabsf(q.z) * hf[2];
     hit = (q.w < 0.0f) | (fabsf(dist) <= rad + tmarg);

@@ -47,8 +47,20 @@ __device__ __forceinline__ long long to_fixed64(float p) {
 // on the order of the atomics, and equals the 64-bit accumulation's.
 constexpr unsigned kLoBias = 0x80000000u;
 
-// bit 31 set: the low word was inside the window (no carry possible)
-__device__ __forceinline__ unsigned lo_window(unsigned old) { return old + 0x40000000u; }
+// Whether the N low words a lane's atomics returned all lay inside the
+// window [2^30, 3 2^30), so that none of its additions can have carried: the
+// smallest and the largest word are tested, and the compiler forms them with
+// three-input v_min3_u32 / v_max3_u32, about one VALU operation per word.
+template <int N>
+__device__ __forceinline__ bool lo_all_in_window(const unsigned *old) {
+  unsigned mn = old[0], mx = old[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    mn = min(mn, old[k]);
+    mx = max(mx, old[k]);
+  }
+  return mn >= 0x40000000u && mx < 0xC0000000u;
+}
 
 // Change of the high word when c is added to a low word that held old.
 __device__ __forceinline__ unsigned hi_step(unsigned old, int c) {

@@ -465,10 +465,7 @@ __global__ __launch_bounds__(kLineBlock) void line_adj_tile_kernel(const f2v *__
         old[t][0] = atomicAdd(&acc[0][e + o[t]], (unsigned)line_cvt_floor(__fmaf_rn(w[t], gs.x, 0.5f)));
         old[t][1] = atomicAdd(&acc[2][e + o[t]], (unsigned)line_cvt_floor(__fmaf_rn(w[t], gs.y, 0.5f)));
       }
-      unsigned inside = 0xffffffffu;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) inside &= lo_window(old[t][0]) & lo_window(old[t][1]);
-      if (!(inside >> 31)) {
+      if (!lo_all_in_window<8>(&old[0][0])) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const unsigned h0 = hi_step(old[t][0], line_cvt_floor(__fmaf_rn(w[t], gs.x, 0.5f)));
