@@ -1023,20 +1023,22 @@ __device__ __forceinline__ void brick_body(
     // left, so that chunk steps run with every lane busy except at the end.
     int rb = 0, cskip = 0;
     while (rb < total_rows) {
-      // item of each lane's row: the few items ending inside this pass are
-      // walked with wave-uniform reads of the prefix
+      // item k of each lane's row t (ia[k-1] <= t < ia[k]): kf items end at
+      // or before the pass's first row; each item ending inside the pass
+      // marks the slot of its end (the first row of the next item), and a
+      // running maximum over the wave carries the marks to the rows that
+      // follow (item ends are strictly increasing: every item has a row).
+      // The row's offset in its item comes from the previous item's end.
       const int t = rb + lane;
       const int kf = __popcll(__ballot(lane < nitems && ia <= rb));
-      int k = kf;
-      int excl = kf > 0 ? __builtin_amdgcn_readlane(ia, kf - 1) : 0;
-      for (int kk = kf; kk < nitems; ++kk) {
-        const int e = __builtin_amdgcn_readlane(ia, kk);
-        if (e > rb + 63) break;
-        if (e <= t) {
-          k = kk + 1;
-          excl = e;
-        }
-      }
+      marks[lane] = 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < nitems && ia > rb && ia < rb + 64) marks[ia - rb] = (unsigned)lane + 1u;
+      __builtin_amdgcn_wave_barrier();
+      const int k = max(kf, (int)wave_incl_max(marks[lane]));
+      const int kprev = __builtin_amdgcn_ds_bpermute(max(k - 1, 0) << 2, ia);
+      const int excl = k > 0 ? kprev : 0;
+      __builtin_amdgcn_wave_barrier();  // the chunk steps rewrite the marks
       // one lane per row: the candidate columns, from the item's FP32 frame
       int len = 0;
       if (t < total_rows) {
