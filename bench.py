@@ -581,18 +581,27 @@ def copy_bandwidth(device, mib=1024, reps=5):
     return 2 * (mib << 20) / ms / 1e6
 
 
+def _code_only(text):
+    """C++ source without its comments and with whitespace runs collapsed, so
+    that edits to comments alone leave the digest unchanged."""
+    import re
+
+    text = re.sub(r"//[^\n]*|/\*.*?\*/", " ", text, flags=re.S)
+    return " ".join(text.split())
+
+
 def kernel_source_digest():
-    """sha256 (first 16 hex digits) over the HIP sources the library is built
-    from.  profiles/traffic.json records the digest of the sources its
-    counters were measured on; its figures are reported only while the
-    sources are unchanged."""
+    """sha256 (first 16 hex digits) over the code of the HIP sources the
+    library is built from (comments and layout excluded).
+    profiles/traffic.json records the digest of the sources its counters were
+    measured on; its figures are reported only while that code is unchanged."""
     h = hashlib.sha256()
     d = os.path.join(ROOT, "teamtomo_amd", "csrc")
     for name in sorted(os.listdir(d)):
         if name.endswith((".hip", ".h")):
             h.update(name.encode())
-            with open(os.path.join(d, name), "rb") as f:
-                h.update(f.read())
+            with open(os.path.join(d, name), encoding="utf-8") as f:
+                h.update(_code_only(f.read()).encode())
     return h.hexdigest()[:16]
 
 
