@@ -138,7 +138,7 @@ int tt_extract_c64(const void *volume, int vol_d, int vol_h, int vol_wr,
  * bit for bit) for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops).  It has no counterpart in the
  * reference.  Its plane records and unit table live in a fixed file-scope
- * device array per GPU (about 2.8 MB, held until the process ends) instead of
+ * device array per GPU (about 3.0 MB, held until the process ends) instead of
  * the allocations tt_extract_c64 makes; the first call for a geometry plans
  * its units on the host (host memory only) and copies them in, stream-ordered.
  *   n_rot <= 16384, and the volume's brick units <= 65536 (256^3 needs
